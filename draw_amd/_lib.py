@@ -1,0 +1,88 @@
+"""ctypes binding of libdrawhip.so (include/drawhip.h).
+
+The library is built in-tree by ``python -m draw_amd.build``; importing this
+module without it raises ImportError -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdrawhip.so")
+
+DH_OK = 0
+ERRORS = {
+    0: "DH_OK", -1: "DH_E_ARG", -2: "DH_E_INDEX", -3: "DH_E_TEXCOORD", -4: "DH_E_CAMERA", -5: "DH_E_PLANE",
+    -6: "DH_E_HIP", -7: "DH_E_NOMEM", -8: "DH_E_HANDLE", -9: "DH_E_UNSUPPORTED", -10: "DH_E_BUDGET",
+    -11: "DH_E_CAPACITY",
+}
+
+
+class DhFrameParams(C.Structure):
+    _fields_ = [("M", C.c_float * 16), ("cam", C.c_float * 3), ("light", C.c_float * 3),
+                ("planes", (C.c_float * 4) * 6), ("W", C.c_int32), ("H", C.c_int32), ("off_x", C.c_float),
+                ("off_y", C.c_float), ("clear_depth", C.c_float), ("clear_bgra", C.c_uint8 * 4)]
+
+
+class DhCamera(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("dir", C.c_float * 3), ("fov_x_deg", C.c_float), ("near_", C.c_float),
+                ("far_", C.c_float)]
+
+
+class DhMaterialDesc(C.Structure):
+    _fields_ = [("ka", C.c_float * 3), ("kd", C.c_float * 3), ("ks", C.c_float * 3), ("alpha", C.c_float),
+                ("power", C.c_int32), ("map_kd", C.c_uint64), ("map_ka", C.c_uint64)]
+
+
+class DhStats(C.Structure):
+    _fields_ = [("tris_in", C.c_uint64), ("tris_out", C.c_uint64), ("bin_entries", C.c_uint64),
+                ("frags", C.c_uint64), ("retries", C.c_uint64), ("ms_geometry", C.c_float),
+                ("ms_binning", C.c_float), ("ms_raster", C.c_float), ("ms_shade", C.c_float),
+                ("ms_frame", C.c_float)]
+
+
+# Every symbol declared in include/drawhip.h, with its ctypes signature.
+_P = C.c_void_p
+_H = C.c_uint64
+SIGNATURES = {
+    "dh_frame_uniforms": [C.POINTER(DhCamera), C.c_int32, C.c_int32, C.POINTER(C.c_float), C.POINTER(DhFrameParams)],
+    "dh_ctx_create": [C.c_int32, C.POINTER(_P)],
+    "dh_ctx_destroy": [_P],
+    "dh_last_error": [_P],
+    "dh_set_tiling": [_P, C.c_int32, C.c_int32],
+    "dh_set_profiling": [_P, C.c_int32],
+    "dh_texture_create": [_P, _P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(_H)],
+    "dh_material_create": [_P, C.POINTER(DhMaterialDesc), C.POINTER(_H)],
+    "dh_object_create": [_P, _P, C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, C.POINTER(_H)],
+    "dh_mesh_create": [_P, _H, _P, C.c_uint32, _H, C.POINTER(_H)],
+    "dh_canvas_create": [_P, C.c_int32, C.c_int32, C.POINTER(_H)],
+    "dh_release": [_P, _H],
+    "dh_render": [_P, C.POINTER(DhFrameParams), C.POINTER(_H), C.c_uint32, _H],
+    "dh_canvas_read": [_P, _H, _P, _P, _P],
+    "dh_canvas_device_ptrs": [_P, _H, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)],
+    "dh_get_stats": [_P, C.POINTER(DhStats)],
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load libdrawhip.so (raises ImportError when it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m draw_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, args in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_char_p if name == "dh_last_error" else C.c_int
+        _lib = L
+    return _lib
+
+
+class DrawHipError(RuntimeError):
+    def __init__(self, code: int, what: str, detail: str = ""):
+        self.code = code
+        super().__init__(f"{what} failed: {ERRORS.get(code, code)}" + (f" ({detail})" if detail else ""))

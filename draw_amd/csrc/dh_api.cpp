@@ -1,0 +1,608 @@
+// dh_api.cpp -- the C ABI of libdrawhip: contexts, device-resident resources
+// (textures, materials, objects, meshes, canvases) and dh_render.
+//
+// A frame is enqueued without host synchronisation; the scratch buffers it
+// needs are sized from the previous frame.  When a device-side count exceeds a
+// capacity, the kernels stop writing past it and raise a flag, and dh_render
+// grows the buffer to the measured size and renders the frame again.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "dh_common.hpp"
+#include "dh_pipeline.hpp"
+
+namespace {
+
+enum Kind : uint32_t { kTex = 1, kMat = 2, kObj = 3, kMesh = 4, kCanvas = 5 };
+
+inline dh_handle make_handle(Kind k, uint64_t id) { return ((uint64_t)k << 56) | id; }
+inline Kind handle_kind(dh_handle h) { return (Kind)(h >> 56); }
+
+struct TexRes {
+  uint8_t* d = nullptr;
+  int32_t w = 0, h = 0, comp = 0;
+};
+struct MatRes {
+  uint32_t slot = 0;
+  float alpha = 1.0f;
+};
+struct ObjRes {
+  float* pos = nullptr;
+  float* nrm = nullptr;
+  float* uv = nullptr;
+  uint32_t n_pos = 0, n_nrm = 0, n_uv = 0;
+};
+struct MeshRes {
+  dh_handle obj = 0;
+  uint32_t* tri = nullptr;
+  uint32_t n_tri = 0;
+  dh_handle mat = 0;
+};
+struct CanvasRes {
+  int32_t W = 0, H = 0;
+  uint32_t* bgra = nullptr;
+  float* depth = nullptr;
+  uint32_t* winner = nullptr;
+};
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  hipError_t ensure(size_t want) {
+    if (want <= n && p) return hipSuccess;
+    if (p) hipFree(p);
+    p = nullptr;
+    n = 0;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), std::max<size_t>(want, 1) * sizeof(T));
+    if (e == hipSuccess) n = std::max<size_t>(want, 1);
+    return e;
+  }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+}  // namespace
+
+struct dh_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  uint64_t next_id = 1;
+  std::unordered_map<uint64_t, TexRes> tex;
+  std::unordered_map<uint64_t, MatRes> mats;
+  std::unordered_map<uint64_t, ObjRes> objs;
+  std::unordered_map<uint64_t, MeshRes> meshes;
+  std::unordered_map<uint64_t, CanvasRes> canvases;
+  // material table (slot 0 = Texture::default)
+  std::vector<dh::DevMaterial> mat_host;
+  DevBuf<dh::DevMaterial> mat_dev;
+  bool mat_dirty = true;
+  uint8_t* white_rgb = nullptr;  // TextureMap::default
+  // raster configuration
+  int TS = 16, waves = 4;
+  bool profiling = false;
+  hipEvent_t ev[5] = {};
+  // scratch
+  DevBuf<uint8_t> cnt;
+  DevBuf<uint32_t> blk;
+  DevBuf<dh::SetupRec> setup;
+  DevBuf<dh::ShadeRec> shade;
+  DevBuf<uint32_t> bincnt, binoff;
+  DevBuf<uint32_t> keys0, keys1, vals0, vals1;
+  DevBuf<uint32_t> hist, partials;
+  DevBuf<uint2> ranges;
+  DevBuf<uint32_t> vis;
+  DevBuf<dh::Status> status;
+  DevBuf<dh::DevDraw> draws;
+  DevBuf<dh::DevView> views;
+  uint32_t cap_tri = 0, cap_pairs = 0;
+  // pinned staging
+  dh::Status* h_status = nullptr;
+  std::vector<dh::DevDraw> h_draws;
+  std::vector<dh::DevView> h_views;
+  dh_stats stats = {};
+};
+
+namespace {
+
+int fail(dh_ctx c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int hip_fail(dh_ctx c, hipError_t e, const char* what) {
+  return fail(c, e == hipErrorOutOfMemory ? DH_E_NOMEM : DH_E_HIP,
+              std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPCHK(c, expr)                                  \
+  do {                                                   \
+    hipError_t _e = (expr);                              \
+    if (_e != hipSuccess) return hip_fail((c), _e, #expr); \
+  } while (0)
+
+int key_bits_for(uint32_t n) {
+  int b = 1;
+  while (b < 32 && (1ull << b) < (uint64_t)n) b++;
+  return b;
+}
+
+int upload_materials(dh_ctx c) {
+  if (!c->mat_dirty) return DH_OK;
+  HIPCHK(c, c->mat_dev.ensure(c->mat_host.size()));
+  HIPCHK(c, hipMemcpyAsync(c->mat_dev.p, c->mat_host.data(), c->mat_host.size() * sizeof(dh::DevMaterial),
+                           hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->mat_dirty = false;
+  return DH_OK;
+}
+
+dh::DevMaterial default_material(const uint8_t* white) {
+  dh::DevMaterial m;
+  memset(&m, 0, sizeof(m));
+  for (int i = 0; i < 3; i++) {  // Texture::default, scene/mod.rs:237-252
+    m.ka[i] = 0.9f;
+    m.kd[i] = 0.4f;
+    m.ks[i] = 0.5f;
+  }
+  m.alpha = 1.0f;
+  m.power = 2;
+  m.kd_img = white;
+  m.ka_img = white;
+  m.kd_w = m.kd_h = m.ka_w = m.ka_h = 1;
+  m.kd_comp = m.ka_comp = 3;
+  return m;
+}
+
+// Grow every scratch buffer to the given capacities.
+int ensure_scratch(dh_ctx c, uint32_t n_in, uint32_t cap_tri, uint32_t cap_pairs, uint32_t n_tiles,
+                   uint32_t n_pixels, uint32_t n_draws, uint32_t n_views) {
+  HIPCHK(c, c->cnt.ensure(n_in));
+  HIPCHK(c, c->blk.ensure((n_in + 255) / 256 + 1));
+  HIPCHK(c, c->setup.ensure(cap_tri));
+  HIPCHK(c, c->shade.ensure(cap_tri));
+  HIPCHK(c, c->bincnt.ensure(cap_tri));
+  HIPCHK(c, c->binoff.ensure(cap_tri));
+  HIPCHK(c, c->keys0.ensure(cap_pairs));
+  HIPCHK(c, c->keys1.ensure(cap_pairs));
+  HIPCHK(c, c->vals0.ensure(cap_pairs));
+  HIPCHK(c, c->vals1.ensure(cap_pairs));
+  HIPCHK(c, c->hist.ensure(256 * 512));
+  HIPCHK(c, c->partials.ensure(1024));
+  HIPCHK(c, c->ranges.ensure(n_tiles));
+  HIPCHK(c, c->vis.ensure(n_pixels));
+  HIPCHK(c, c->status.ensure(1));
+  HIPCHK(c, c->draws.ensure(n_draws));
+  HIPCHK(c, c->views.ensure(n_views));
+  c->cap_tri = (uint32_t)std::min<size_t>(c->setup.n, 0xFFFFFFF0u);
+  c->cap_pairs = (uint32_t)std::min<size_t>(c->keys0.n, 0xFFFFFFF0u);
+  return DH_OK;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ context
+extern "C" int dh_ctx_create(int32_t device, dh_ctx* out) {
+  if (!out) return DH_E_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return DH_E_UNSUPPORTED;
+  if (device < 0 || device >= n) return DH_E_ARG;
+  dh_ctx c = new dh_ctx_s();
+  c->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_status), sizeof(dh::Status));
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->white_rgb), 4);
+  if (e == hipSuccess) {
+    const uint8_t w[4] = {255, 255, 255, 255};  // Color::White.as_slice(), scene/mod.rs:130
+    e = hipMemcpy(c->white_rgb, w, 4, hipMemcpyHostToDevice);
+  }
+  for (int i = 0; i < 5 && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
+  if (e != hipSuccess) {
+    dh_ctx_destroy(c);
+    return DH_E_HIP;
+  }
+  c->mat_host.push_back(default_material(c->white_rgb));
+  *out = c;
+  return DH_OK;
+}
+
+extern "C" int dh_ctx_destroy(dh_ctx c) {
+  if (!c) return DH_E_ARG;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  for (auto& kv : c->tex) hipFree(kv.second.d);
+  for (auto& kv : c->objs) {
+    hipFree(kv.second.pos);
+    hipFree(kv.second.nrm);
+    hipFree(kv.second.uv);
+  }
+  for (auto& kv : c->meshes) hipFree(kv.second.tri);
+  for (auto& kv : c->canvases) {
+    hipFree(kv.second.bgra);
+    hipFree(kv.second.depth);
+    hipFree(kv.second.winner);
+  }
+  c->mat_dev.release();
+  c->cnt.release(); c->blk.release(); c->setup.release(); c->shade.release();
+  c->bincnt.release(); c->binoff.release();
+  c->keys0.release(); c->keys1.release(); c->vals0.release(); c->vals1.release();
+  c->hist.release(); c->partials.release(); c->ranges.release(); c->vis.release();
+  c->status.release(); c->draws.release(); c->views.release();
+  if (c->white_rgb) hipFree(c->white_rgb);
+  if (c->h_status) hipHostFree(c->h_status);
+  for (int i = 0; i < 5; i++)
+    if (c->ev[i]) hipEventDestroy(c->ev[i]);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return DH_OK;
+}
+
+extern "C" const char* dh_last_error(dh_ctx c) { return c ? c->err.c_str() : "null context"; }
+
+extern "C" int dh_set_tiling(dh_ctx c, int32_t tile, int32_t waves) {
+  if (!c) return DH_E_ARG;
+  if (!dh::raster_config_supported(tile, waves)) return fail(c, DH_E_ARG, "unsupported tile/waves");
+  c->TS = tile;
+  c->waves = waves;
+  return DH_OK;
+}
+
+extern "C" int dh_set_profiling(dh_ctx c, int32_t enable) {
+  if (!c) return DH_E_ARG;
+  c->profiling = enable != 0;
+  return DH_OK;
+}
+
+// ---------------------------------------------------------------- resources
+extern "C" int dh_texture_create(dh_ctx c, const uint8_t* img, int32_t w, int32_t h, int32_t comp, dh_handle* out) {
+  if (!c || !img || !out || w <= 0 || h <= 0 || (comp != 3 && comp != 4)) return DH_E_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  TexRes t;
+  t.w = w;
+  t.h = h;
+  t.comp = comp;
+  const size_t bytes = (size_t)w * (size_t)h * (size_t)comp;
+  HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&t.d), bytes + 4));  // +4: 32-bit texel loads
+  HIPCHK(c, hipMemcpy(t.d, img, bytes, hipMemcpyHostToDevice));
+  const uint64_t id = c->next_id++;
+  c->tex[id] = t;
+  *out = make_handle(kTex, id);
+  return DH_OK;
+}
+
+extern "C" int dh_material_create(dh_ctx c, const dh_material_desc* d, dh_handle* out) {
+  if (!c || !d || !out || d->power < 0) return DH_E_ARG;
+  dh::DevMaterial m = default_material(c->white_rgb);
+  for (int i = 0; i < 3; i++) {
+    m.ka[i] = d->ka[i];
+    m.kd[i] = d->kd[i];
+    m.ks[i] = d->ks[i];
+  }
+  m.alpha = d->alpha;
+  m.power = d->power;
+  const dh_handle maps[2] = {d->map_kd, d->map_ka};
+  for (int k = 0; k < 2; k++) {
+    if (!maps[k]) continue;
+    if (handle_kind(maps[k]) != kTex) return fail(c, DH_E_HANDLE, "material map is not a texture");
+    auto it = c->tex.find(maps[k] & ((1ull << 56) - 1));
+    if (it == c->tex.end()) return fail(c, DH_E_HANDLE, "unknown texture");
+    if (k == 0) {
+      m.kd_img = it->second.d; m.kd_w = it->second.w; m.kd_h = it->second.h; m.kd_comp = it->second.comp;
+    } else {
+      m.ka_img = it->second.d; m.ka_w = it->second.w; m.ka_h = it->second.h; m.ka_comp = it->second.comp;
+    }
+  }
+  MatRes r;
+  r.slot = (uint32_t)c->mat_host.size();
+  r.alpha = d->alpha;
+  c->mat_host.push_back(m);
+  c->mat_dirty = true;
+  const uint64_t id = c->next_id++;
+  c->mats[id] = r;
+  *out = make_handle(kMat, id);
+  return DH_OK;
+}
+
+extern "C" int dh_object_create(dh_ctx c, const float* pos, uint32_t n_pos, const float* nrm, uint32_t n_nrm,
+                                const float* uv, uint32_t n_uv, dh_handle* out) {
+  if (!c || !out || (n_pos && !pos) || (n_nrm && !nrm) || (n_uv && !uv)) return DH_E_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  ObjRes o;
+  o.n_pos = n_pos;
+  o.n_nrm = n_nrm;
+  o.n_uv = n_uv;
+  HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&o.pos), std::max<size_t>(1, 3ull * n_pos) * 4));
+  HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&o.nrm), std::max<size_t>(1, 3ull * n_nrm) * 4));
+  HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&o.uv), std::max<size_t>(1, 2ull * n_uv) * 4));
+  if (n_pos) HIPCHK(c, hipMemcpy(o.pos, pos, 12ull * n_pos, hipMemcpyHostToDevice));
+  if (n_nrm) HIPCHK(c, hipMemcpy(o.nrm, nrm, 12ull * n_nrm, hipMemcpyHostToDevice));
+  if (n_uv) HIPCHK(c, hipMemcpy(o.uv, uv, 8ull * n_uv, hipMemcpyHostToDevice));
+  const uint64_t id = c->next_id++;
+  c->objs[id] = o;
+  *out = make_handle(kObj, id);
+  return DH_OK;
+}
+
+extern "C" int dh_mesh_create(dh_ctx c, dh_handle object, const uint32_t* tri, uint32_t n_tri, dh_handle material,
+                              dh_handle* out) {
+  if (!c || !out || (n_tri && !tri)) return DH_E_ARG;
+  if (handle_kind(object) != kObj) return fail(c, DH_E_HANDLE, "not an object handle");
+  auto ot = c->objs.find(object & ((1ull << 56) - 1));
+  if (ot == c->objs.end()) return fail(c, DH_E_HANDLE, "unknown object");
+  if (material && (handle_kind(material) != kMat || !c->mats.count(material & ((1ull << 56) - 1))))
+    return fail(c, DH_E_HANDLE, "unknown material");
+  const ObjRes& o = ot->second;
+  // IndexedMesh::vec3_list_from_indexed asserts (scene/mesh.rs:46-48)
+  for (uint64_t t = 0; t < n_tri; t++) {
+    const uint32_t* q = tri + 9 * t;
+    for (int k = 0; k < 3; k++) {
+      if (q[k] >= o.n_pos || q[3 + k] >= o.n_uv || q[6 + k] >= o.n_nrm) {
+        char buf[128];
+        snprintf(buf, sizeof buf, "triangle %llu: vertex index out of range", (unsigned long long)t);
+        return fail(c, DH_E_INDEX, buf);
+      }
+    }
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  MeshRes m;
+  m.obj = object;
+  m.n_tri = n_tri;
+  m.mat = material;
+  HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&m.tri), std::max<size_t>(1, 9ull * n_tri) * 4));
+  if (n_tri) HIPCHK(c, hipMemcpy(m.tri, tri, 36ull * n_tri, hipMemcpyHostToDevice));
+  const uint64_t id = c->next_id++;
+  c->meshes[id] = m;
+  *out = make_handle(kMesh, id);
+  return DH_OK;
+}
+
+extern "C" int dh_canvas_create(dh_ctx c, int32_t W, int32_t H, dh_handle* out) {
+  if (!c || !out || W <= 0 || H <= 0 || W > 65535 || H > 65535) return DH_E_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  CanvasRes cv;
+  cv.W = W;
+  cv.H = H;
+  const size_t n = (size_t)W * (size_t)H;
+  HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&cv.bgra), n * 4));
+  HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&cv.depth), n * 4));
+  HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&cv.winner), n * 4));
+  const uint64_t id = c->next_id++;
+  c->canvases[id] = cv;
+  *out = make_handle(kCanvas, id);
+  return DH_OK;
+}
+
+extern "C" int dh_release(dh_ctx c, dh_handle h) {
+  if (!c) return DH_E_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  const uint64_t id = h & ((1ull << 56) - 1);
+  switch (handle_kind(h)) {
+    case kTex: {
+      auto it = c->tex.find(id);
+      if (it == c->tex.end()) break;
+      hipFree(it->second.d);
+      c->tex.erase(it);
+      return DH_OK;
+    }
+    case kMat: {
+      auto it = c->mats.find(id);
+      if (it == c->mats.end()) break;
+      c->mats.erase(it);  // the slot stays in the table
+      return DH_OK;
+    }
+    case kObj: {
+      auto it = c->objs.find(id);
+      if (it == c->objs.end()) break;
+      hipFree(it->second.pos);
+      hipFree(it->second.nrm);
+      hipFree(it->second.uv);
+      c->objs.erase(it);
+      return DH_OK;
+    }
+    case kMesh: {
+      auto it = c->meshes.find(id);
+      if (it == c->meshes.end()) break;
+      hipFree(it->second.tri);
+      c->meshes.erase(it);
+      return DH_OK;
+    }
+    case kCanvas: {
+      auto it = c->canvases.find(id);
+      if (it == c->canvases.end()) break;
+      hipFree(it->second.bgra);
+      hipFree(it->second.depth);
+      hipFree(it->second.winner);
+      c->canvases.erase(it);
+      return DH_OK;
+    }
+  }
+  return fail(c, DH_E_HANDLE, "unknown handle");
+}
+
+// ---------------------------------------------------------------- rendering
+extern "C" int dh_render(dh_ctx c, const dh_frame_params* fr, const dh_handle* meshes, uint32_t n_meshes,
+                         dh_handle canvas) {
+  if (!c || !fr || (n_meshes && !meshes)) return DH_E_ARG;
+  if (fr->W <= 0 || fr->H <= 0) return fail(c, DH_E_ARG, "bad frame size");
+  if (handle_kind(canvas) != kCanvas) return fail(c, DH_E_HANDLE, "not a canvas handle");
+  auto ct = c->canvases.find(canvas & ((1ull << 56) - 1));
+  if (ct == c->canvases.end()) return fail(c, DH_E_HANDLE, "unknown canvas");
+  const CanvasRes& cv = ct->second;
+  if (cv.W != fr->W || cv.H != fr->H) return fail(c, DH_E_ARG, "canvas size differs from frame size");
+  HIPCHK(c, hipSetDevice(c->device));
+
+  // draw list in submission order (Scene::render's object/mesh loops)
+  c->h_draws.clear();
+  uint64_t n_in = 0;
+  for (uint32_t i = 0; i < n_meshes; i++) {
+    if (handle_kind(meshes[i]) != kMesh) return fail(c, DH_E_HANDLE, "not a mesh handle");
+    auto mt = c->meshes.find(meshes[i] & ((1ull << 56) - 1));
+    if (mt == c->meshes.end()) return fail(c, DH_E_HANDLE, "unknown mesh");
+    const MeshRes& m = mt->second;
+    auto ot = c->objs.find(m.obj & ((1ull << 56) - 1));
+    if (ot == c->objs.end()) return fail(c, DH_E_HANDLE, "mesh object was released");
+    uint32_t slot = 0;
+    float alpha = 1.0f;
+    if (m.mat) {
+      auto it = c->mats.find(m.mat & ((1ull << 56) - 1));
+      if (it == c->mats.end()) return fail(c, DH_E_HANDLE, "mesh material was released");
+      slot = it->second.slot;
+      alpha = it->second.alpha;
+    }
+    if (alpha < 1.0f) return fail(c, DH_E_UNSUPPORTED, "transparent meshes (alpha < 1) are not rendered by dh_render");
+    if (m.n_tri == 0) continue;
+    dh::DevDraw d;
+    d.pos = ot->second.pos;
+    d.nrm = ot->second.nrm;
+    d.uv = ot->second.uv;
+    d.tri = m.tri;
+    d.n_tri = m.n_tri;
+    d.tri_begin = (uint32_t)n_in;
+    d.material = slot;
+    d.view = 0;
+    c->h_draws.push_back(d);
+    n_in += m.n_tri;
+  }
+  if (n_in > 0x7FFFFFFFull) return fail(c, DH_E_ARG, "too many triangles in one frame");
+  if (c->h_draws.empty()) {  // keep the draw table non-empty
+    dh::DevDraw d;
+    memset(&d, 0, sizeof(d));
+    c->h_draws.push_back(d);
+  }
+  int rc = upload_materials(c);
+  if (rc) return rc;
+
+  const int TS = c->TS;
+  dh::DevView v;
+  memset(&v, 0, sizeof(v));
+  v.fr = *fr;
+  v.ntx = (uint32_t)((fr->W + TS - 1) / TS);
+  v.nty = (uint32_t)((fr->H + TS - 1) / TS);
+  v.tile_base = 0;
+  v.px_base = 0;
+  v.out_bgra = cv.bgra;
+  v.out_depth = cv.depth;
+  v.out_winner = cv.winner;
+  c->h_views.assign(1, v);
+  const uint32_t n_tiles = v.ntx * v.nty;
+  const uint32_t n_pixels = (uint32_t)fr->W * (uint32_t)fr->H;
+
+  uint64_t want_tri = std::max<uint64_t>(c->cap_tri, n_in + n_in / 4 + 1024);
+  uint64_t want_pairs = std::max<uint64_t>(c->cap_pairs, 4 * want_tri + n_tiles);
+  c->stats = dh_stats{};
+  c->stats.tris_in = n_in;
+  for (int attempt = 0; attempt < 6; attempt++) {
+    want_tri = std::min<uint64_t>(want_tri, 0xFFFFFFF0ull);
+    want_pairs = std::min<uint64_t>(want_pairs, 0xFFFFFFF0ull);
+    rc = ensure_scratch(c, (uint32_t)n_in, (uint32_t)want_tri, (uint32_t)want_pairs, n_tiles, n_pixels,
+                        (uint32_t)c->h_draws.size(), 1);
+    if (rc) return rc;
+    HIPCHK(c, hipMemcpyAsync(c->draws.p, c->h_draws.data(), c->h_draws.size() * sizeof(dh::DevDraw),
+                             hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->views.p, c->h_views.data(), sizeof(dh::DevView), hipMemcpyHostToDevice,
+                             c->stream));
+    dh::PipeArgs a;
+    memset(&a, 0, sizeof(a));
+    a.draws = c->draws.p;
+    a.n_draws = (uint32_t)c->h_draws.size();
+    a.n_tri_in = (uint32_t)n_in;
+    a.views = c->views.p;
+    a.n_views = 1;
+    a.n_tiles = n_tiles;
+    a.n_pixels = n_pixels;
+    a.key_bits = key_bits_for(n_tiles);
+    a.mats = c->mat_dev.p;
+    a.TS = TS;
+    a.waves = c->waves;
+    a.cnt = c->cnt.p;
+    a.blk = c->blk.p;
+    a.setup = c->setup.p;
+    a.shade = c->shade.p;
+    a.cap_tri = c->cap_tri;
+    a.bincnt = c->bincnt.p;
+    a.binoff = c->binoff.p;
+    a.keys[0] = c->keys0.p;
+    a.keys[1] = c->keys1.p;
+    a.vals[0] = c->vals0.p;
+    a.vals[1] = c->vals1.p;
+    a.cap_pairs = c->cap_pairs;
+    a.hist = c->hist.p;
+    a.partials = c->partials.p;
+    a.ranges = c->ranges.p;
+    a.vis = c->vis.p;
+    a.status = c->status.p;
+    for (int i = 0; i < 5; i++) a.ev[i] = c->profiling ? c->ev[i] : nullptr;
+    dh::enqueue_frame(a, c->stream);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(c->h_status, c->status.p, sizeof(dh::Status), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const dh::Status st = *c->h_status;
+    c->stats.tris_out = st.n_out;
+    c->stats.bin_entries = st.n_pairs;
+    c->stats.frags = st.frags;
+    if (st.overflow) {
+      c->stats.retries++;
+      if (st.overflow & 1u) want_tri = (uint64_t)st.n_out + st.n_out / 8 + 1024;
+      if (st.overflow & 2u) want_pairs = (uint64_t)st.n_pairs + st.n_pairs / 8 + 4096;
+      // bin entries are only exact once the triangle records fit
+      if (st.overflow & 1u) want_pairs = std::max<uint64_t>(want_pairs, 4 * want_tri + n_tiles);
+      continue;
+    }
+    if (c->profiling) {
+      float ms[4];
+      for (int i = 0; i < 4; i++) hipEventElapsedTime(&ms[i], c->ev[i], c->ev[i + 1]);
+      c->stats.ms_geometry = ms[0];
+      c->stats.ms_binning = ms[1];
+      c->stats.ms_raster = ms[2];
+      c->stats.ms_shade = ms[3];
+      hipEventElapsedTime(&c->stats.ms_frame, c->ev[0], c->ev[4]);
+    }
+    if (st.error & 1u) return fail(c, DH_E_TEXCOORD, "texture coordinate out of range (get_rgb_slice would panic)");
+    return DH_OK;
+  }
+  return fail(c, DH_E_CAPACITY, "scratch capacity did not converge");
+}
+
+extern "C" int dh_canvas_read(dh_ctx c, dh_handle canvas, uint8_t* bgra, float* depth, uint32_t* winner) {
+  if (!c) return DH_E_ARG;
+  if (handle_kind(canvas) != kCanvas) return fail(c, DH_E_HANDLE, "not a canvas handle");
+  auto ct = c->canvases.find(canvas & ((1ull << 56) - 1));
+  if (ct == c->canvases.end()) return fail(c, DH_E_HANDLE, "unknown canvas");
+  const CanvasRes& cv = ct->second;
+  const size_t n = (size_t)cv.W * (size_t)cv.H;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (bgra) HIPCHK(c, hipMemcpyAsync(bgra, cv.bgra, n * 4, hipMemcpyDeviceToHost, c->stream));
+  if (depth) HIPCHK(c, hipMemcpyAsync(depth, cv.depth, n * 4, hipMemcpyDeviceToHost, c->stream));
+  if (winner) HIPCHK(c, hipMemcpyAsync(winner, cv.winner, n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return DH_OK;
+}
+
+extern "C" int dh_canvas_device_ptrs(dh_ctx c, dh_handle canvas, void** bgra, void** depth, void** winner) {
+  if (!c) return DH_E_ARG;
+  auto ct = c->canvases.find(canvas & ((1ull << 56) - 1));
+  if (handle_kind(canvas) != kCanvas || ct == c->canvases.end()) return fail(c, DH_E_HANDLE, "unknown canvas");
+  if (bgra) *bgra = ct->second.bgra;
+  if (depth) *depth = ct->second.depth;
+  if (winner) *winner = ct->second.winner;
+  return DH_OK;
+}
+
+extern "C" int dh_get_stats(dh_ctx c, dh_stats* out) {
+  if (!c || !out) return DH_E_ARG;
+  *out = c->stats;
+  return DH_OK;
+}

@@ -1,0 +1,989 @@
+// dh_kernels.hip -- gfx950 kernels of the per-frame raster path.
+//
+// Pipeline (one stream, no host synchronisation inside a frame):
+//   1 k_geom_count   per input triangle: back-face cull, frustum clip count
+//                    (scene/mod.rs:1016-1033, 43-90, 641-746), positions only
+//   2 scan           block counts -> ordered output offsets (submission order)
+//   3 k_geom_emit    per input triangle: per-vertex lighting (917-926), clip
+//                    with attributes, projection (1047-1063), raster setup
+//                    (canvas.rs:585-666) -> SetupRec + ShadeRec at its
+//                    compacted index
+//   4 k_bin_count    per clipped triangle: conservative tile overlap count
+//   5 scan           -> bin entry offsets
+//   6 k_bin_emit     (tile, triangle) pairs in triangle order
+//   7 radix sort     stable LSD sort of the pairs by tile -> per-tile lists
+//                    that keep the submission order
+//   8 k_tile_ranges  [start, end) of every tile's list
+//   9 k_raster       one workgroup per screen tile: coverage and the strict
+//                    depth test (canvas.rs:668-682, 923) into an LDS visibility
+//                    buffer of (depth key << 32 | triangle id)
+//  10 k_shade        deferred Phong + texture shading of each pixel's winner
+//                    (canvas.rs:684-745), BGRA8 + f32 depth
+#include <math.h>
+#include <stdio.h>
+
+#include <algorithm>
+
+#include "dh_common.hpp"
+#include "dh_pipeline.hpp"
+
+namespace dh {
+
+// ===================================================================== common
+__device__ inline uint32_t find_draw(const DevDraw* draws, uint32_t n, uint32_t t) {
+  uint32_t lo = 0, hi = n;  // last draw with tri_begin <= t
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (draws[mid].tri_begin <= t) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+__device__ inline uint32_t find_view_tile(const DevView* views, uint32_t n, uint32_t t) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (views[mid].tile_base <= t) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+__device__ inline uint32_t find_view_px(const DevView* views, uint32_t n, uint32_t p) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (views[mid].px_base <= p) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+__device__ inline V3 ld3(const float* a, uint32_t i) {
+  const float* p = a + 3ull * i;
+  return v3(p[0], p[1], p[2]);
+}
+
+// Block of 256 threads (4 waves): exclusive scan of one value per thread.
+__device__ inline uint32_t block_exclusive_scan256(uint32_t v, uint32_t* s_tmp /*[4]*/, uint32_t* total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_tmp[wave] = x;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+#pragma unroll
+  for (int w = 0; w < 4; w++) {
+    uint32_t t = s_tmp[w];
+    if (w < wave) before += t;
+    all += t;
+  }
+  __syncthreads();
+  *total = all;
+  return before + x - v;
+}
+
+// XCD-aware block order: workgroups are dispatched round-robin over the 8
+// XCDs, so give each XCD one contiguous run of logical indices (neighbouring
+// tiles share triangles and their records then share one L2).
+__device__ inline uint32_t xcd_remap(uint32_t b, uint32_t nb) {
+  const uint32_t xcd = b & 7u, k = b >> 3;
+  const uint32_t per = nb >> 3, rem = nb & 7u;
+  return (xcd < rem) ? xcd * (per + 1u) + k : rem * (per + 1u) + (xcd - rem) * per + k;
+}
+
+// =================================================================== clipping
+// Clip vertex with positions only (for counting) and with all attributes.
+struct PV {
+  V3 p;
+};
+struct FV {
+  V3 p;
+  float d;
+  V3 n, l, h;
+  float u, v;
+};
+
+// VertexAttributes a + (c - a) * t, canvas.rs:242-291 (screen_coord is still
+// (0,0) at this stage and is overwritten by the projection).
+__device__ inline PV lerpv(const PV& a, const PV& c, float t) {
+  PV r;
+  r.p = add(a.p, muls(sub(c.p, a.p), t));
+  return r;
+}
+__device__ inline FV lerpv(const FV& a, const FV& c, float t) {
+  FV r;
+  r.p = add(a.p, muls(sub(c.p, a.p), t));
+  r.d = a.d + ((c.d - a.d) * t);
+  r.n = add(a.n, muls(sub(c.n, a.n), t));
+  r.l = add(a.l, muls(sub(c.l, a.l), t));
+  r.h = add(a.h, muls(sub(c.h, a.h), t));
+  r.u = a.u + ((c.u - a.u) * t);
+  r.v = a.v + ((c.v - a.v) * t);
+  return r;
+}
+
+// Member-wise select (keeps the clip vertices in registers).
+__device__ inline V3 sel3(bool q, V3 x, V3 y) { return v3(q ? x.x : y.x, q ? x.y : y.y, q ? x.z : y.z); }
+__device__ inline PV selv(bool q, const PV& x, const PV& y) {
+  PV r;
+  r.p = sel3(q, x.p, y.p);
+  return r;
+}
+__device__ inline FV selv(bool q, const FV& x, const FV& y) {
+  FV r;
+  r.p = sel3(q, x.p, y.p);
+  r.d = q ? x.d : y.d;
+  r.n = sel3(q, x.n, y.n);
+  r.l = sel3(q, x.l, y.l);
+  r.h = sel3(q, x.h, y.h);
+  r.u = q ? x.u : y.u;
+  r.v = q ? x.v : y.v;
+  return r;
+}
+
+__device__ inline float plane_f(const float* pl, V3 p) { return dot(v3(pl[0], pl[1], pl[2]), p) + pl[3]; }
+
+// ViewPlane::at_least_partially_visible, scene/mod.rs:641-660.
+__device__ inline bool part_visible(const float* pl, V3 a, V3 b, V3 c) {
+  float fa = plane_f(pl, a), fb = plane_f(pl, b), fc = plane_f(pl, c);
+  return !(fa <= 0.0f && fb <= 0.0f && fc <= 0.0f);
+}
+
+// ViewPlane::clip, scene/mod.rs:662-746.  First output (o0,o1,o2), second
+// output (p0,p1,p2); returns the count.
+template <class V>
+__device__ inline int clip_plane(const float* pl, V a, V b, V c, V& o0, V& o1, V& o2, V& p0, V& p1, V& p2) {
+  float fa = plane_f(pl, a.p), fb = plane_f(pl, b.p), fc = plane_f(pl, c.p);
+  if (fa > 0.0f && fb > 0.0f && fc > 0.0f) {
+    o0 = a; o1 = b; o2 = c;
+    return 1;
+  }
+  if (fa <= 0.0f && fb <= 0.0f && fc <= 0.0f) return 0;
+  // rotate so that c is the vertex alone on its side: r1 (a,b,c) <- (c,a,b),
+  // r2 (a,b,c) <- (b,c,a)
+  const bool r1 = fa * fc >= 0.0f;
+  const bool r2 = !r1 && (fb * fc >= 0.0f);
+  const V A = selv(r1, c, selv(r2, b, a));
+  const V B = selv(r1, a, selv(r2, c, b));
+  const V C = selv(r1, b, selv(r2, a, c));
+  fc = r1 ? fb : (r2 ? fa : fc);
+  a = A;
+  b = B;
+  c = C;
+  const V3 n = v3(pl[0], pl[1], pl[2]);
+  const float ta = plane_f(pl, a.p) / dot(n, sub(a.p, c.p)) - kEps;
+  const V na = lerpv(a, c, ta);
+  const float tb = plane_f(pl, b.p) / dot(n, sub(b.p, c.p)) - kEps;
+  const V nb = lerpv(b, c, tb);
+  // one vertex outside (f_c <= 0): [a, A', B'] and [a, b, B']; else [c, A', B']
+  const bool two = fc <= 0.0f;
+  o0 = selv(two, a, c);
+  o1 = na;
+  o2 = nb;
+  p0 = a;
+  p1 = b;
+  p2 = nb;
+  return two ? 2 : 1;
+}
+
+// Back-face test (scene/mod.rs:1016-1020) and Triangle::clip_against_planes
+// (43-90): lateral trivial reject, then near and far full clipping.  Calls
+// emit(k, v0, v1, v2) for each output in order; returns the count.
+template <class V, class Emit>
+__device__ inline int cull_and_clip(const FrameParamsPod& fr, const V& a, const V& b, const V& c, Emit emit) {
+  const V3 cam = v3(fr.cam[0], fr.cam[1], fr.cam[2]);
+  const V3 nrm = cross(sub(b.p, a.p), sub(c.p, b.p));                             // Triangle::calc_normal
+  const V3 ctr = divs(add(add(add(v3(0.0f, 0.0f, 0.0f), a.p), b.p), c.p), 3.0f);  // get_center
+  if (dot(sub(cam, ctr), nrm) <= 0.0f) return 0;
+  if (!(part_visible(fr.planes[2], a.p, b.p, c.p) && part_visible(fr.planes[3], a.p, b.p, c.p) &&
+        part_visible(fr.planes[4], a.p, b.p, c.p) && part_visible(fr.planes[5], a.p, b.p, c.p)))
+    return 0;
+  V n0, n1, n2, m0, m1, m2;
+  const int nn = clip_plane(fr.planes[0], a, b, c, n0, n1, n2, m0, m1, m2);
+  int cnt = 0;
+  if (nn >= 1) {
+    V f0, f1, f2, g0, g1, g2;
+    const int nf = clip_plane(fr.planes[1], n0, n1, n2, f0, f1, f2, g0, g1, g2);
+    if (nf >= 1) emit(cnt++, f0, f1, f2);
+    if (nf >= 2) emit(cnt++, g0, g1, g2);
+  }
+  if (nn >= 2) {
+    V f0, f1, f2, g0, g1, g2;
+    const int nf = clip_plane(fr.planes[1], m0, m1, m2, f0, f1, f2, g0, g1, g2);
+    if (nf >= 1) emit(cnt++, f0, f1, f2);
+    if (nf >= 2) emit(cnt++, g0, g1, g2);
+  }
+  return cnt;
+}
+
+// ====================================================================== setup
+// Rectangle::from_coords + Rectangle::clip along one axis (canvas.rs:315-350,
+// 646-658).
+__device__ inline void rect_axis(uint64_t a0, uint64_t a1, uint64_t smax, uint32_t* lo, uint32_t* hi) {
+  uint64_t mn = a0 < a1 ? a0 : a1, mx = a0 < a1 ? a1 : a0;
+  mx = mx < smax ? mx : smax;
+  if (mn > mx) { mn = 0; mx = 0; }
+  *lo = (uint32_t)mn;
+  *hi = (uint32_t)mx;
+}
+
+// Projection (scene/mod.rs:1047-1058: Matrix4 * Vec4 then as_vec2()/w),
+// snapping (canvas.rs:585-587, 896-904), edge constants, normalizers,
+// tie-break products (660-666) and the scan rectangle (640-658).  Writes the
+// SetupRec and ShadeRec of one clipped triangle with 16-byte stores.
+__device__ inline void make_setup(const FrameParamsPod& fr, const FV& A, const FV& B, const FV& C, uint32_t material,
+                                  uint32_t view, SetupRec* __restrict__ rout, ShadeRec* __restrict__ sout) {
+  const float* M = fr.M;
+  float X[3], Y[3];
+  const V3 P[3] = {A.p, B.p, C.p};
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    float o0 = 0.0f + M[0] * P[k].x;
+    o0 = o0 + M[1] * P[k].y;
+    o0 = o0 + M[2] * P[k].z;
+    o0 = o0 + M[3] * 1.0f;
+    float o1 = 0.0f + M[4] * P[k].x;
+    o1 = o1 + M[5] * P[k].y;
+    o1 = o1 + M[6] * P[k].z;
+    o1 = o1 + M[7] * 1.0f;
+    float o3 = 0.0f + M[12] * P[k].x;
+    o3 = o3 + M[13] * P[k].y;
+    o3 = o3 + M[14] * P[k].z;
+    o3 = o3 + M[15] * 1.0f;
+    const float sx = o0 / o3, sy = o1 / o3;
+    X[k] = floorf((sx + (-fr.off_x)) + 0.5f);
+    Y[k] = floorf((sy + (-fr.off_y)) + 0.5f);
+  }
+  // f_bc (alpha), f_ca (beta), f_ab (gamma)
+  const float ea[4] = {Y[1] - Y[2], X[2] - X[1], X[1] * Y[2], X[2] * Y[1]};
+  const float eb[4] = {Y[2] - Y[0], X[0] - X[2], X[2] * Y[0], X[0] * Y[2]};
+  const float eg[4] = {Y[0] - Y[1], X[1] - X[0], X[0] * Y[1], X[1] * Y[0]};
+  const float fa = edge_eval(ea, X[0], Y[0]);
+  const float fb = edge_eval(eb, X[1], Y[1]);
+  const float fg = edge_eval(eg, X[2], Y[2]);
+  const float oa = edge_eval(ea, -1.0f, -1.0f);
+  const float ob = edge_eval(eb, -1.0f, -1.0f);
+  const float og = edge_eval(eg, -1.0f, -1.0f);
+  uint32_t flags = 0;
+  if (fa * oa > 0.0f) flags |= kTieA;
+  if (fb * ob > 0.0f) flags |= kTieB;
+  if (fg * og > 0.0f) flags |= kTieG;
+  // bbox: min/max closures (NaN skipped), `as usize`, clipped to the screen
+  float mnx = INFINITY, mny = INFINITY, mxx = -INFINITY, mxy = -INFINITY;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    if (X[k] < mnx) mnx = X[k];
+    if (Y[k] < mny) mny = Y[k];
+    if (X[k] > mxx) mxx = X[k];
+    if (Y[k] > mxy) mxy = Y[k];
+  }
+  uint32_t x0, x1, y0, y1;
+  rect_axis(f2usize(mnx), f2usize(mxx), (uint64_t)fr.W - 1u, &x0, &x1);
+  rect_axis(f2usize(mny), f2usize(mxy), (uint64_t)fr.H - 1u, &y0, &y1);
+  const bool fin = isfinite(fa) && isfinite(fb) && isfinite(fg);
+  if (fin && fa != 0.0f && fb != 0.0f && fg != 0.0f) flags |= kFast;
+  // Exact drops.  (1) A NaN normalizer makes that barycentric NaN at every
+  // pixel, which fails `>= 0`.  (2) If all three normalizers are +0 and every
+  // edge evaluation over the scan rectangle and the vertices is exact (all
+  // partial sums below 2^24), the three edge values sum to 0 at every pixel;
+  // coverage would need all three > 0 (x/+0 = +inf), so no pixel is covered.
+  if (isnan(fa) || isnan(fb) || isnan(fg)) flags |= kDrop;
+  if (__float_as_uint(fa) == 0u && __float_as_uint(fb) == 0u && __float_as_uint(fg) == 0u) {
+    double R = (double)x1 > (double)y1 ? (double)x1 : (double)y1;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      R = fmax(R, fabs((double)X[k]));
+      R = fmax(R, fabs((double)Y[k]));
+    }
+    auto exact_edge = [R](const float e[4]) {
+      const double S = fabs((double)e[0]) * R + fabs((double)e[1]) * R + fabs((double)e[2]) + fabs((double)e[3]);
+      return S < 16777216.0;
+    };
+    if (isfinite(R) && exact_edge(ea) && exact_edge(eb) && exact_edge(eg)) flags |= kDrop;
+  }
+  float4* ro = reinterpret_cast<float4*>(rout);
+  ro[0] = make_float4(ea[0], ea[1], ea[2], ea[3]);
+  ro[1] = make_float4(eb[0], eb[1], eb[2], eb[3]);
+  ro[2] = make_float4(eg[0], eg[1], eg[2], eg[3]);
+  ro[3] = make_float4(fa, fb, fg, 0.0f);
+  ro[4] = make_float4(A.d, B.d, C.d, 0.0f);
+  ro[5] = make_float4(__uint_as_float(x0 | (x1 << 16)), __uint_as_float(y0 | (y1 << 16)), __uint_as_float(flags),
+                      __uint_as_float(view));
+  // shading attributes: uv[6], material, pad, n[9], l[9], h[9], pad
+  float4* so = reinterpret_cast<float4*>(sout);
+  so[0] = make_float4(A.u, A.v, B.u, B.v);
+  so[1] = make_float4(C.u, C.v, __uint_as_float(material), 0.0f);
+  so[2] = make_float4(A.n.x, A.n.y, A.n.z, B.n.x);
+  so[3] = make_float4(B.n.y, B.n.z, C.n.x, C.n.y);
+  so[4] = make_float4(C.n.z, A.l.x, A.l.y, A.l.z);
+  so[5] = make_float4(B.l.x, B.l.y, B.l.z, C.l.x);
+  so[6] = make_float4(C.l.y, C.l.z, A.h.x, A.h.y);
+  so[7] = make_float4(A.h.z, B.h.x, B.h.y, B.h.z);
+  so[8] = make_float4(C.h.x, C.h.y, C.h.z, 0.0f);
+}
+
+// ================================================================== geometry
+__global__ __launch_bounds__(256) void k_geom_count(const DevDraw* __restrict__ draws, uint32_t n_draws,
+                                                    const DevView* __restrict__ views, uint32_t n_in,
+                                                    uint8_t* __restrict__ cnt, uint32_t* __restrict__ blk) {
+  __shared__ uint32_t s_tmp[4];
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  uint32_t c = 0;
+  if (t < n_in) {
+    const DevDraw& dr = draws[find_draw(draws, n_draws, t)];
+    const FrameParamsPod& fr = views[dr.view].fr;
+    const uint32_t* ti = dr.tri + 9ull * (t - dr.tri_begin);
+    PV a, b, cc;
+    a.p = ld3(dr.pos, ti[0]);
+    b.p = ld3(dr.pos, ti[1]);
+    cc.p = ld3(dr.pos, ti[2]);
+    c = (uint32_t)cull_and_clip(fr, a, b, cc, [](int, const PV&, const PV&, const PV&) {});
+    cnt[t] = (uint8_t)c;
+  }
+  uint32_t total;
+  block_exclusive_scan256(c, s_tmp, &total);
+  if (threadIdx.x == 0) blk[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(256) void k_geom_emit(const DevDraw* __restrict__ draws, uint32_t n_draws,
+                                                   const DevView* __restrict__ views, uint32_t n_in,
+                                                   const uint8_t* __restrict__ cnt,
+                                                   const uint32_t* __restrict__ blk_off, SetupRec* __restrict__ setup,
+                                                   ShadeRec* __restrict__ shade, uint32_t cap_tri,
+                                                   Status* __restrict__ status) {
+  __shared__ uint32_t s_tmp[4];
+  const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t c = t < n_in ? (uint32_t)cnt[t] : 0u;
+  uint32_t total;
+  const uint32_t local = block_exclusive_scan256(c, s_tmp, &total);
+  if (c == 0) return;
+  const uint32_t base = blk_off[blockIdx.x] + local;
+  const DevDraw& dr = draws[find_draw(draws, n_draws, t)];
+  const FrameParamsPod& fr = views[dr.view].fr;
+  const uint32_t* ti = dr.tri + 9ull * (t - dr.tri_begin);
+  const V3 cam = v3(fr.cam[0], fr.cam[1], fr.cam[2]);
+  const V3 L = v3(fr.light[0], fr.light[1], fr.light[2]);
+  FV v[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const V3 p = ld3(dr.pos, ti[k]);
+    v[k].p = p;
+    // per-vertex visual info, scene/mod.rs:920-925
+    const V3 eye_dir = sub(p, cam);
+    const V3 light = normalized(sub(p, L));
+    const V3 eye = normalized(eye_dir);
+    v[k].d = norm(eye_dir);
+    v[k].l = light;
+    v[k].h = normalized(add(light, eye));
+    v[k].n = ld3(dr.nrm, ti[6 + k]);
+    const float* uv = dr.uv + 2ull * ti[3 + k];
+    v[k].u = uv[0];
+    v[k].v = uv[1];
+  }
+  const uint32_t material = dr.material, view = dr.view;
+  cull_and_clip(fr, v[0], v[1], v[2], [&](int k, const FV& A, const FV& B, const FV& C) {
+    const uint32_t idx = base + (uint32_t)k;
+    if (idx >= cap_tri) {
+      atomicOr(&status->overflow, 1u);
+      return;
+    }
+    make_setup(fr, A, B, C, material, view, setup + idx, shade + idx);
+  });
+}
+
+// ====================================================================== scan
+// Generic exclusive scan of u32 with the element count possibly on the device.
+constexpr int kScanG = 1024;  // partial sums per scan
+
+__device__ inline uint32_t scan_count(const uint32_t* n_ptr, uint32_t n_const, uint32_t cap) {
+  if (!n_ptr) return n_const;
+  uint32_t n = *n_ptr;
+  return n < cap ? n : cap;
+}
+__device__ inline uint32_t scan_chunk(uint32_t n) { return (((n + kScanG - 1u) / kScanG) + 1023u) & ~1023u; }
+
+__global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict__ in, const uint32_t* n_ptr,
+                                                     uint32_t n_const, uint32_t cap, uint32_t* __restrict__ partials) {
+  __shared__ uint32_t s_tmp[4];
+  const uint32_t n = scan_count(n_ptr, n_const, cap);
+  const uint32_t C = scan_chunk(n);
+  const uint32_t b0 = blockIdx.x * C;
+  const uint32_t b1 = min(b0 + C, n);
+  uint32_t s = 0;
+  for (uint32_t i = b0 + threadIdx.x; i < b1; i += 256) s += in[i];
+  uint32_t total;
+  block_exclusive_scan256(s, s_tmp, &total);
+  if (threadIdx.x == 0) partials[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_spine(uint32_t* __restrict__ partials, uint32_t* total_out,
+                                                     uint32_t cap_check, uint32_t* overflow_word,
+                                                     uint32_t overflow_bit) {
+  // kScanG == 1024: one value per thread.
+  __shared__ uint32_t s_w[16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t v = partials[threadIdx.x];
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_w[wave] = x;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+  for (int w = 0; w < 16; w++) {
+    if (w < wave) before += s_w[w];
+    all += s_w[w];
+  }
+  partials[threadIdx.x] = before + x - v;
+  if (threadIdx.x == 0) {
+    if (total_out) *total_out = all;
+    if (overflow_word && all > cap_check) atomicOr(overflow_word, overflow_bit);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                    const uint32_t* n_ptr, uint32_t n_const, uint32_t cap,
+                                                    const uint32_t* __restrict__ partials) {
+  __shared__ uint32_t s_tmp[4];
+  const uint32_t n = scan_count(n_ptr, n_const, cap);
+  const uint32_t C = scan_chunk(n);
+  const uint32_t b0 = blockIdx.x * C;
+  const uint32_t b1 = min(b0 + C, n);
+  uint32_t running = partials[blockIdx.x];
+  for (uint32_t tb = b0; tb < b1; tb += 1024) {
+    const uint32_t i0 = tb + threadIdx.x * 4u;
+    uint32_t v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) v[k] = (i0 + k < b1) ? in[i0 + k] : 0u;
+    const uint32_t s = v[0] + v[1] + v[2] + v[3];
+    uint32_t total;
+    uint32_t ex = block_exclusive_scan256(s, s_tmp, &total) + running;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      if (i0 + k < b1) out[i0 + k] = ex;
+      ex += v[k];
+    }
+    running += total;
+  }
+}
+
+static void scan_u32(const uint32_t* in, uint32_t* out, const uint32_t* n_ptr, uint32_t n_const, uint32_t cap,
+                     uint32_t* partials, uint32_t* total_out, uint32_t cap_check, uint32_t* overflow_word,
+                     uint32_t overflow_bit, hipStream_t s) {
+  hipLaunchKernelGGL(k_scan_reduce, dim3(kScanG), dim3(256), 0, s, in, n_ptr, n_const, cap, partials);
+  hipLaunchKernelGGL(k_scan_spine, dim3(1), dim3(1024), 0, s, partials, total_out, cap_check, overflow_word,
+                     overflow_bit);
+  hipLaunchKernelGGL(k_scan_apply, dim3(kScanG), dim3(256), 0, s, in, out, n_ptr, n_const, cap, partials);
+}
+
+// =================================================================== binning
+// Conservative tile-row spans of a clipped triangle.  For a FAST triangle
+// every edge value at an integer pixel is e = fl(fl(fl(fl(Ax)+fl(By))+C)-D)
+// with |e - (Ax+By+C-D)| <= 4.01 u S, S = |A|x + |B|y + |C| + |D|; a pixel can
+// only be covered when sign(f) * e >= 0 on all three edges.  Each edge's
+// exact-arithmetic function, widened by that bound (plus 1), therefore gives
+// a superset of the tiles holding covered pixels.  Other triangles are binned
+// to every tile of their scan rectangle.
+struct Spans {
+  int cx0, cx1, cy0, cy1;
+  bool exact;
+  double a[3], b[3], k[3], m[3];
+};
+
+__device__ inline void spans_init(const SetupRec& r, int TS, Spans& g) {
+  const uint32_t x0 = r.rx & 0xFFFFu, x1 = r.rx >> 16, y0 = r.ry & 0xFFFFu, y1 = r.ry >> 16;
+  g.cx0 = (int)(x0 / TS); g.cx1 = (int)(x1 / TS);
+  g.cy0 = (int)(y0 / TS); g.cy1 = (int)(y1 / TS);
+  g.exact = (r.flags & kFast) != 0;
+  const float* es[3] = {r.ea, r.eb, r.eg};
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const double A = es[i][0], B = es[i][1], C = es[i][2], D = es[i][3];
+    const double s = r.f[i] > 0.0f ? 1.0 : -1.0;
+    const double S = fabs(A) * (double)x1 + fabs(B) * (double)y1 + fabs(C) + fabs(D);
+    g.a[i] = s * A;
+    g.b[i] = s * B;
+    g.k[i] = s * (C - D);
+    g.m[i] = 4.02 * 5.9604644775390625e-08 * S + 1.0 + 1e-9 * S;
+    if (!isfinite(S)) g.exact = false;
+  }
+}
+
+// Columns [lo, hi] of tile row ty that may hold covered pixels (lo > hi: none).
+__device__ inline void spans_row(const Spans& g, int TS, int ty, int* lo, int* hi) {
+  int l = g.cx0, h = g.cx1;
+  if (g.exact) {
+    const double ya = (double)(ty * TS), yb = (double)(ty * TS + TS - 1);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      const double K = g.k[i] + fmax(g.b[i] * ya, g.b[i] * yb);
+      const double a = g.a[i];
+      if (a > 0.0) {
+        // need a*(c*TS + TS-1) + K >= -m
+        double xl = (-g.m[i] - K) / a - 1.0;
+        double c = (xl - (double)(TS - 1)) / (double)TS;
+        c = fmin(fmax(c, -1.0), 1.0e6);
+        const int ci = (int)ceil(c);
+        l = max(l, ci);
+      } else if (a < 0.0) {
+        // need a*(c*TS) + K >= -m
+        double xr = (K + g.m[i]) / (-a) + 1.0;
+        double c = xr / (double)TS;
+        c = fmin(fmax(c, -2.0), 1.0e6);
+        const int ci = (int)floor(c);
+        h = min(h, ci);
+      } else if (K < -g.m[i]) {
+        h = l - 1;
+      }
+    }
+  }
+  *lo = l;
+  *hi = h;
+}
+
+__global__ __launch_bounds__(256) void k_bin_count(const SetupRec* __restrict__ setup, const Status* status,
+                                                   uint32_t cap_tri, int TS, uint32_t* __restrict__ bincnt) {
+  const uint32_t n = min(status->n_out, cap_tri);
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+    const SetupRec r = setup[i];
+    uint32_t c = 0;
+    if (!(r.flags & kDrop)) {
+      Spans g;
+      spans_init(r, TS, g);
+      for (int ty = g.cy0; ty <= g.cy1; ty++) {
+        int lo, hi;
+        spans_row(g, TS, ty, &lo, &hi);
+        if (hi >= lo) c += (uint32_t)(hi - lo + 1);
+      }
+    }
+    bincnt[i] = c;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_bin_emit(const SetupRec* __restrict__ setup, const DevView* __restrict__ views,
+                                                  const Status* status, uint32_t cap_tri, int TS,
+                                                  const uint32_t* __restrict__ binoff, uint32_t cap_pairs,
+                                                  uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const uint32_t n = min(status->n_out, cap_tri);
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+    const SetupRec r = setup[i];
+    if (r.flags & kDrop) continue;
+    const DevView& vw = views[r.view];
+    uint32_t o = binoff[i];
+    Spans g;
+    spans_init(r, TS, g);
+    for (int ty = g.cy0; ty <= g.cy1; ty++) {
+      int lo, hi;
+      spans_row(g, TS, ty, &lo, &hi);
+      for (int tx = lo; tx <= hi; tx++, o++) {
+        if (o < cap_pairs) {
+          keys[o] = vw.tile_base + (uint32_t)ty * vw.ntx + (uint32_t)tx;
+          vals[o] = i;
+        }
+      }
+    }
+  }
+}
+
+// ================================================================ radix sort
+// Stable LSD radix sort of (tile key, triangle id) pairs, 8-bit digits.  Block
+// g owns one contiguous chunk of the input and ranks its items in input order
+// (wave ballot match + per-wave digit counts), so equal keys keep the
+// triangle order.
+constexpr int kRadixG = 512;
+
+__device__ inline uint32_t radix_chunk(uint32_t n) { return (((n + kRadixG - 1u) / kRadixG) + 255u) & ~255u; }
+
+__global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* __restrict__ keys, const uint32_t* n_ptr,
+                                                    uint32_t cap, int shift, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[256];
+  const uint32_t n = min(*n_ptr, cap);
+  const uint32_t C = radix_chunk(n);
+  const uint32_t b0 = blockIdx.x * C, b1 = min(b0 + C, n);
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  for (uint32_t i = b0 + threadIdx.x; i < b1; i += 256) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+  __syncthreads();
+  hist[threadIdx.x * kRadixG + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+                                                       uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                       const uint32_t* n_ptr, uint32_t cap, int shift,
+                                                       const uint32_t* __restrict__ hist_off) {
+  __shared__ uint32_t running[256];
+  __shared__ uint32_t wcnt[4][256];
+  const uint32_t n = min(*n_ptr, cap);
+  const uint32_t C = radix_chunk(n);
+  const uint32_t b0 = blockIdx.x * C, b1 = min(b0 + C, n);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  running[tid] = hist_off[tid * kRadixG + blockIdx.x];
+#pragma unroll
+  for (int w = 0; w < 4; w++) wcnt[w][tid] = 0;
+  __syncthreads();
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (uint32_t tb = b0; tb < b1; tb += 256) {
+    const uint32_t i = tb + (uint32_t)tid;
+    const bool valid = i < b1;
+    const uint32_t key = valid ? kin[i] : 0u;
+    const uint32_t val = valid ? vin[i] : 0u;
+    const uint32_t dgt = (key >> shift) & 255u;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < 8; bit++) {
+      const bool on = (dgt >> bit) & 1u;
+      const uint64_t m = __ballot(on);
+      peers &= on ? m : ~m;
+    }
+    const uint32_t rank = (uint32_t)__popcll(peers & lt);
+    if (valid && rank == 0) wcnt[wave][dgt] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = running[dgt] + rank;
+      for (int w = 0; w < wave; w++) pos += wcnt[w][dgt];
+      kout[pos] = key;
+      vout[pos] = val;
+    }
+    __syncthreads();
+    running[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
+#pragma unroll
+    for (int w = 0; w < 4; w++) wcnt[w][tid] = 0;
+    __syncthreads();
+  }
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_tile_ranges(const uint32_t* __restrict__ keys, const uint32_t* n_ptr,
+                                                     uint32_t cap, uint2* __restrict__ ranges) {
+  const uint32_t n = min(*n_ptr, cap);
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+    const uint32_t k = keys[i];
+    if (i == 0 || keys[i - 1] != k) ranges[k].x = i;
+    if (i + 1 == n || keys[i + 1] != k) ranges[k].y = i + 1;
+  }
+}
+
+// ==================================================================== raster
+// Coverage of one pixel for a FAST triangle, decided without division.  Edge
+// values are integers here (snapped vertices are floor() results and pixel
+// centres are integers), so for a finite nonzero normalizer f the quotient
+// e/f is NaN iff e is NaN, zero iff e is zero, and otherwise nonzero with the
+// sign of e*f (|e/f| >= 1/FLT_MAX is representable as a denormal).  Hence
+// alpha >= 0 && (alpha > 0 || tie) <=> e*f > 0 || (e == 0 && tie).
+__device__ inline bool covered_fast(float e, float f, bool tie) { return (e * f > 0.0f) || (e == 0.0f && tie); }
+
+struct FragOut {
+  float al, be, ga, d;
+};
+
+// Barycentrics and depth at a pixel, canvas.rs:673-682.
+__device__ inline FragOut bary(const SetupRec& r, float x, float y) {
+  FragOut o;
+  o.al = edge_eval(r.ea, x, y) / r.f[0];
+  o.be = edge_eval(r.eb, x, y) / r.f[1];
+  o.ga = edge_eval(r.eg, x, y) / r.f[2];
+  o.d = (o.al * r.d[0] + o.be * r.d[1]) + o.ga * r.d[2];
+  return o;
+}
+
+// The reference samples both maps of every covered fragment before its depth
+// test (canvas.rs:689-695), so an out-of-range texture coordinate fails the
+// frame even for a hidden fragment.
+__device__ inline bool uv_ok(const ShadeRec* __restrict__ shade, const DevMaterial* __restrict__ mats, uint32_t id,
+                             const FragOut& o) {
+  const ShadeRec& s = shade[id];
+  const float u = (s.uv[0] * o.al + s.uv[2] * o.be) + s.uv[4] * o.ga;
+  const float v = (s.uv[1] * o.al + s.uv[3] * o.be) + s.uv[5] * o.ga;
+  const DevMaterial& m = mats[s.material];
+  uint64_t off;
+  return rgb_offset(m.kd_w, m.kd_h, m.kd_comp, u, v, &off) && rgb_offset(m.ka_w, m.ka_h, m.ka_comp, u, v, &off);
+}
+
+template <int TS, int QPW>
+__global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
+    const DevView* __restrict__ views, uint32_t n_views, const uint2* __restrict__ ranges,
+    const uint32_t* __restrict__ vals, const SetupRec* __restrict__ setup, const ShadeRec* __restrict__ shade,
+    const DevMaterial* __restrict__ mats, uint32_t* __restrict__ vis_out, Status* __restrict__ status) {
+  constexpr int QS = TS / 8, NQ = QS * QS, NW = NQ / QPW, NT = 64 * NW;
+  constexpr int CH = 128;    // records staged per round
+  constexpr int QCAP = 256;  // fragment queue per wave
+  __shared__ SetupRec s_rec[CH];
+  __shared__ uint32_t s_id[CH];
+  __shared__ unsigned long long s_vis[TS * TS];
+  __shared__ uint32_t s_q[NW][QCAP];
+
+  const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+  const DevView& vw = views[find_view_tile(views, n_views, tile)];
+  const uint32_t lt = tile - vw.tile_base;
+  const int tx = (int)(lt % vw.ntx), ty = (int)(lt / vw.ntx);
+  const int W = vw.fr.W, H = vw.fr.H;
+  const float clear_depth = vw.fr.clear_depth;
+  const int tx0 = tx * TS, ty0 = ty * TS;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint2 rg = ranges[tile];
+
+  for (int p = tid; p < TS * TS; p += NT) s_vis[p] = ~0ull;
+
+  const int lx = lane & 7, ly = lane >> 3;
+  const uint64_t lanemask_lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  uint32_t qn = 0;
+  uint32_t nfrag = 0;
+
+  auto submit = [&](const SetupRec& r, uint32_t id, int px, int py) {
+    const FragOut o = bary(r, (float)px, (float)py);
+    if (!uv_ok(shade, mats, id, o)) atomicOr(&status->error, 1u);
+    if (o.d < clear_depth) {
+      const unsigned long long key = ((unsigned long long)depth_key(o.d) << 32) | id;
+      atomicMin(&s_vis[(py - ty0) * TS + (px - tx0)], key);
+    }
+  };
+  auto flush = [&]() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // queue writes by other lanes of this wave
+    for (uint32_t j = (uint32_t)lane; j < qn; j += 64) {
+      const uint32_t q = s_q[wave][j];
+      const uint32_t k = q >> 16, lp = q & 0xFFFFu;
+      const int px = tx0 + (int)(lp % TS), py = ty0 + (int)(lp / TS);
+      submit(s_rec[k], s_id[k], px, py);
+    }
+    nfrag += qn;
+    qn = 0;
+  };
+
+  for (uint32_t base = rg.x; base < rg.y; base += CH) {
+    const int n = (int)min((uint32_t)CH, rg.y - base);
+    __syncthreads();
+    for (int i = tid; i < n; i += NT) {
+      const uint32_t id = vals[base + i];
+      s_id[i] = id;
+      const uint4* src = reinterpret_cast<const uint4*>(setup + id);
+      uint4* dst = reinterpret_cast<uint4*>(s_rec + i);
+#pragma unroll
+      for (int w = 0; w < 6; w++) dst[w] = src[w];
+    }
+    __syncthreads();
+    for (int k = 0; k < n; k++) {
+      const SetupRec& r = s_rec[k];
+      const int rx0 = (int)(r.rx & 0xFFFFu), rx1 = (int)(r.rx >> 16);
+      const int ry0 = (int)(r.ry & 0xFFFFu), ry1 = (int)(r.ry >> 16);
+      const uint32_t flags = r.flags;
+#pragma unroll
+      for (int qq = 0; qq < QPW; qq++) {
+        const int Q = wave * QPW + qq;
+        const int qx0 = tx0 + (Q % QS) * 8, qy0 = ty0 + (Q / QS) * 8;
+        if (rx1 < qx0 || rx0 > qx0 + 7 || ry1 < qy0 || ry0 > qy0 + 7) continue;
+        const int px = qx0 + lx, py = qy0 + ly;
+        const bool in = px >= rx0 && px <= rx1 && py >= ry0 && py <= ry1;
+        const float xf = (float)px, yf = (float)py;
+        if (flags & kFast) {
+          bool cov = false;
+          if (in) {
+            const float e0 = edge_eval(r.ea, xf, yf);
+            const float e1 = edge_eval(r.eb, xf, yf);
+            const float e2 = edge_eval(r.eg, xf, yf);
+            cov = covered_fast(e0, r.f[0], flags & kTieA) && covered_fast(e1, r.f[1], flags & kTieB) &&
+                  covered_fast(e2, r.f[2], flags & kTieG);
+          }
+          const uint64_t m = __ballot(cov);
+          if (m) {
+            if (cov) {
+              const uint32_t pos = qn + (uint32_t)__popcll(m & lanemask_lt);
+              s_q[wave][pos] = ((uint32_t)k << 16) | (uint32_t)((py - ty0) * TS + (px - tx0));
+            }
+            qn += (uint32_t)__popcll(m);
+            if (qn > QCAP - 64) flush();
+          }
+        } else {
+          // Any other triangle: the reference's predicate on the divided values.
+          bool cov = false;
+          if (in) {
+            const FragOut o = bary(r, xf, yf);
+            cov = o.al >= 0.0f && o.be >= 0.0f && o.ga >= 0.0f && (o.al > 0.0f || (flags & kTieA)) &&
+                  (o.be > 0.0f || (flags & kTieB)) && (o.ga > 0.0f || (flags & kTieG));
+          }
+          if (cov) submit(r, s_id[k], px, py);
+          nfrag += (uint32_t)__popcll(__ballot(cov));
+        }
+      }
+    }
+    flush();
+  }
+  __syncthreads();
+  for (int p = tid; p < TS * TS; p += NT) {
+    const int px = tx0 + p % TS, py = ty0 + p / TS;
+    if (px < W && py < H) {
+      const unsigned long long v = s_vis[p];
+      vis_out[vw.px_base + (uint32_t)py * (uint32_t)W + (uint32_t)px] = (v == ~0ull) ? kNone : (uint32_t)v;
+    }
+  }
+  if (lane == 0) atomicAdd(&status->frags, nfrag);  // nfrag is wave-uniform
+}
+
+// ===================================================================== shade
+__device__ inline void fetch_rgb(const uint8_t* img, int32_t comp, uint64_t off, uint32_t c[3]) {
+  if (comp == 4) {
+    const uint32_t w = *reinterpret_cast<const uint32_t*>(img + off);
+    c[0] = w & 255u; c[1] = (w >> 8) & 255u; c[2] = (w >> 16) & 255u;
+  } else {
+    c[0] = img[off]; c[1] = img[off + 1]; c[2] = img[off + 2];
+  }
+}
+
+// Shading of a covered pixel, canvas.rs:685-741; returns B | G<<8 | R<<16 | A<<24.
+__device__ inline uint32_t shade_pixel(const ShadeRec& s, const DevMaterial& m, const FragOut& o, Status* status) {
+  const float u = (s.uv[0] * o.al + s.uv[2] * o.be) + s.uv[4] * o.ga;
+  const float v = (s.uv[1] * o.al + s.uv[3] * o.be) + s.uv[5] * o.ga;
+  uint64_t okd, oka;
+  const bool ok1 = rgb_offset(m.kd_w, m.kd_h, m.kd_comp, u, v, &okd);
+  const bool ok2 = rgb_offset(m.ka_w, m.ka_h, m.ka_comp, u, v, &oka);
+  if (!(ok1 && ok2)) {
+    atomicOr(&status->error, 1u);
+    okd = 0;
+    oka = 0;
+  }
+  uint32_t dk[3], ak[3];
+  fetch_rgb(m.kd_img, m.kd_comp, okd, dk);
+  fetch_rgb(m.ka_img, m.ka_comp, oka, ak);
+  const V3 diffuse = v3((float)dk[0] / 255.0f, (float)dk[1] / 255.0f, (float)dk[2] / 255.0f);
+  const V3 ambient = v3((float)ak[0] / 255.0f, (float)ak[1] / 255.0f, (float)ak[2] / 255.0f);
+  const V3 pn = add(add(muls(v3(s.n[0], s.n[1], s.n[2]), o.al), muls(v3(s.n[3], s.n[4], s.n[5]), o.be)),
+                    muls(v3(s.n[6], s.n[7], s.n[8]), o.ga));
+  const V3 pl = add(add(muls(v3(s.l[0], s.l[1], s.l[2]), o.al), muls(v3(s.l[3], s.l[4], s.l[5]), o.be)),
+                    muls(v3(s.l[6], s.l[7], s.l[8]), o.ga));
+  const V3 ph = add(add(muls(v3(s.h[0], s.h[1], s.h[2]), o.al), muls(v3(s.h[3], s.h[4], s.h[5]), o.be)),
+                    muls(v3(s.h[6], s.h[7], s.h[8]), o.ga));
+  const V3 c_l = v3(m.ks[0], m.ks[1], m.ks[2]);
+  const V3 c_r = v3(diffuse.x * m.kd[0], diffuse.y * m.kd[1], diffuse.z * m.kd[2]);
+  const V3 c_a = v3(ambient.x * m.ka[0], ambient.y * m.ka[1], ambient.z * m.ka[2]);
+  const float dln = dot(pl, pn);
+  const float mx = (dln > 0.0f) ? dln : 0.0f;  // 0.0_f32.max(x); the sign of a zero is irrelevant below
+  const float sfac = 1.0f - mx;
+  const V3 inner = add(c_a, muls(c_l, sfac));
+  const V3 t1 = v3(c_r.x * inner.x, c_r.y * inner.y, c_r.z * inner.z);
+  const float h2 = powi_f(dot(ph, pn), m.power);
+  const V3 col = add(t1, muls(c_l, h2));
+  const V3 sc = muls(col, 255.0f);
+  return f2u8(sc.z) | (f2u8(sc.y) << 8) | (f2u8(sc.x) << 16) | (255u << 24);
+}
+
+__global__ __launch_bounds__(256) void k_shade(const DevView* __restrict__ views, uint32_t n_views, uint32_t n_pixels,
+                                               const uint32_t* __restrict__ vis, const SetupRec* __restrict__ setup,
+                                               const ShadeRec* __restrict__ shade,
+                                               const DevMaterial* __restrict__ mats, Status* __restrict__ status) {
+  const uint32_t g = xcd_remap(blockIdx.x, gridDim.x) * 256u + threadIdx.x;
+  if (g >= n_pixels) return;
+  const DevView& vw = views[find_view_px(views, n_views, g)];
+  const uint32_t p = g - vw.px_base;
+  const uint32_t W = (uint32_t)vw.fr.W, H = (uint32_t)vw.fr.H;
+  const uint32_t y = p / W, x = p - y * W;
+  const uint32_t id = vis[g];
+  uint32_t bgra;
+  float depth;
+  if (id == kNone) {
+    const uint8_t* c = vw.fr.clear_bgra;
+    bgra = (uint32_t)c[0] | ((uint32_t)c[1] << 8) | ((uint32_t)c[2] << 16) | ((uint32_t)c[3] << 24);
+    depth = vw.fr.clear_depth;
+  } else {
+    const SetupRec r = setup[id];
+    const FragOut o = bary(r, (float)x, (float)y);
+    const ShadeRec& s = shade[id];
+    bgra = shade_pixel(s, mats[s.material], o, status);
+    depth = o.d;
+  }
+  vw.out_bgra[(H - 1u - y) * W + x] = bgra;  // draw_pixel_coord row flip, canvas.rs:955
+  vw.out_depth[p] = depth;
+  if (vw.out_winner) vw.out_winner[p] = id;
+}
+
+// ============================================================= orchestration
+static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+template <int TS, int QPW>
+static void launch_raster(const PipeArgs& a, hipStream_t s) {
+  constexpr int NT = 64 * (TS / 8) * (TS / 8) / QPW;
+  hipLaunchKernelGGL((k_raster<TS, QPW>), dim3(a.n_tiles), dim3(NT), 0, s, a.views, a.n_views, a.ranges, a.vals[0],
+                     a.setup, a.shade, a.mats, a.vis, a.status);
+}
+
+bool raster_config_supported(int TS, int waves) {
+  const int nq = (TS / 8) * (TS / 8);
+  if (TS != 8 && TS != 16 && TS != 32) return false;
+  if (waves < 1 || nq % waves) return false;
+  const int qpw = nq / waves;
+  return qpw == 1 || qpw == 2 || qpw == 4 || qpw == 8 || qpw == 16;
+}
+
+static void dispatch_raster(const PipeArgs& a, hipStream_t s) {
+  const int nq = (a.TS / 8) * (a.TS / 8);
+  const int qpw = nq / a.waves;
+  if (a.TS == 8) launch_raster<8, 1>(a, s);
+  else if (a.TS == 16) {
+    if (qpw == 1) launch_raster<16, 1>(a, s);
+    else if (qpw == 2) launch_raster<16, 2>(a, s);
+    else launch_raster<16, 4>(a, s);
+  } else {
+    if (qpw == 1) launch_raster<32, 1>(a, s);
+    else if (qpw == 2) launch_raster<32, 2>(a, s);
+    else if (qpw == 4) launch_raster<32, 4>(a, s);
+    else if (qpw == 8) launch_raster<32, 8>(a, s);
+    else launch_raster<32, 16>(a, s);
+  }
+}
+
+void enqueue_frame(const PipeArgs& a, hipStream_t s) {
+  if (a.ev[0]) (void)hipEventRecord(a.ev[0], s);
+  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(a.status), 0u,
+                     (uint32_t)(sizeof(Status) / 4));
+  const uint32_t n_blk = cdiv(a.n_tri_in, 256);
+  // 1-3 geometry with ordered compaction
+  if (n_blk) {
+    hipLaunchKernelGGL(k_geom_count, dim3(n_blk), dim3(256), 0, s, a.draws, a.n_draws, a.views, a.n_tri_in, a.cnt,
+                       a.blk);
+    scan_u32(a.blk, a.blk, nullptr, n_blk, n_blk, a.partials, &a.status->n_out, a.cap_tri, &a.status->overflow, 1u,
+             s);
+    hipLaunchKernelGGL(k_geom_emit, dim3(n_blk), dim3(256), 0, s, a.draws, a.n_draws, a.views, a.n_tri_in, a.cnt,
+                       a.blk, a.setup, a.shade, a.cap_tri, a.status);
+  }
+  if (a.ev[1]) (void)hipEventRecord(a.ev[1], s);
+  // 4-6 binning
+  const uint32_t gb = std::min<uint32_t>(cdiv(a.cap_tri, 256), 4096u);
+  hipLaunchKernelGGL(k_bin_count, dim3(gb), dim3(256), 0, s, a.setup, a.status, a.cap_tri, a.TS, a.bincnt);
+  scan_u32(a.bincnt, a.binoff, &a.status->n_out, 0, a.cap_tri, a.partials, &a.status->n_pairs, a.cap_pairs,
+           &a.status->overflow, 2u, s);
+  hipLaunchKernelGGL(k_bin_emit, dim3(gb), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri, a.TS, a.binoff,
+                     a.cap_pairs, a.keys[0], a.vals[0]);
+  // 7 stable radix sort by tile key
+  int cur = 0;
+  for (int shift = 0; shift < a.key_bits; shift += 8) {
+    hipLaunchKernelGGL(k_radix_hist, dim3(kRadixG), dim3(256), 0, s, a.keys[cur], &a.status->n_pairs, a.cap_pairs,
+                       shift, a.hist);
+    scan_u32(a.hist, a.hist, nullptr, 256u * kRadixG, 256u * kRadixG, a.partials, nullptr, 0, nullptr, 0, s);
+    hipLaunchKernelGGL(k_radix_scatter, dim3(kRadixG), dim3(256), 0, s, a.keys[cur], a.vals[cur], a.keys[cur ^ 1],
+                       a.vals[cur ^ 1], &a.status->n_pairs, a.cap_pairs, shift, a.hist);
+    cur ^= 1;
+  }
+  const uint32_t* skeys = a.keys[cur];
+  PipeArgs b = a;
+  b.vals[0] = a.vals[cur];
+  // 8 tile ranges
+  hipLaunchKernelGGL(k_fill_u32, dim3(std::min<uint32_t>(cdiv(2ull * a.n_tiles, 256), 2048u)), dim3(256), 0, s,
+                     reinterpret_cast<uint32_t*>(a.ranges), 0u, 2u * a.n_tiles);
+  hipLaunchKernelGGL(k_tile_ranges, dim3(std::min<uint32_t>(cdiv(a.cap_pairs, 256), 4096u)), dim3(256), 0, s, skeys,
+                     &a.status->n_pairs, a.cap_pairs, a.ranges);
+  if (a.ev[2]) (void)hipEventRecord(a.ev[2], s);
+  // 9 raster, 10 shade
+  dispatch_raster(b, s);
+  if (a.ev[3]) (void)hipEventRecord(a.ev[3], s);
+  hipLaunchKernelGGL(k_shade, dim3(cdiv(a.n_pixels, 256)), dim3(256), 0, s, a.views, a.n_views, a.n_pixels, a.vis,
+                     a.setup, a.shade, a.mats, a.status);
+  if (a.ev[4]) (void)hipEventRecord(a.ev[4], s);
+}
+
+}  // namespace dh

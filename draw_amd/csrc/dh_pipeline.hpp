@@ -1,0 +1,43 @@
+// dh_pipeline.hpp -- device buffers of one frame (or one batch of views) and
+// the entry that enqueues the whole pipeline on a stream.
+#pragma once
+#include "dh_common.hpp"
+
+namespace dh {
+
+struct PipeArgs {
+  // inputs
+  const DevDraw* draws;
+  uint32_t n_draws;
+  uint32_t n_tri_in;
+  const DevView* views;
+  uint32_t n_views;
+  uint32_t n_tiles;    // over all views
+  uint32_t n_pixels;   // over all views
+  int key_bits;        // bits of (n_tiles - 1)
+  const DevMaterial* mats;
+  int TS;              // tile size in pixels
+  int waves;           // waves per tile
+  // scratch
+  uint8_t* cnt;        // [n_tri_in] clipped-triangle count per input triangle
+  uint32_t* blk;       // [ceil(n_tri_in/256)] block sums -> offsets
+  SetupRec* setup;     // [cap_tri]
+  ShadeRec* shade;     // [cap_tri]
+  uint32_t cap_tri;
+  uint32_t* bincnt;    // [cap_tri]
+  uint32_t* binoff;    // [cap_tri]
+  uint32_t* keys[2];   // [cap_pairs] each
+  uint32_t* vals[2];
+  uint32_t cap_pairs;
+  uint32_t* hist;      // [256 * kRadixG]
+  uint32_t* partials;  // [1024]
+  uint2* ranges;       // [n_tiles]
+  uint32_t* vis;       // [n_pixels] winner id per pixel
+  Status* status;
+  hipEvent_t ev[5];    // optional stage events (all null when not profiling)
+};
+
+void enqueue_frame(const PipeArgs& a, hipStream_t s);
+bool raster_config_supported(int TS, int waves);
+
+}  // namespace dh

@@ -1,0 +1,168 @@
+"""GPU renderer: Scene::render (scene/mod.rs:901) on libdrawhip.
+
+``Renderer`` owns one libdrawhip context (one HIP stream, device-resident
+resources).  It uploads a ``Scene``'s textures, materials, objects and meshes
+once (cached per Python object) and renders frames into device canvases;
+``render(scene, canvas)`` copies the result into a host ``Canvas`` the way the
+reference fills its canvas.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Dict, List, Tuple
+
+import numpy as np
+
+from . import _lib
+from ._lib import DhCamera, DhFrameParams, DhMaterialDesc, DhStats, DrawHipError
+from .scene import Canvas, Scene
+
+
+def _f3(v):
+    return (C.c_float * 3)(*[float(np.float32(x)) for x in v])
+
+
+def frame_uniforms(scene: Scene) -> DhFrameParams:
+    """a1 on the host: Camera::new + gen_transformation_matrix + gen_view_planes."""
+    cam = scene.camera
+    cd = DhCamera(_f3(cam.position), _f3(cam.direction), float(cam.fov_x), float(cam.near), float(cam.far))
+    fr = DhFrameParams()
+    rc = _lib.lib().dh_frame_uniforms(C.byref(cd), int(scene.width), int(scene.height), _f3(scene.light_source),
+                                      C.byref(fr))
+    if rc != 0:
+        raise DrawHipError(rc, "dh_frame_uniforms")
+    return fr
+
+
+class Renderer:
+    def __init__(self, device: int = 0, tile: int = 16, waves: int = 4, profiling: bool = False):
+        self.L = _lib.lib()
+        ctx = C.c_void_p()
+        rc = self.L.dh_ctx_create(int(device), C.byref(ctx))
+        if rc != 0:
+            raise DrawHipError(rc, "dh_ctx_create", "no HIP device" if rc == -9 else "")
+        self.ctx = ctx
+        self._tex: Dict[int, int] = {}
+        self._mat: Dict[Tuple[int, int], int] = {}
+        self._obj: Dict[int, Tuple[int, List[int]]] = {}
+        self._canvas: Dict[Tuple[int, int], int] = {}
+        self._keep = []
+        self.set_tiling(tile, waves)
+        self.set_profiling(profiling)
+
+    # ------------------------------------------------------------ plumbing
+    def _check(self, rc: int, what: str):
+        if rc != 0:
+            detail = self.L.dh_last_error(self.ctx)
+            raise DrawHipError(rc, what, detail.decode() if detail else "")
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.L.dh_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tiling(self, tile: int, waves: int):
+        self._check(self.L.dh_set_tiling(self.ctx, int(tile), int(waves)), "dh_set_tiling")
+
+    def set_profiling(self, on: bool):
+        self._check(self.L.dh_set_profiling(self.ctx, int(bool(on))), "dh_set_profiling")
+
+    # ----------------------------------------------------------- resources
+    def texture(self, tm) -> int:
+        key = id(tm)
+        if key not in self._tex:
+            img = np.ascontiguousarray(tm.img, np.uint8)
+            h = C.c_uint64()
+            self._check(self.L.dh_texture_create(self.ctx, img.ctypes.data, tm.width, tm.height, tm.components,
+                                                 C.byref(h)), "dh_texture_create")
+            self._tex[key] = h.value
+            self._keep.append(tm)
+        return self._tex[key]
+
+    def material(self, tex) -> int:
+        key = (id(tex), 0)
+        if key not in self._mat:
+            d = DhMaterialDesc(_f3(tex.ka), _f3(tex.kd), _f3(tex.ks), float(np.float32(tex.alpha)), int(tex.power),
+                               self.texture(tex.map_kd), self.texture(tex.map_ka))
+            h = C.c_uint64()
+            self._check(self.L.dh_material_create(self.ctx, C.byref(d), C.byref(h)), "dh_material_create")
+            self._mat[key] = h.value
+            self._keep.append(tex)
+        return self._mat[key]
+
+    def upload(self, obj) -> List[int]:
+        """Object + its meshes -> mesh handles in submission order."""
+        key = id(obj)
+        if key not in self._obj:
+            pos = np.ascontiguousarray(obj.vertices, np.float32)
+            nrm = np.ascontiguousarray(obj.normals_vertices, np.float32)
+            uv = np.ascontiguousarray(obj.texture_vertices, np.float32)
+            oh = C.c_uint64()
+            self._check(self.L.dh_object_create(self.ctx, pos.ctypes.data, pos.shape[0], nrm.ctypes.data,
+                                                nrm.shape[0], uv.ctypes.data, uv.shape[0], C.byref(oh)),
+                        "dh_object_create")
+            meshes = []
+            for m in obj.meshes:
+                mat = self.material(obj.textures[m.texture_idx]) if 0 <= m.texture_idx < len(obj.textures) else 0
+                tri = np.ascontiguousarray(m.triangles, np.uint32)
+                mh = C.c_uint64()
+                self._check(self.L.dh_mesh_create(self.ctx, oh.value, tri.ctypes.data, tri.shape[0], mat,
+                                                  C.byref(mh)), "dh_mesh_create")
+                meshes.append(mh.value)
+            self._obj[key] = (oh.value, meshes)
+            self._keep.append(obj)
+        return self._obj[key][1]
+
+    def canvas(self, W: int, H: int) -> int:
+        key = (W, H)
+        if key not in self._canvas:
+            h = C.c_uint64()
+            self._check(self.L.dh_canvas_create(self.ctx, int(W), int(H), C.byref(h)), "dh_canvas_create")
+            self._canvas[key] = h.value
+        return self._canvas[key]
+
+    def mesh_handles(self, scene: Scene) -> List[int]:
+        hs: List[int] = []
+        for o in scene.objects:
+            hs += self.upload(o)
+        return hs
+
+    # ------------------------------------------------------------ rendering
+    def render_device(self, scene: Scene, frame: DhFrameParams | None = None, meshes: List[int] | None = None) -> int:
+        """Render into the device canvas of the scene's size; returns its handle."""
+        fr = frame if frame is not None else frame_uniforms(scene)
+        hs = meshes if meshes is not None else self.mesh_handles(scene)
+        arr = (C.c_uint64 * max(1, len(hs)))(*hs)
+        cv = self.canvas(fr.W, fr.H)
+        self._check(self.L.dh_render(self.ctx, C.byref(fr), arr, len(hs), cv), "dh_render")
+        return cv
+
+    def read(self, canvas_handle: int, W: int, H: int):
+        bgra = np.empty((H, W, 4), np.uint8)
+        depth = np.empty((H, W), np.float32)
+        winner = np.empty((H, W), np.uint32)
+        self._check(self.L.dh_canvas_read(self.ctx, canvas_handle, bgra.ctypes.data, depth.ctypes.data,
+                                          winner.ctypes.data), "dh_canvas_read")
+        return bgra, depth, winner
+
+    def render(self, scene: Scene, canvas: Canvas | None = None, frame: DhFrameParams | None = None):
+        """Scene::render(&mut canvas): returns (canvas, winner ids)."""
+        fr = frame if frame is not None else frame_uniforms(scene)
+        cv = self.render_device(scene, fr)
+        if canvas is None:
+            canvas = Canvas(fr.W, fr.H)
+        bgra, depth, winner = self.read(cv, fr.W, fr.H)
+        canvas.frame[...] = bgra
+        canvas.depth[...] = depth
+        return canvas, winner
+
+    def stats(self) -> dict:
+        st = DhStats()
+        self._check(self.L.dh_get_stats(self.ctx, C.byref(st)), "dh_get_stats")
+        return {n: getattr(st, n) for n, _ in DhStats._fields_}

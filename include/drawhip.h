@@ -1,0 +1,162 @@
+/* drawhip.h -- C ABI of libdrawhip, the MI355X (gfx950) raster path of
+ * mororo18/draw.
+ *
+ * The reference has no FFI: its seam is two Rust methods, Scene::render
+ * (src/renderer/scene/mod.rs:901) and Canvas::draw_triangle_with_attributes
+ * (src/renderer/canvas.rs:577), called once per frame and once per clipped
+ * triangle.  A per-triangle FFI would be launch-bound, so this ABI replaces
+ * the body of Scene::render (scene/mod.rs:901-1249) for whole frames, while
+ * the caller keeps its own scene, camera and draw-call API.
+ *
+ * Conventions
+ *  - Every call returns int: DH_OK (0) or a negative DH_E_* code.  Nothing
+ *    unwinds across the ABI.  The reference panics where this returns an error.
+ *  - Host buffers are caller-owned and only borrowed for the duration of a call.
+ *  - Device objects (textures, materials, objects, meshes, canvases) live in
+ *    HBM, owned by the context, behind opaque 64-bit handles.
+ *  - One context = one HIP device + one HIP stream + its scratch memory.  A
+ *    context may move between threads but must not be used by two at once.
+ */
+#ifndef DRAWHIP_H
+#define DRAWHIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    DH_OK = 0,
+    DH_E_ARG = -1,        /* invalid argument                                           */
+    DH_E_INDEX = -2,      /* vertex index out of range: IndexedMesh::vec3_list_from_indexed
+                             assert! (scene/mesh.rs:46-48)                               */
+    DH_E_TEXCOORD = -3,   /* texture slice out of range: TextureMap::get_rgb_slice
+                             (scene/mod.rs:167) panics on that fragment                   */
+    DH_E_CAMERA = -4,     /* Camera::new asserts (scene/mod.rs:327-331)                   */
+    DH_E_PLANE = -5,      /* ViewPlane::new assert (scene/mod.rs:624)                      */
+    DH_E_HIP = -6,        /* HIP runtime error                                           */
+    DH_E_NOMEM = -7,      /* device allocation failed                                    */
+    DH_E_HANDLE = -8,     /* unknown or wrong-kind handle                                */
+    DH_E_UNSUPPORTED = -9,/* not available in this build (e.g. no GPU)                    */
+    DH_E_BUDGET = -10,    /* request larger than the server's memory budget              */
+    DH_E_CAPACITY = -11,  /* scratch capacity could not be grown                         */
+};
+
+typedef struct dh_ctx_s* dh_ctx;
+typedef uint64_t dh_handle;
+
+/* Per-frame uniforms, computed on the host (reference a1 row): the product
+ * M = viewport*orth*persp*view of Scene::gen_transformation_matrix
+ * (scene/mod.rs:817-899), row-major; the six world-space planes of
+ * Camera::gen_view_planes (scene/mod.rs:481-593) in the order near, far,
+ * right, left, top, bottom, each (normal.xyz, k); camera position; light
+ * position; canvas size, offset (Canvas::apply_offset, canvas.rs:382-385),
+ * clear depth (Canvas::init_depth, canvas.rs:403) and clear colour
+ * (Pixel::azul_bb, canvas.rs:131-133) as B,G,R,A. */
+typedef struct {
+    float M[16];
+    float cam[3];
+    float light[3];
+    float planes[6][4];
+    int32_t W, H;
+    float off_x, off_y;
+    float clear_depth;
+    uint8_t clear_bgra[4];
+} dh_frame_params;
+
+/* Camera::new(pos, dir, ratio) (scene/mod.rs:297-357) with the three values
+ * it hard-codes made explicit: near (-10), far (near-500) and fov_x (135 deg).
+ * near/far keep the reference's sign convention: far < near < 0. */
+typedef struct {
+    float pos[3];
+    float dir[3];
+    float fov_x_deg;
+    float near_;
+    float far_;
+} dh_camera;
+
+/* Texture (material) of scene/mod.rs:206-216.  map_kd/map_ka: texture
+ * handles, 0 = TextureMap::default (1x1 white RGB, scene/mod.rs:128-135).
+ * power: the specular exponent canvas.rs:724 fixes at 2 (must be >= 0). */
+typedef struct {
+    float ka[3], kd[3], ks[3];
+    float alpha;
+    int32_t power;
+    dh_handle map_kd;
+    dh_handle map_ka;
+} dh_material_desc;
+
+/* Frame counters of the last render on a context. */
+typedef struct {
+    uint64_t tris_in;        /* input triangles submitted                       */
+    uint64_t tris_out;       /* clipped triangles emitted (opaque)              */
+    uint64_t bin_entries;    /* (tile, triangle) bin entries                    */
+    uint64_t frags;          /* covered fragments that reached the depth test   */
+    uint64_t retries;        /* capacity regrowths during the last render       */
+    /* Device time of the last render's final attempt (HIP events on the
+     * context stream; zero unless dh_set_profiling(ctx, 1)). */
+    float ms_geometry;       /* cull + clip + setup                             */
+    float ms_binning;        /* tile counts, pairs, radix sort, tile ranges     */
+    float ms_raster;         /* per-tile coverage + depth test                  */
+    float ms_shade;          /* deferred shading + framebuffer writes           */
+    float ms_frame;          /* whole frame                                     */
+} dh_stats;
+
+/* ---- host-side uniforms ---------------------------------------------- */
+/* Camera::new + Scene::gen_transformation_matrix + Camera::gen_view_planes
+ * (scene/mod.rs:297-357, 817-899, 481-593).  Pure host code; light may be
+ * NULL (then (0,300,300), scene/mod.rs:766).  Sets clear_depth 100000
+ * (app/mod.rs:80), clear colour azul_bb and offset 0. */
+int dh_frame_uniforms(const dh_camera* cam, int32_t W, int32_t H, const float light[3],
+                      dh_frame_params* out);
+
+/* ---- context --------------------------------------------------------- */
+int dh_ctx_create(int32_t device, dh_ctx* out);
+int dh_ctx_destroy(dh_ctx ctx);
+/* Human-readable text of the last error on this context (never NULL). */
+const char* dh_last_error(dh_ctx ctx);
+/* Raster tuning: tile size 8/16/32 px and waves per tile (1..16, divides
+ * (tile/8)^2).  Defaults 16 and 4. */
+int dh_set_tiling(dh_ctx ctx, int32_t tile, int32_t waves_per_tile);
+
+/* ---- resources (Scene::add_obj, scene/mod.rs:788-801; Object::new,
+ *      object.rs:34-71; TextureMap::new, scene/mod.rs:113-126) ---------- */
+/* img: h rows of w pixels, row 0 = top, comp = 3 or 4 bytes per pixel. */
+int dh_texture_create(dh_ctx ctx, const uint8_t* img, int32_t w, int32_t h, int32_t comp,
+                      dh_handle* out);
+int dh_material_create(dh_ctx ctx, const dh_material_desc* desc, dh_handle* out);
+/* Object vertex arrays: pos[3*n_pos], nrm[3*n_nrm], uv[2*n_uv]. */
+int dh_object_create(dh_ctx ctx, const float* pos, uint32_t n_pos, const float* nrm, uint32_t n_nrm,
+                     const float* uv, uint32_t n_uv, dh_handle* out);
+/* IndexedMesh of an object (scene/mesh.rs:30-35): tri[9*n_tri] = pos a,b,c,
+ * uv a,b,c, normal a,b,c.  Indices are validated here (DH_E_INDEX).
+ * material 0 = Texture::default (scene/mod.rs:1072-1075). */
+int dh_mesh_create(dh_ctx ctx, dh_handle object, const uint32_t* tri, uint32_t n_tri, dh_handle material,
+                   dh_handle* out);
+/* Device-resident Canvas (canvas.rs:353-433): BGRA8 frame + f32 depth +
+ * u32 winner id per pixel. */
+int dh_canvas_create(dh_ctx ctx, int32_t W, int32_t H, dh_handle* out);
+int dh_release(dh_ctx ctx, dh_handle h);
+
+/* ---- rendering -------------------------------------------------------- */
+/* Scene::render (scene/mod.rs:901-1249) of the given meshes, in submission
+ * order, into a device canvas.  Opaque meshes (material alpha >= 1) only;
+ * returns when the frame is complete on the device. */
+int dh_render(dh_ctx ctx, const dh_frame_params* fr, const dh_handle* meshes, uint32_t n_meshes,
+              dh_handle canvas);
+/* Copy a canvas to host memory: bgra W*H*4 bytes, rows top-down
+ * (Canvas::as_bytes_slice, canvas.rs:974-978); depth W*H floats, rows
+ * bottom-up (canvas.rs:415); winner W*H ids, rows bottom-up, 0xFFFFFFFF =
+ * clear.  Any pointer may be NULL. */
+int dh_canvas_read(dh_ctx ctx, dh_handle canvas, uint8_t* bgra, float* depth, uint32_t* winner);
+/* Device pointers of a canvas (for zero-copy consumers on the same device). */
+int dh_canvas_device_ptrs(dh_ctx ctx, dh_handle canvas, void** bgra, void** depth, void** winner);
+int dh_get_stats(dh_ctx ctx, dh_stats* out);
+/* Enable (1) or disable (0) per-stage HIP event timing of dh_render. */
+int dh_set_profiling(dh_ctx ctx, int32_t enable);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -472,6 +472,16 @@ static void put_pixel(const or_frame* fr, uint64_t x, uint64_t y, const uint8_t 
     }
 }
 
+/* Pixel sampling grid: only pixels with x % sx == ox and y % sy == oy are
+ * evaluated (1,1,0,0 = every pixel).  Each pixel's result depends only on the
+ * fragments at that pixel, so a sampled render is exact on its samples. */
+static uint32_t g_sx = 1, g_sy = 1, g_ox = 0, g_oy = 0;
+
+static uint64_t first_on_grid(uint64_t lo, uint32_t step, uint32_t off) {
+    uint64_t r = lo - (lo % step) + off;
+    return r < lo ? r + step : r;
+}
+
 int or_draw_triangle(const or_frame* fr, const or_attr* aa, const or_attr* ba, const or_attr* ca,
                      const or_material* mat, int depth_update, uint32_t id, uint8_t* bgra, float* depth,
                      uint32_t* winner, or_stats* st) {
@@ -491,9 +501,9 @@ int or_draw_triangle(const or_frame* fr, const or_attr* aa, const or_attr* ba, c
     float f_gama_out = edge_fn(a, b, -1.0f, -1.0f);
     int tie_a = f_alpha * f_alpha_out > 0.0f, tie_b = f_beta * f_beta_out > 0.0f,
         tie_g = f_gama * f_gama_out > 0.0f;
-    for (uint64_t y = yl; y <= yh; y++) {
+    for (uint64_t y = first_on_grid(yl, g_sy, g_oy); y <= yh; y += g_sy) {
         float yf = (float)y;
-        for (uint64_t x = xl; x <= xh; x++) {
+        for (uint64_t x = first_on_grid(xl, g_sx, g_ox); x <= xh; x += g_sx) {
             float xf = (float)x;
             if (st) st->px_tested++;
             float al = edge_fn(b, c, xf, yf) / f_alpha;
@@ -676,6 +686,16 @@ static int render_impl(const or_frame* fr, const or_object* objs, uint32_t n_obj
 int or_render(const or_frame* fr, const or_object* objs, uint32_t n_obj, const or_material* mats, uint32_t n_mat,
               uint8_t* bgra, float* depth, uint32_t* winner, or_stats* st) {
     return render_impl(fr, objs, n_obj, mats, n_mat, 0, UINT64_MAX, bgra, depth, winner, st);
+}
+
+int or_render_grid(const or_frame* fr, const or_object* objs, uint32_t n_obj, const or_material* mats,
+                   uint32_t n_mat, uint32_t sx, uint32_t sy, uint32_t ox, uint32_t oy, uint8_t* bgra, float* depth,
+                   uint32_t* winner, or_stats* st) {
+    if (sx == 0 || sy == 0 || ox >= sx || oy >= sy) return OR_E_ARG;
+    g_sx = sx; g_sy = sy; g_ox = ox; g_oy = oy;
+    int rc = render_impl(fr, objs, n_obj, mats, n_mat, 0, UINT64_MAX, bgra, depth, winner, st);
+    g_sx = 1; g_sy = 1; g_ox = 0; g_oy = 0;
+    return rc;
 }
 
 int or_render_range(const or_frame* fr, const or_object* objs, uint32_t n_obj, const or_material* mats,

@@ -124,6 +124,13 @@ int or_render_range(const or_frame* fr, const or_object* objs, uint32_t n_obj,
                     uint64_t tri_begin, uint64_t tri_end,
                     uint8_t* bgra, float* depth, uint32_t* winner, or_stats* st);
 
+/* or_render restricted to the pixels with x % sx == ox and y % sy == oy
+ * (other pixels keep the clear values).  Exact on the sampled pixels. */
+int or_render_grid(const or_frame* fr, const or_object* objs, uint32_t n_obj,
+                   const or_material* mats, uint32_t n_mat,
+                   uint32_t sx, uint32_t sy, uint32_t ox, uint32_t oy,
+                   uint8_t* bgra, float* depth, uint32_t* winner, or_stats* st);
+
 #ifdef __cplusplus
 }
 #endif

@@ -74,6 +74,9 @@ def lib():
         L.or_render_range.argtypes = [C.POINTER(OrFrame), C.POINTER(OrObject), C.c_uint32, C.POINTER(OrMaterial),
                                       C.c_uint32, C.c_uint64, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p,
                                       C.POINTER(OrStats)]
+        L.or_render_grid.argtypes = [C.POINTER(OrFrame), C.POINTER(OrObject), C.c_uint32, C.POINTER(OrMaterial),
+                                     C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p,
+                                     C.c_void_p, C.c_void_p, C.POINTER(OrStats)]
         L.or_vertex_visual.argtypes = [C.POINTER(C.c_float)] * 4
         L.or_vertex_visual.restype = None
         L.or_clip.argtypes = [C.POINTER(OrFrame), C.POINTER(C.c_float), C.POINTER(OrAttr), C.POINTER(C.c_float),
@@ -147,10 +150,11 @@ def _marshal(scene, keep):
     return obj_arr, len(objs), mat_arr, len(mats)
 
 
-def render(scene, frame: OrFrame | None = None, tri_range=None) -> OracleResult:
+def render(scene, frame: OrFrame | None = None, tri_range=None, grid=None) -> OracleResult:
     """Scene::render on the CPU.  ``frame`` defaults to the oracle's own a1
     uniforms; ``tri_range`` = (begin, end) restricts the opaque submission
-    stream (for bounded timing samples)."""
+    stream (for bounded timing samples); ``grid`` = (sx, sy, ox, oy) evaluates
+    only the pixels with x % sx == ox and y % sy == oy (exact there)."""
     fr = frame if frame is not None else frame_uniforms(scene)
     W, H = fr.W, fr.H
     bgra = np.empty((H, W, 4), np.uint8)
@@ -159,7 +163,10 @@ def render(scene, frame: OrFrame | None = None, tri_range=None) -> OracleResult:
     keep = []
     objs, n_obj, mats, n_mat = _marshal(scene, keep)
     st = OrStats()
-    if tri_range is None:
+    if grid is not None:
+        rc = lib().or_render_grid(C.byref(fr), objs, n_obj, mats, n_mat, *[int(g) for g in grid], bgra.ctypes.data,
+                                  depth.ctypes.data, winner.ctypes.data, C.byref(st))
+    elif tri_range is None:
         rc = lib().or_render(C.byref(fr), objs, n_obj, mats, n_mat, bgra.ctypes.data, depth.ctypes.data,
                              winner.ctypes.data, C.byref(st))
     else:
