@@ -98,7 +98,7 @@ struct dh_ctx_s {
   DevBuf<uint32_t> blk;
   DevBuf<dh::SetupRec> setup;
   DevBuf<dh::ShadeRec> shade;
-  DevBuf<uint32_t> bincnt, binoff;
+  DevBuf<uint32_t> bincnt, binoff, big;
   DevBuf<uint32_t> keys0, keys1, vals0, vals1;
   DevBuf<uint32_t> hist, partials;
   DevBuf<uint2> ranges;
@@ -174,6 +174,7 @@ int ensure_scratch(dh_ctx c, uint32_t n_in, uint32_t cap_tri, uint32_t cap_pairs
   HIPCHK(c, c->shade.ensure(cap_tri));
   HIPCHK(c, c->bincnt.ensure(cap_tri));
   HIPCHK(c, c->binoff.ensure(cap_tri));
+  HIPCHK(c, c->big.ensure(cap_tri));
   HIPCHK(c, c->keys0.ensure(cap_pairs));
   HIPCHK(c, c->keys1.ensure(cap_pairs));
   HIPCHK(c, c->vals0.ensure(cap_pairs));
@@ -237,7 +238,7 @@ extern "C" int dh_ctx_destroy(dh_ctx c) {
   }
   c->mat_dev.release();
   c->cnt.release(); c->blk.release(); c->setup.release(); c->shade.release();
-  c->bincnt.release(); c->binoff.release();
+  c->bincnt.release(); c->binoff.release(); c->big.release();
   c->keys0.release(); c->keys1.release(); c->vals0.release(); c->vals1.release();
   c->hist.release(); c->partials.release(); c->ranges.release(); c->vis.release();
   c->status.release(); c->draws.release(); c->views.release();
@@ -534,6 +535,7 @@ extern "C" int dh_render(dh_ctx c, const dh_frame_params* fr, const dh_handle* m
     a.cap_tri = c->cap_tri;
     a.bincnt = c->bincnt.p;
     a.binoff = c->binoff.p;
+    a.big = c->big.p;
     a.keys[0] = c->keys0.p;
     a.keys[1] = c->keys1.p;
     a.vals[0] = c->vals0.p;

@@ -23,7 +23,11 @@ enum : uint32_t {
   kTieA = 1u, kTieB = 2u, kTieG = 4u,  // f * f_outside > 0 (canvas.rs:678-680)
   kFast = 8u,                          // all normalizers finite and nonzero
   kDrop = 16u,                         // provably writes no pixel: not binned
+  kUvSafe = 32u,                       // every covered fragment's texel offset is in range
 };
+
+// Triangles with more bin entries than this are emitted by whole workgroups.
+constexpr uint32_t kBigEntries = 64;
 
 // Status words written by the kernels and read back once per frame.
 struct Status {
@@ -32,7 +36,8 @@ struct Status {
   uint32_t overflow;  // bit0: triangle records, bit1: bin entries
   uint32_t error;     // bit0: texture slice out of range (scene/mod.rs:167 panic)
   uint32_t frags;     // covered fragments submitted to the visibility test
-  uint32_t pad[3];
+  uint32_t n_big;     // triangles on the big-triangle bin list
+  uint32_t pad[2];
 };
 
 // Material = the reference's `Texture` (scene/mod.rs:206-216) with device maps.

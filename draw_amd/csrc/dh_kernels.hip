@@ -235,7 +235,8 @@ __device__ inline void rect_axis(uint64_t a0, uint64_t a1, uint64_t smax, uint32
 // tie-break products (660-666) and the scan rectangle (640-658).  Writes the
 // SetupRec and ShadeRec of one clipped triangle with 16-byte stores.
 __device__ inline void make_setup(const FrameParamsPod& fr, const FV& A, const FV& B, const FV& C, uint32_t material,
-                                  uint32_t view, SetupRec* __restrict__ rout, ShadeRec* __restrict__ sout) {
+                                  const DevMaterial& mat, uint32_t view, SetupRec* __restrict__ rout,
+                                  ShadeRec* __restrict__ sout) {
   const float* M = fr.M;
   float X[3], Y[3];
   const V3 P[3] = {A.p, B.p, C.p};
@@ -291,19 +292,31 @@ __device__ inline void make_setup(const FrameParamsPod& fr, const FV& A, const F
   // partial sums below 2^24), the three edge values sum to 0 at every pixel;
   // coverage would need all three > 0 (x/+0 = +inf), so no pixel is covered.
   if (isnan(fa) || isnan(fb) || isnan(fg)) flags |= kDrop;
-  if (__float_as_uint(fa) == 0u && __float_as_uint(fb) == 0u && __float_as_uint(fg) == 0u) {
-    double R = (double)x1 > (double)y1 ? (double)x1 : (double)y1;
+  // `exact`: every edge evaluation at |x|,|y| <= R (the vertices and the scan
+  // rectangle) has all partial sums below 2^24 in magnitude, so with integer
+  // operands each one is exact and e_bc + e_ca + e_ab = f_alpha = f_beta =
+  // f_gama (twice the signed area) at every pixel.
+  double R = (double)x1 > (double)y1 ? (double)x1 : (double)y1;
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-      R = fmax(R, fabs((double)X[k]));
-      R = fmax(R, fabs((double)Y[k]));
-    }
-    auto exact_edge = [R](const float e[4]) {
-      const double S = fabs((double)e[0]) * R + fabs((double)e[1]) * R + fabs((double)e[2]) + fabs((double)e[3]);
-      return S < 16777216.0;
-    };
-    if (isfinite(R) && exact_edge(ea) && exact_edge(eb) && exact_edge(eg)) flags |= kDrop;
+  for (int k = 0; k < 3; k++) {
+    R = fmax(R, fabs((double)X[k]));
+    R = fmax(R, fabs((double)Y[k]));
   }
+  auto exact_edge = [R](const float e[4]) {
+    const double S = fabs((double)e[0]) * R + fabs((double)e[1]) * R + fabs((double)e[2]) + fabs((double)e[3]);
+    return S < 16777216.0;
+  };
+  const bool exact = isfinite(R) && exact_edge(ea) && exact_edge(eb) && exact_edge(eg);
+  if (exact && __float_as_uint(fa) == 0u && __float_as_uint(fb) == 0u && __float_as_uint(fg) == 0u) flags |= kDrop;
+  // Texture-index safety.  With `exact`, a covered pixel has e_i / f >= 0 and
+  // sum e_i / f = 1, so alpha + beta + gama <= 1 + 2^-24 after rounding; with
+  // all six uv in [0,1] the interpolated u, v lie in [0, 1 + 5*2^-24], and for
+  // maps of at most 16384 texels per side floor(u*(w-1)) <= w-1 and
+  // floor(v*(h-1)) <= h-1: get_rgb_slice (scene/mod.rs:154-168) stays in range.
+  auto unit = [](float t) { return t >= 0.0f && t <= 1.0f; };
+  if ((flags & kFast) && exact && unit(A.u) && unit(A.v) && unit(B.u) && unit(B.v) && unit(C.u) && unit(C.v) &&
+      mat.kd_w <= 16384 && mat.kd_h <= 16384 && mat.ka_w <= 16384 && mat.ka_h <= 16384)
+    flags |= kUvSafe;
   float4* ro = reinterpret_cast<float4*>(rout);
   ro[0] = make_float4(ea[0], ea[1], ea[2], ea[3]);
   ro[1] = make_float4(eb[0], eb[1], eb[2], eb[3]);
@@ -351,7 +364,8 @@ __global__ __launch_bounds__(256) void k_geom_count(const DevDraw* __restrict__ 
 __global__ __launch_bounds__(256) void k_geom_emit(const DevDraw* __restrict__ draws, uint32_t n_draws,
                                                    const DevView* __restrict__ views, uint32_t n_in,
                                                    const uint8_t* __restrict__ cnt,
-                                                   const uint32_t* __restrict__ blk_off, SetupRec* __restrict__ setup,
+                                                   const uint32_t* __restrict__ blk_off,
+                                                   const DevMaterial* __restrict__ mats, SetupRec* __restrict__ setup,
                                                    ShadeRec* __restrict__ shade, uint32_t cap_tri,
                                                    Status* __restrict__ status) {
   __shared__ uint32_t s_tmp[4];
@@ -390,7 +404,7 @@ __global__ __launch_bounds__(256) void k_geom_emit(const DevDraw* __restrict__ d
       atomicOr(&status->overflow, 1u);
       return;
     }
-    make_setup(fr, A, B, C, material, view, setup + idx, shade + idx);
+    make_setup(fr, A, B, C, material, mats[material], view, setup + idx, shade + idx);
   });
 }
 
@@ -546,8 +560,9 @@ __device__ inline void spans_row(const Spans& g, int TS, int ty, int* lo, int* h
   *hi = h;
 }
 
-__global__ __launch_bounds__(256) void k_bin_count(const SetupRec* __restrict__ setup, const Status* status,
-                                                   uint32_t cap_tri, int TS, uint32_t* __restrict__ bincnt) {
+__global__ __launch_bounds__(256) void k_bin_count(const SetupRec* __restrict__ setup, Status* status,
+                                                   uint32_t cap_tri, int TS, uint32_t* __restrict__ bincnt,
+                                                   uint32_t* __restrict__ big) {
   const uint32_t n = min(status->n_out, cap_tri);
   for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
     const SetupRec r = setup[i];
@@ -562,15 +577,19 @@ __global__ __launch_bounds__(256) void k_bin_count(const SetupRec* __restrict__ 
       }
     }
     bincnt[i] = c;
+    if (c > kBigEntries) big[atomicAdd(&status->n_big, 1u)] = i;  // list order is irrelevant: offsets are fixed
   }
 }
 
+// Small triangles: one thread writes its entries at its scanned offset.
 __global__ __launch_bounds__(256) void k_bin_emit(const SetupRec* __restrict__ setup, const DevView* __restrict__ views,
                                                   const Status* status, uint32_t cap_tri, int TS,
+                                                  const uint32_t* __restrict__ bincnt,
                                                   const uint32_t* __restrict__ binoff, uint32_t cap_pairs,
                                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   const uint32_t n = min(status->n_out, cap_tri);
   for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+    if (bincnt[i] > kBigEntries) continue;
     const SetupRec r = setup[i];
     if (r.flags & kDrop) continue;
     const DevView& vw = views[r.view];
@@ -586,6 +605,53 @@ __global__ __launch_bounds__(256) void k_bin_emit(const SetupRec* __restrict__ s
           vals[o] = i;
         }
       }
+    }
+  }
+}
+
+// Big triangles: one workgroup per triangle; threads take tile rows, a block
+// scan places each row's span, and the entries are spread over all threads.
+__global__ __launch_bounds__(256) void k_bin_emit_big(const SetupRec* __restrict__ setup,
+                                                      const DevView* __restrict__ views, const Status* status,
+                                                      const uint32_t* __restrict__ big, int TS,
+                                                      const uint32_t* __restrict__ binoff, uint32_t cap_pairs,
+                                                      uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  __shared__ int s_lo[256];
+  __shared__ uint32_t s_off[256];
+  __shared__ uint32_t s_tmp[4];
+  const uint32_t nb = status->n_big;
+  for (uint32_t j = blockIdx.x; j < nb; j += gridDim.x) {
+    const uint32_t i = big[j];
+    const SetupRec r = setup[i];
+    const DevView& vw = views[r.view];
+    Spans g;
+    spans_init(r, TS, g);
+    uint32_t o = binoff[i];
+    for (int row0 = g.cy0; row0 <= g.cy1; row0 += 256) {
+      const int ty = row0 + (int)threadIdx.x;
+      int lo = 0, hi = -1;
+      if (ty <= g.cy1) spans_row(g, TS, ty, &lo, &hi);
+      const uint32_t c = hi >= lo ? (uint32_t)(hi - lo + 1) : 0u;
+      uint32_t total;
+      const uint32_t ex = block_exclusive_scan256(c, s_tmp, &total);
+      s_lo[threadIdx.x] = lo;
+      s_off[threadIdx.x] = ex;
+      __syncthreads();
+      for (uint32_t e = threadIdx.x; e < total; e += 256) {
+        // the row of entry e: the last row whose offset is <= e (rows without
+        // entries share their successor's offset, so the last one has entries)
+        int k = 0;
+#pragma unroll
+        for (int step = 128; step >= 1; step >>= 1)
+          if (s_off[k + step] <= e) k += step;
+        const uint32_t oo = o + e;
+        if (oo < cap_pairs) {
+          keys[oo] = vw.tile_base + (uint32_t)(row0 + k) * vw.ntx + (uint32_t)(s_lo[k] + (int)(e - s_off[k]));
+          vals[oo] = i;
+        }
+      }
+      o += total;
+      __syncthreads();
     }
   }
 }
@@ -716,6 +782,7 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
   constexpr int CH = 128;    // records staged per round
   constexpr int QCAP = 256;  // fragment queue per wave
   __shared__ SetupRec s_rec[CH];
+  __shared__ uint2 s_hdr[CH];  // scan rectangles, packed for the per-lane pre-filter
   __shared__ uint32_t s_id[CH];
   __shared__ unsigned long long s_vis[TS * TS];
   __shared__ uint32_t s_q[NW][QCAP];
@@ -734,12 +801,18 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
 
   const int lx = lane & 7, ly = lane >> 3;
   const uint64_t lanemask_lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  // The wave's quadrants form one rectangle: part of a quadrant row
+  // (QPW <= QS) or whole quadrant rows (QPW a multiple of QS).
+  const int wq0 = wave * QPW;
+  const int wx0 = tx0 + (wq0 % QS) * 8, wy0 = ty0 + (wq0 / QS) * 8;
+  const int wx1 = wx0 + (QPW < QS ? QPW : QS) * 8 - 1;
+  const int wy1 = wy0 + (QPW < QS ? 1 : QPW / QS) * 8 - 1;
   uint32_t qn = 0;
   uint32_t nfrag = 0;
 
   auto submit = [&](const SetupRec& r, uint32_t id, int px, int py) {
     const FragOut o = bary(r, (float)px, (float)py);
-    if (!uv_ok(shade, mats, id, o)) atomicOr(&status->error, 1u);
+    if (!(r.flags & kUvSafe) && !uv_ok(shade, mats, id, o)) atomicOr(&status->error, 1u);
     if (o.d < clear_depth) {
       const unsigned long long key = ((unsigned long long)depth_key(o.d) << 32) | id;
       atomicMin(&s_vis[(py - ty0) * TS + (px - tx0)], key);
@@ -765,11 +838,27 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
       s_id[i] = id;
       const uint4* src = reinterpret_cast<const uint4*>(setup + id);
       uint4* dst = reinterpret_cast<uint4*>(s_rec + i);
+      uint4 w[6];
 #pragma unroll
-      for (int w = 0; w < 6; w++) dst[w] = src[w];
+      for (int q = 0; q < 6; q++) w[q] = src[q];
+#pragma unroll
+      for (int q = 0; q < 6; q++) dst[q] = w[q];
+      s_hdr[i] = make_uint2(w[5].x, w[5].y);  // rx, ry
     }
     __syncthreads();
-    for (int k = 0; k < n; k++) {
+    // Each lane tests one staged record against the wave's pixel rectangle;
+    // the wave then visits the hits in list order.
+    for (int g0 = 0; g0 < n; g0 += 64) {
+      bool hit = false;
+      if (g0 + lane < n) {
+        const uint2 h = s_hdr[g0 + lane];
+        hit = !((int)(h.x >> 16) < wx0 || (int)(h.x & 0xFFFFu) > wx1 || (int)(h.y >> 16) < wy0 ||
+                (int)(h.y & 0xFFFFu) > wy1);
+      }
+      uint64_t hm = __ballot(hit);
+      while (hm) {
+      const int k = g0 + (int)__builtin_ctzll(hm);
+      hm &= hm - 1;
       const SetupRec& r = s_rec[k];
       const int rx0 = (int)(r.rx & 0xFFFFu), rx1 = (int)(r.rx >> 16);
       const int ry0 = (int)(r.ry & 0xFFFFu), ry1 = (int)(r.ry >> 16);
@@ -811,6 +900,7 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
           if (cov) submit(r, s_id[k], px, py);
           nfrag += (uint32_t)__popcll(__ballot(cov));
         }
+      }
       }
     }
     flush();
@@ -949,15 +1039,17 @@ void enqueue_frame(const PipeArgs& a, hipStream_t s) {
     scan_u32(a.blk, a.blk, nullptr, n_blk, n_blk, a.partials, &a.status->n_out, a.cap_tri, &a.status->overflow, 1u,
              s);
     hipLaunchKernelGGL(k_geom_emit, dim3(n_blk), dim3(256), 0, s, a.draws, a.n_draws, a.views, a.n_tri_in, a.cnt,
-                       a.blk, a.setup, a.shade, a.cap_tri, a.status);
+                       a.blk, a.mats, a.setup, a.shade, a.cap_tri, a.status);
   }
   if (a.ev[1]) (void)hipEventRecord(a.ev[1], s);
   // 4-6 binning
   const uint32_t gb = std::min<uint32_t>(cdiv(a.cap_tri, 256), 4096u);
-  hipLaunchKernelGGL(k_bin_count, dim3(gb), dim3(256), 0, s, a.setup, a.status, a.cap_tri, a.TS, a.bincnt);
+  hipLaunchKernelGGL(k_bin_count, dim3(gb), dim3(256), 0, s, a.setup, a.status, a.cap_tri, a.TS, a.bincnt, a.big);
   scan_u32(a.bincnt, a.binoff, &a.status->n_out, 0, a.cap_tri, a.partials, &a.status->n_pairs, a.cap_pairs,
            &a.status->overflow, 2u, s);
-  hipLaunchKernelGGL(k_bin_emit, dim3(gb), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri, a.TS, a.binoff,
+  hipLaunchKernelGGL(k_bin_emit, dim3(gb), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri, a.TS, a.bincnt,
+                     a.binoff, a.cap_pairs, a.keys[0], a.vals[0]);
+  hipLaunchKernelGGL(k_bin_emit_big, dim3(1024), dim3(256), 0, s, a.setup, a.views, a.status, a.big, a.TS, a.binoff,
                      a.cap_pairs, a.keys[0], a.vals[0]);
   // 7 stable radix sort by tile key
   int cur = 0;

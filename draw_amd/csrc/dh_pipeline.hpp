@@ -26,6 +26,7 @@ struct PipeArgs {
   uint32_t cap_tri;
   uint32_t* bincnt;    // [cap_tri]
   uint32_t* binoff;    // [cap_tri]
+  uint32_t* big;       // [cap_tri] triangles with more than kBigEntries bin entries
   uint32_t* keys[2];   // [cap_pairs] each
   uint32_t* vals[2];
   uint32_t cap_pairs;
