@@ -156,6 +156,34 @@ __device__ inline uint32_t depth_key(float d) {
   return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 
+// Views are stored in increasing tile_base / px_base order: the view owning a
+// global tile or pixel index is the last one whose base is <= it.
+__device__ inline uint32_t find_view_tile(const DevView* views, uint32_t n, uint32_t t) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (views[mid].tile_base <= t) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+__device__ inline uint32_t find_view_px(const DevView* views, uint32_t n, uint32_t p) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (views[mid].px_base <= p) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// XCD-aware block order: workgroups are dispatched round-robin over the 8
+// XCDs, so give each XCD one contiguous run of logical indices (neighbouring
+// tiles share triangles and their records then share one L2).
+__device__ inline uint32_t xcd_remap(uint32_t b, uint32_t nb) {
+  const uint32_t xcd = b & 7u, k = b >> 3;
+  const uint32_t per = nb >> 3, rem = nb & 7u;
+  return (xcd < rem) ? xcd * (per + 1u) + k : rem * (per + 1u) + (xcd - rem) * per + k;
+}
+
 // f32::powi with a non-negative exponent (binary exponentiation, low bit
 // first; power 2 is x*x as at canvas.rs:739).
 __host__ __device__ inline float powi_f(float x, int p) {

@@ -39,24 +39,6 @@ __device__ inline uint32_t find_draw(const DevDraw* draws, uint32_t n, uint32_t 
   return lo;
 }
 
-__device__ inline uint32_t find_view_tile(const DevView* views, uint32_t n, uint32_t t) {
-  uint32_t lo = 0, hi = n;
-  while (hi - lo > 1) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (views[mid].tile_base <= t) lo = mid; else hi = mid;
-  }
-  return lo;
-}
-
-__device__ inline uint32_t find_view_px(const DevView* views, uint32_t n, uint32_t p) {
-  uint32_t lo = 0, hi = n;
-  while (hi - lo > 1) {
-    uint32_t mid = (lo + hi) >> 1;
-    if (views[mid].px_base <= p) lo = mid; else hi = mid;
-  }
-  return lo;
-}
-
 __device__ inline V3 ld3(const float* a, uint32_t i) {
   const float* p = a + 3ull * i;
   return v3(p[0], p[1], p[2]);
@@ -83,15 +65,6 @@ __device__ inline uint32_t block_exclusive_scan256(uint32_t v, uint32_t* s_tmp /
   __syncthreads();
   *total = all;
   return before + x - v;
-}
-
-// XCD-aware block order: workgroups are dispatched round-robin over the 8
-// XCDs, so give each XCD one contiguous run of logical indices (neighbouring
-// tiles share triangles and their records then share one L2).
-__device__ inline uint32_t xcd_remap(uint32_t b, uint32_t nb) {
-  const uint32_t xcd = b & 7u, k = b >> 3;
-  const uint32_t per = nb >> 3, rem = nb & 7u;
-  return (xcd < rem) ? xcd * (per + 1u) + k : rem * (per + 1u) + (xcd - rem) * per + k;
 }
 
 // =================================================================== clipping
@@ -317,12 +290,25 @@ __device__ inline void make_setup(const FrameParamsPod& fr, const FV& A, const F
   if ((flags & kFast) && exact && unit(A.u) && unit(A.v) && unit(B.u) && unit(B.v) && unit(C.u) && unit(C.v) &&
       mat.kd_w <= 16384 && mat.kd_h <= 16384 && mat.ka_w <= 16384 && mat.ka_h <= 16384)
     flags |= kUvSafe;
+  // Depth lower bound for early rejection.  For a UV-safe triangle (exact
+  // edge arithmetic) a covered pixel has alpha + beta + gama >= 1 - 2^-24 with
+  // non-negative weights, so with non-negative vertex depths its depth is at
+  // least d_min * (1 - 2^-24)^4 > zlb = RN(d_min * (1 - 2^-20)).  A fragment
+  // may be skipped only when zlb is strictly above a depth already recorded at
+  // its pixel, and only on UV-safe triangles (whose texture reads cannot fail).
+  float zlb = -INFINITY;
+  {
+    const float dmin = fminf(fminf(A.d, B.d), C.d);
+    if ((flags & kUvSafe) && A.d >= 0.0f && B.d >= 0.0f && C.d >= 0.0f && dmin >= 1e-30f)
+      zlb = dmin * (1.0f - 9.5367431640625e-07f);
+    if ((flags & kUvSafe) && zlb >= fr.clear_depth) flags |= kDrop;  // no pixel can pass `<` the clear depth
+  }
   float4* ro = reinterpret_cast<float4*>(rout);
   ro[0] = make_float4(ea[0], ea[1], ea[2], ea[3]);
   ro[1] = make_float4(eb[0], eb[1], eb[2], eb[3]);
   ro[2] = make_float4(eg[0], eg[1], eg[2], eg[3]);
   ro[3] = make_float4(fa, fb, fg, 0.0f);
-  ro[4] = make_float4(A.d, B.d, C.d, 0.0f);
+  ro[4] = make_float4(A.d, B.d, C.d, zlb);
   ro[5] = make_float4(__uint_as_float(x0 | (x1 << 16)), __uint_as_float(y0 | (y1 << 16)), __uint_as_float(flags),
                       __uint_as_float(view));
   // shading attributes: uv[6], material, pad, n[9], l[9], h[9], pad
@@ -737,295 +723,8 @@ __global__ __launch_bounds__(256) void k_tile_ranges(const uint32_t* __restrict_
   }
 }
 
-// ==================================================================== raster
-// Coverage of one pixel for a FAST triangle, decided without division.  Edge
-// values are integers here (snapped vertices are floor() results and pixel
-// centres are integers), so for a finite nonzero normalizer f the quotient
-// e/f is NaN iff e is NaN, zero iff e is zero, and otherwise nonzero with the
-// sign of e*f (|e/f| >= 1/FLT_MAX is representable as a denormal).  Hence
-// alpha >= 0 && (alpha > 0 || tie) <=> e*f > 0 || (e == 0 && tie).
-__device__ inline bool covered_fast(float e, float f, bool tie) { return (e * f > 0.0f) || (e == 0.0f && tie); }
-
-struct FragOut {
-  float al, be, ga, d;
-};
-
-// Barycentrics and depth at a pixel, canvas.rs:673-682.
-__device__ inline FragOut bary(const SetupRec& r, float x, float y) {
-  FragOut o;
-  o.al = edge_eval(r.ea, x, y) / r.f[0];
-  o.be = edge_eval(r.eb, x, y) / r.f[1];
-  o.ga = edge_eval(r.eg, x, y) / r.f[2];
-  o.d = (o.al * r.d[0] + o.be * r.d[1]) + o.ga * r.d[2];
-  return o;
-}
-
-// The reference samples both maps of every covered fragment before its depth
-// test (canvas.rs:689-695), so an out-of-range texture coordinate fails the
-// frame even for a hidden fragment.
-__device__ inline bool uv_ok(const ShadeRec* __restrict__ shade, const DevMaterial* __restrict__ mats, uint32_t id,
-                             const FragOut& o) {
-  const ShadeRec& s = shade[id];
-  const float u = (s.uv[0] * o.al + s.uv[2] * o.be) + s.uv[4] * o.ga;
-  const float v = (s.uv[1] * o.al + s.uv[3] * o.be) + s.uv[5] * o.ga;
-  const DevMaterial& m = mats[s.material];
-  uint64_t off;
-  return rgb_offset(m.kd_w, m.kd_h, m.kd_comp, u, v, &off) && rgb_offset(m.ka_w, m.ka_h, m.ka_comp, u, v, &off);
-}
-
-template <int TS, int QPW>
-__global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
-    const DevView* __restrict__ views, uint32_t n_views, const uint2* __restrict__ ranges,
-    const uint32_t* __restrict__ vals, const SetupRec* __restrict__ setup, const ShadeRec* __restrict__ shade,
-    const DevMaterial* __restrict__ mats, uint32_t* __restrict__ vis_out, Status* __restrict__ status) {
-  constexpr int QS = TS / 8, NQ = QS * QS, NW = NQ / QPW, NT = 64 * NW;
-  constexpr int CH = 128;    // records staged per round
-  constexpr int QCAP = 256;  // fragment queue per wave
-  __shared__ SetupRec s_rec[CH];
-  __shared__ uint2 s_hdr[CH];  // scan rectangles, packed for the per-lane pre-filter
-  __shared__ uint32_t s_id[CH];
-  __shared__ unsigned long long s_vis[TS * TS];
-  __shared__ uint32_t s_q[NW][QCAP];
-
-  const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
-  const DevView& vw = views[find_view_tile(views, n_views, tile)];
-  const uint32_t lt = tile - vw.tile_base;
-  const int tx = (int)(lt % vw.ntx), ty = (int)(lt / vw.ntx);
-  const int W = vw.fr.W, H = vw.fr.H;
-  const float clear_depth = vw.fr.clear_depth;
-  const int tx0 = tx * TS, ty0 = ty * TS;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const uint2 rg = ranges[tile];
-
-  for (int p = tid; p < TS * TS; p += NT) s_vis[p] = ~0ull;
-
-  const int lx = lane & 7, ly = lane >> 3;
-  const uint64_t lanemask_lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  // The wave's quadrants form one rectangle: part of a quadrant row
-  // (QPW <= QS) or whole quadrant rows (QPW a multiple of QS).
-  const int wq0 = wave * QPW;
-  const int wx0 = tx0 + (wq0 % QS) * 8, wy0 = ty0 + (wq0 / QS) * 8;
-  const int wx1 = wx0 + (QPW < QS ? QPW : QS) * 8 - 1;
-  const int wy1 = wy0 + (QPW < QS ? 1 : QPW / QS) * 8 - 1;
-  uint32_t qn = 0;
-  uint32_t nfrag = 0;
-
-  auto submit = [&](const SetupRec& r, uint32_t id, int px, int py) {
-    const FragOut o = bary(r, (float)px, (float)py);
-    if (!(r.flags & kUvSafe) && !uv_ok(shade, mats, id, o)) atomicOr(&status->error, 1u);
-    if (o.d < clear_depth) {
-      const unsigned long long key = ((unsigned long long)depth_key(o.d) << 32) | id;
-      atomicMin(&s_vis[(py - ty0) * TS + (px - tx0)], key);
-    }
-  };
-  auto flush = [&]() {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // queue writes by other lanes of this wave
-    for (uint32_t j = (uint32_t)lane; j < qn; j += 64) {
-      const uint32_t q = s_q[wave][j];
-      const uint32_t k = q >> 16, lp = q & 0xFFFFu;
-      const int px = tx0 + (int)(lp % TS), py = ty0 + (int)(lp / TS);
-      submit(s_rec[k], s_id[k], px, py);
-    }
-    nfrag += qn;
-    qn = 0;
-  };
-
-  for (uint32_t base = rg.x; base < rg.y; base += CH) {
-    const int n = (int)min((uint32_t)CH, rg.y - base);
-    __syncthreads();
-    for (int i = tid; i < n; i += NT) {
-      const uint32_t id = vals[base + i];
-      s_id[i] = id;
-      const uint4* src = reinterpret_cast<const uint4*>(setup + id);
-      uint4* dst = reinterpret_cast<uint4*>(s_rec + i);
-      uint4 w[6];
-#pragma unroll
-      for (int q = 0; q < 6; q++) w[q] = src[q];
-#pragma unroll
-      for (int q = 0; q < 6; q++) dst[q] = w[q];
-      s_hdr[i] = make_uint2(w[5].x, w[5].y);  // rx, ry
-    }
-    __syncthreads();
-    // Each lane tests one staged record against the wave's pixel rectangle;
-    // the wave then visits the hits in list order.
-    for (int g0 = 0; g0 < n; g0 += 64) {
-      bool hit = false;
-      if (g0 + lane < n) {
-        const uint2 h = s_hdr[g0 + lane];
-        hit = !((int)(h.x >> 16) < wx0 || (int)(h.x & 0xFFFFu) > wx1 || (int)(h.y >> 16) < wy0 ||
-                (int)(h.y & 0xFFFFu) > wy1);
-      }
-      uint64_t hm = __ballot(hit);
-      while (hm) {
-      const int k = g0 + (int)__builtin_ctzll(hm);
-      hm &= hm - 1;
-      const SetupRec& r = s_rec[k];
-      const int rx0 = (int)(r.rx & 0xFFFFu), rx1 = (int)(r.rx >> 16);
-      const int ry0 = (int)(r.ry & 0xFFFFu), ry1 = (int)(r.ry >> 16);
-      const uint32_t flags = r.flags;
-#pragma unroll
-      for (int qq = 0; qq < QPW; qq++) {
-        const int Q = wave * QPW + qq;
-        const int qx0 = tx0 + (Q % QS) * 8, qy0 = ty0 + (Q / QS) * 8;
-        if (rx1 < qx0 || rx0 > qx0 + 7 || ry1 < qy0 || ry0 > qy0 + 7) continue;
-        const int px = qx0 + lx, py = qy0 + ly;
-        const bool in = px >= rx0 && px <= rx1 && py >= ry0 && py <= ry1;
-        const float xf = (float)px, yf = (float)py;
-        if (flags & kFast) {
-          bool cov = false;
-          if (in) {
-            const float e0 = edge_eval(r.ea, xf, yf);
-            const float e1 = edge_eval(r.eb, xf, yf);
-            const float e2 = edge_eval(r.eg, xf, yf);
-            cov = covered_fast(e0, r.f[0], flags & kTieA) && covered_fast(e1, r.f[1], flags & kTieB) &&
-                  covered_fast(e2, r.f[2], flags & kTieG);
-          }
-          const uint64_t m = __ballot(cov);
-          if (m) {
-            if (cov) {
-              const uint32_t pos = qn + (uint32_t)__popcll(m & lanemask_lt);
-              s_q[wave][pos] = ((uint32_t)k << 16) | (uint32_t)((py - ty0) * TS + (px - tx0));
-            }
-            qn += (uint32_t)__popcll(m);
-            if (qn > QCAP - 64) flush();
-          }
-        } else {
-          // Any other triangle: the reference's predicate on the divided values.
-          bool cov = false;
-          if (in) {
-            const FragOut o = bary(r, xf, yf);
-            cov = o.al >= 0.0f && o.be >= 0.0f && o.ga >= 0.0f && (o.al > 0.0f || (flags & kTieA)) &&
-                  (o.be > 0.0f || (flags & kTieB)) && (o.ga > 0.0f || (flags & kTieG));
-          }
-          if (cov) submit(r, s_id[k], px, py);
-          nfrag += (uint32_t)__popcll(__ballot(cov));
-        }
-      }
-      }
-    }
-    flush();
-  }
-  __syncthreads();
-  for (int p = tid; p < TS * TS; p += NT) {
-    const int px = tx0 + p % TS, py = ty0 + p / TS;
-    if (px < W && py < H) {
-      const unsigned long long v = s_vis[p];
-      vis_out[vw.px_base + (uint32_t)py * (uint32_t)W + (uint32_t)px] = (v == ~0ull) ? kNone : (uint32_t)v;
-    }
-  }
-  if (lane == 0) atomicAdd(&status->frags, nfrag);  // nfrag is wave-uniform
-}
-
-// ===================================================================== shade
-__device__ inline void fetch_rgb(const uint8_t* img, int32_t comp, uint64_t off, uint32_t c[3]) {
-  if (comp == 4) {
-    const uint32_t w = *reinterpret_cast<const uint32_t*>(img + off);
-    c[0] = w & 255u; c[1] = (w >> 8) & 255u; c[2] = (w >> 16) & 255u;
-  } else {
-    c[0] = img[off]; c[1] = img[off + 1]; c[2] = img[off + 2];
-  }
-}
-
-// Shading of a covered pixel, canvas.rs:685-741; returns B | G<<8 | R<<16 | A<<24.
-__device__ inline uint32_t shade_pixel(const ShadeRec& s, const DevMaterial& m, const FragOut& o, Status* status) {
-  const float u = (s.uv[0] * o.al + s.uv[2] * o.be) + s.uv[4] * o.ga;
-  const float v = (s.uv[1] * o.al + s.uv[3] * o.be) + s.uv[5] * o.ga;
-  uint64_t okd, oka;
-  const bool ok1 = rgb_offset(m.kd_w, m.kd_h, m.kd_comp, u, v, &okd);
-  const bool ok2 = rgb_offset(m.ka_w, m.ka_h, m.ka_comp, u, v, &oka);
-  if (!(ok1 && ok2)) {
-    atomicOr(&status->error, 1u);
-    okd = 0;
-    oka = 0;
-  }
-  uint32_t dk[3], ak[3];
-  fetch_rgb(m.kd_img, m.kd_comp, okd, dk);
-  fetch_rgb(m.ka_img, m.ka_comp, oka, ak);
-  const V3 diffuse = v3((float)dk[0] / 255.0f, (float)dk[1] / 255.0f, (float)dk[2] / 255.0f);
-  const V3 ambient = v3((float)ak[0] / 255.0f, (float)ak[1] / 255.0f, (float)ak[2] / 255.0f);
-  const V3 pn = add(add(muls(v3(s.n[0], s.n[1], s.n[2]), o.al), muls(v3(s.n[3], s.n[4], s.n[5]), o.be)),
-                    muls(v3(s.n[6], s.n[7], s.n[8]), o.ga));
-  const V3 pl = add(add(muls(v3(s.l[0], s.l[1], s.l[2]), o.al), muls(v3(s.l[3], s.l[4], s.l[5]), o.be)),
-                    muls(v3(s.l[6], s.l[7], s.l[8]), o.ga));
-  const V3 ph = add(add(muls(v3(s.h[0], s.h[1], s.h[2]), o.al), muls(v3(s.h[3], s.h[4], s.h[5]), o.be)),
-                    muls(v3(s.h[6], s.h[7], s.h[8]), o.ga));
-  const V3 c_l = v3(m.ks[0], m.ks[1], m.ks[2]);
-  const V3 c_r = v3(diffuse.x * m.kd[0], diffuse.y * m.kd[1], diffuse.z * m.kd[2]);
-  const V3 c_a = v3(ambient.x * m.ka[0], ambient.y * m.ka[1], ambient.z * m.ka[2]);
-  const float dln = dot(pl, pn);
-  const float mx = (dln > 0.0f) ? dln : 0.0f;  // 0.0_f32.max(x); the sign of a zero is irrelevant below
-  const float sfac = 1.0f - mx;
-  const V3 inner = add(c_a, muls(c_l, sfac));
-  const V3 t1 = v3(c_r.x * inner.x, c_r.y * inner.y, c_r.z * inner.z);
-  const float h2 = powi_f(dot(ph, pn), m.power);
-  const V3 col = add(t1, muls(c_l, h2));
-  const V3 sc = muls(col, 255.0f);
-  return f2u8(sc.z) | (f2u8(sc.y) << 8) | (f2u8(sc.x) << 16) | (255u << 24);
-}
-
-__global__ __launch_bounds__(256) void k_shade(const DevView* __restrict__ views, uint32_t n_views, uint32_t n_pixels,
-                                               const uint32_t* __restrict__ vis, const SetupRec* __restrict__ setup,
-                                               const ShadeRec* __restrict__ shade,
-                                               const DevMaterial* __restrict__ mats, Status* __restrict__ status) {
-  const uint32_t g = xcd_remap(blockIdx.x, gridDim.x) * 256u + threadIdx.x;
-  if (g >= n_pixels) return;
-  const DevView& vw = views[find_view_px(views, n_views, g)];
-  const uint32_t p = g - vw.px_base;
-  const uint32_t W = (uint32_t)vw.fr.W, H = (uint32_t)vw.fr.H;
-  const uint32_t y = p / W, x = p - y * W;
-  const uint32_t id = vis[g];
-  uint32_t bgra;
-  float depth;
-  if (id == kNone) {
-    const uint8_t* c = vw.fr.clear_bgra;
-    bgra = (uint32_t)c[0] | ((uint32_t)c[1] << 8) | ((uint32_t)c[2] << 16) | ((uint32_t)c[3] << 24);
-    depth = vw.fr.clear_depth;
-  } else {
-    const SetupRec r = setup[id];
-    const FragOut o = bary(r, (float)x, (float)y);
-    const ShadeRec& s = shade[id];
-    bgra = shade_pixel(s, mats[s.material], o, status);
-    depth = o.d;
-  }
-  vw.out_bgra[(H - 1u - y) * W + x] = bgra;  // draw_pixel_coord row flip, canvas.rs:955
-  vw.out_depth[p] = depth;
-  if (vw.out_winner) vw.out_winner[p] = id;
-}
-
 // ============================================================= orchestration
 static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
-
-template <int TS, int QPW>
-static void launch_raster(const PipeArgs& a, hipStream_t s) {
-  constexpr int NT = 64 * (TS / 8) * (TS / 8) / QPW;
-  hipLaunchKernelGGL((k_raster<TS, QPW>), dim3(a.n_tiles), dim3(NT), 0, s, a.views, a.n_views, a.ranges, a.vals[0],
-                     a.setup, a.shade, a.mats, a.vis, a.status);
-}
-
-bool raster_config_supported(int TS, int waves) {
-  const int nq = (TS / 8) * (TS / 8);
-  if (TS != 8 && TS != 16 && TS != 32) return false;
-  if (waves < 1 || nq % waves) return false;
-  const int qpw = nq / waves;
-  return qpw == 1 || qpw == 2 || qpw == 4 || qpw == 8 || qpw == 16;
-}
-
-static void dispatch_raster(const PipeArgs& a, hipStream_t s) {
-  const int nq = (a.TS / 8) * (a.TS / 8);
-  const int qpw = nq / a.waves;
-  if (a.TS == 8) launch_raster<8, 1>(a, s);
-  else if (a.TS == 16) {
-    if (qpw == 1) launch_raster<16, 1>(a, s);
-    else if (qpw == 2) launch_raster<16, 2>(a, s);
-    else launch_raster<16, 4>(a, s);
-  } else {
-    if (qpw == 1) launch_raster<32, 1>(a, s);
-    else if (qpw == 2) launch_raster<32, 2>(a, s);
-    else if (qpw == 4) launch_raster<32, 4>(a, s);
-    else if (qpw == 8) launch_raster<32, 8>(a, s);
-    else launch_raster<32, 16>(a, s);
-  }
-}
 
 void enqueue_frame(const PipeArgs& a, hipStream_t s) {
   if (a.ev[0]) (void)hipEventRecord(a.ev[0], s);
@@ -1070,11 +769,10 @@ void enqueue_frame(const PipeArgs& a, hipStream_t s) {
   hipLaunchKernelGGL(k_tile_ranges, dim3(std::min<uint32_t>(cdiv(a.cap_pairs, 256), 4096u)), dim3(256), 0, s, skeys,
                      &a.status->n_pairs, a.cap_pairs, a.ranges);
   if (a.ev[2]) (void)hipEventRecord(a.ev[2], s);
-  // 9 raster, 10 shade
-  dispatch_raster(b, s);
+  // 9 raster, 10 shade (dh_raster.hip)
+  enqueue_raster(b, s);
   if (a.ev[3]) (void)hipEventRecord(a.ev[3], s);
-  hipLaunchKernelGGL(k_shade, dim3(cdiv(a.n_pixels, 256)), dim3(256), 0, s, a.views, a.n_views, a.n_pixels, a.vis,
-                     a.setup, a.shade, a.mats, a.status);
+  enqueue_shade(a, s);
   if (a.ev[4]) (void)hipEventRecord(a.ev[4], s);
 }
 

@@ -40,5 +40,8 @@ struct PipeArgs {
 
 void enqueue_frame(const PipeArgs& a, hipStream_t s);
 bool raster_config_supported(int TS, int waves);
+// Per-pixel stages (dh_raster.hip): a.vals[0] must hold the sorted bin values.
+void enqueue_raster(const PipeArgs& a, hipStream_t s);
+void enqueue_shade(const PipeArgs& a, hipStream_t s);
 
 }  // namespace dh

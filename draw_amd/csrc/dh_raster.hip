@@ -1,0 +1,349 @@
+// dh_raster.hip -- the per-pixel stages of a frame on gfx950:
+//   k_raster  one workgroup per screen tile: coverage and the strict depth test
+//             of canvas.rs:668-682, 923 into an LDS visibility buffer of
+//             (depth key << 32 | triangle id), with hierarchical early-Z
+//   k_shade   deferred Phong + texture shading of each pixel's winner
+//             (canvas.rs:684-745), BGRA8 + f32 depth + winner id writes
+#include <math.h>
+
+#include "dh_common.hpp"
+#include "dh_pipeline.hpp"
+
+namespace dh {
+
+// Coverage of one pixel for a FAST triangle, decided without division.  Edge
+// values are integers here (snapped vertices are floor() results and pixel
+// centres are integers), so for a finite nonzero normalizer f the quotient
+// e/f is NaN iff e is NaN, zero iff e is zero, and otherwise nonzero with the
+// sign of e*f (|e/f| >= 1/FLT_MAX is representable as a denormal).  Hence
+// alpha >= 0 && (alpha > 0 || tie) <=> e*f > 0 || (e == 0 && tie).
+__device__ inline bool covered_fast(float e, float f, bool tie) { return (e * f > 0.0f) || (e == 0.0f && tie); }
+
+struct FragOut {
+  float al, be, ga, d;
+};
+
+// Barycentrics and depth at a pixel, canvas.rs:673-682.
+__device__ inline FragOut bary(const SetupRec& r, float x, float y) {
+  FragOut o;
+  o.al = edge_eval(r.ea, x, y) / r.f[0];
+  o.be = edge_eval(r.eb, x, y) / r.f[1];
+  o.ga = edge_eval(r.eg, x, y) / r.f[2];
+  o.d = (o.al * r.d[0] + o.be * r.d[1]) + o.ga * r.d[2];
+  return o;
+}
+
+// The reference samples both maps of every covered fragment before its depth
+// test (canvas.rs:689-695), so an out-of-range texture coordinate fails the
+// frame even for a hidden fragment.
+__device__ inline bool uv_ok(const ShadeRec* __restrict__ shade, const DevMaterial* __restrict__ mats, uint32_t id,
+                             const FragOut& o) {
+  const ShadeRec& s = shade[id];
+  const float u = (s.uv[0] * o.al + s.uv[2] * o.be) + s.uv[4] * o.ga;
+  const float v = (s.uv[1] * o.al + s.uv[3] * o.be) + s.uv[5] * o.ga;
+  const DevMaterial& m = mats[s.material];
+  uint64_t off;
+  return rgb_offset(m.kd_w, m.kd_h, m.kd_comp, u, v, &off) && rgb_offset(m.ka_w, m.ka_h, m.ka_comp, u, v, &off);
+}
+
+
+__device__ inline uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+  return v;
+}
+
+// Per-tile visibility.  The tile's bin list is in submission order; every
+// covered fragment with depth < clear_depth contributes its key
+// (depth_key(d) << 32 | id) to an LDS atomicMin, which leaves at each pixel the
+// first fragment of minimum depth -- the winner of the reference's sequential
+// strict `<` test.  Each wave owns QPW 8x8 quadrants of the tile:
+//  1. records are staged in LDS in chunks of CH;
+//  2. lanes pre-filter 64 records at once against the wave's rectangle and its
+//     largest recorded depth (hierarchical Z);
+//  3. the wave visits the hits in order; lanes test coverage of their own
+//     pixel without division (covered_fast) and against their pixel's recorded
+//     depth, and append covered fragments to a per-wave LDS queue;
+//  4. the queue is drained densely: divisions, depth, key, atomicMin.
+// Early-Z only drops a fragment whose depth lower bound is strictly above a
+// depth already recorded at its pixel, so it cannot be the minimum nor tie it.
+template <int TS, int QPW>
+__global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
+    const DevView* __restrict__ views, uint32_t n_views, const uint2* __restrict__ ranges,
+    const uint32_t* __restrict__ vals, const SetupRec* __restrict__ setup, const ShadeRec* __restrict__ shade,
+    const DevMaterial* __restrict__ mats, uint32_t* __restrict__ vis_out, Status* __restrict__ status) {
+  constexpr int QS = TS / 8, NQ = QS * QS, NW = NQ / QPW, NT = 64 * NW;
+  constexpr int CH = 128;    // records staged per round
+  constexpr int QCAP = 256;  // fragment queue per wave
+  __shared__ SetupRec s_rec[CH];
+  __shared__ uint4 s_hdr[CH];  // scan rectangle x, y; flags; depth-bound key
+  __shared__ uint32_t s_id[CH];
+  __shared__ unsigned long long s_vis[TS * TS];
+  __shared__ uint32_t s_q[NW][QCAP];
+
+  const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+  const DevView& vw = views[find_view_tile(views, n_views, tile)];
+  const uint32_t lt = tile - vw.tile_base;
+  const int tx = (int)(lt % vw.ntx), ty = (int)(lt / vw.ntx);
+  const int W = vw.fr.W, H = vw.fr.H;
+  const float clear_depth = vw.fr.clear_depth;
+  const int tx0 = tx * TS, ty0 = ty * TS;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint2 rg = ranges[tile];
+
+  for (int p = tid; p < TS * TS; p += NT) s_vis[p] = ~0ull;
+
+  const int lx = lane & 7, ly = lane >> 3;
+  const uint64_t lanemask_lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  // The wave's quadrants form one rectangle: part of a quadrant row
+  // (QPW <= QS) or whole quadrant rows (QPW a multiple of QS).
+  const int wq0 = wave * QPW;
+  const int wx0 = tx0 + (wq0 % QS) * 8, wy0 = ty0 + (wq0 / QS) * 8;
+  const int wx1 = wx0 + (QPW < QS ? QPW : QS) * 8 - 1;
+  const int wy1 = wy0 + (QPW < QS ? 1 : QPW / QS) * 8 - 1;
+  int lp[QPW];        // local index of the lane's pixel in each of its quadrants
+  uint32_t zk[QPW];   // recorded depth key at that pixel (0xFFFFFFFF: none yet)
+#pragma unroll
+  for (int qq = 0; qq < QPW; qq++) {
+    const int Q = wq0 + qq;
+    lp[qq] = ((Q / QS) * 8 + ly) * TS + (Q % QS) * 8 + lx;
+    zk[qq] = 0xFFFFFFFFu;
+  }
+  uint32_t wmax = 0xFFFFFFFFu;  // largest recorded depth key over the wave's pixels
+  uint32_t qn = 0;
+  uint32_t nfrag = 0;
+
+  auto submit = [&](const SetupRec& r, uint32_t id, int px, int py) {
+    const FragOut o = bary(r, (float)px, (float)py);
+    if (!(r.flags & kUvSafe) && !uv_ok(shade, mats, id, o)) atomicOr(&status->error, 1u);
+    if (o.d < clear_depth) {
+      const unsigned long long key = ((unsigned long long)depth_key(o.d) << 32) | id;
+      atomicMin(&s_vis[(py - ty0) * TS + (px - tx0)], key);
+    }
+  };
+  auto flush = [&]() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // queue writes by other lanes of this wave
+    for (uint32_t j = (uint32_t)lane; j < qn; j += 64) {
+      const uint32_t q = s_q[wave][j];
+      const uint32_t k = q >> 16, l = q & 0xFFFFu;
+      if (s_hdr[k].w > (uint32_t)(s_vis[l] >> 32)) continue;  // early-Z recheck
+      submit(s_rec[k], s_id[k], tx0 + (int)(l % TS), ty0 + (int)(l / TS));
+    }
+    nfrag += qn;
+    qn = 0;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    uint32_t m = 0;
+#pragma unroll
+    for (int qq = 0; qq < QPW; qq++) {
+      zk[qq] = (uint32_t)(s_vis[lp[qq]] >> 32);
+      m = max(m, zk[qq]);
+    }
+    wmax = wave_max_u32(m);
+  };
+
+  for (uint32_t base = rg.x; base < rg.y; base += CH) {
+    const int n = (int)min((uint32_t)CH, rg.y - base);
+    __syncthreads();
+    for (int i = tid; i < n; i += NT) {
+      const uint32_t id = vals[base + i];
+      s_id[i] = id;
+      const uint4* src = reinterpret_cast<const uint4*>(setup + id);
+      uint4* dst = reinterpret_cast<uint4*>(s_rec + i);
+      uint4 w[6];
+#pragma unroll
+      for (int q = 0; q < 6; q++) w[q] = src[q];
+#pragma unroll
+      for (int q = 0; q < 6; q++) dst[q] = w[q];
+      s_hdr[i] = make_uint4(w[5].x, w[5].y, w[5].z, depth_key(__uint_as_float(w[4].w)));
+    }
+    __syncthreads();
+    for (int g0 = 0; g0 < n; g0 += 64) {
+      bool hit = false;
+      if (g0 + lane < n) {
+        const uint4 h = s_hdr[g0 + lane];
+        hit = !((int)(h.x >> 16) < wx0 || (int)(h.x & 0xFFFFu) > wx1 || (int)(h.y >> 16) < wy0 ||
+                (int)(h.y & 0xFFFFu) > wy1) &&
+              !(h.w > wmax);
+      }
+      uint64_t hm = __ballot(hit);
+      while (hm) {
+        const int k = g0 + (int)__builtin_ctzll(hm);
+        hm &= hm - 1;
+        const SetupRec& r = s_rec[k];
+        const uint4 h = s_hdr[k];
+        const int rx0 = (int)__builtin_amdgcn_readfirstlane(h.x & 0xFFFFu);
+        const int rx1 = (int)__builtin_amdgcn_readfirstlane(h.x >> 16);
+        const int ry0 = (int)__builtin_amdgcn_readfirstlane(h.y & 0xFFFFu);
+        const int ry1 = (int)__builtin_amdgcn_readfirstlane(h.y >> 16);
+        const uint32_t flags = __builtin_amdgcn_readfirstlane(h.z);
+        const uint32_t zkey = __builtin_amdgcn_readfirstlane(h.w);
+#pragma unroll
+        for (int qq = 0; qq < QPW; qq++) {
+          const int Q = wq0 + qq;
+          const int qx0 = tx0 + (Q % QS) * 8, qy0 = ty0 + (Q / QS) * 8;
+          if (rx1 < qx0 || rx0 > qx0 + 7 || ry1 < qy0 || ry0 > qy0 + 7) continue;
+          const int px = qx0 + lx, py = qy0 + ly;
+          const bool in = px >= rx0 && px <= rx1 && py >= ry0 && py <= ry1;
+          const float xf = (float)px, yf = (float)py;
+          if (flags & kFast) {
+            bool cov = false;
+            if (in && !(zkey > zk[qq])) {
+              const float e0 = edge_eval(r.ea, xf, yf);
+              const float e1 = edge_eval(r.eb, xf, yf);
+              const float e2 = edge_eval(r.eg, xf, yf);
+              cov = covered_fast(e0, r.f[0], flags & kTieA) && covered_fast(e1, r.f[1], flags & kTieB) &&
+                    covered_fast(e2, r.f[2], flags & kTieG);
+            }
+            const uint64_t m = __ballot(cov);
+            if (m) {
+              if (cov) {
+                const uint32_t pos = qn + (uint32_t)__popcll(m & lanemask_lt);
+                s_q[wave][pos] = ((uint32_t)k << 16) | (uint32_t)lp[qq];
+              }
+              qn += (uint32_t)__popcll(m);
+              if (qn > QCAP - 64) flush();
+            }
+          } else {
+            // Any other triangle: the reference's predicate on the divided values.
+            bool cov = false;
+            if (in) {
+              const FragOut o = bary(r, xf, yf);
+              cov = o.al >= 0.0f && o.be >= 0.0f && o.ga >= 0.0f && (o.al > 0.0f || (flags & kTieA)) &&
+                    (o.be > 0.0f || (flags & kTieB)) && (o.ga > 0.0f || (flags & kTieG));
+            }
+            if (cov) submit(r, s_id[k], px, py);
+            nfrag += (uint32_t)__popcll(__ballot(cov));
+          }
+        }
+      }
+      if (qn) flush();
+    }
+  }
+  __syncthreads();
+  for (int p = tid; p < TS * TS; p += NT) {
+    const int px = tx0 + p % TS, py = ty0 + p / TS;
+    if (px < W && py < H) {
+      const unsigned long long v = s_vis[p];
+      vis_out[vw.px_base + (uint32_t)py * (uint32_t)W + (uint32_t)px] = (v == ~0ull) ? kNone : (uint32_t)v;
+    }
+  }
+  if (lane == 0) atomicAdd(&status->frags, nfrag);  // nfrag is wave-uniform
+}
+
+// ===================================================================== shade
+__device__ inline void fetch_rgb(const uint8_t* img, int32_t comp, uint64_t off, uint32_t c[3]) {
+  if (comp == 4) {
+    const uint32_t w = *reinterpret_cast<const uint32_t*>(img + off);
+    c[0] = w & 255u; c[1] = (w >> 8) & 255u; c[2] = (w >> 16) & 255u;
+  } else {
+    c[0] = img[off]; c[1] = img[off + 1]; c[2] = img[off + 2];
+  }
+}
+
+// Shading of a covered pixel, canvas.rs:685-741; returns B | G<<8 | R<<16 | A<<24.
+__device__ inline uint32_t shade_pixel(const ShadeRec& s, const DevMaterial& m, const FragOut& o, Status* status) {
+  const float u = (s.uv[0] * o.al + s.uv[2] * o.be) + s.uv[4] * o.ga;
+  const float v = (s.uv[1] * o.al + s.uv[3] * o.be) + s.uv[5] * o.ga;
+  uint64_t okd, oka;
+  const bool ok1 = rgb_offset(m.kd_w, m.kd_h, m.kd_comp, u, v, &okd);
+  const bool ok2 = rgb_offset(m.ka_w, m.ka_h, m.ka_comp, u, v, &oka);
+  if (!(ok1 && ok2)) {
+    atomicOr(&status->error, 1u);
+    okd = 0;
+    oka = 0;
+  }
+  uint32_t dk[3], ak[3];
+  fetch_rgb(m.kd_img, m.kd_comp, okd, dk);
+  fetch_rgb(m.ka_img, m.ka_comp, oka, ak);
+  const V3 diffuse = v3((float)dk[0] / 255.0f, (float)dk[1] / 255.0f, (float)dk[2] / 255.0f);
+  const V3 ambient = v3((float)ak[0] / 255.0f, (float)ak[1] / 255.0f, (float)ak[2] / 255.0f);
+  const V3 pn = add(add(muls(v3(s.n[0], s.n[1], s.n[2]), o.al), muls(v3(s.n[3], s.n[4], s.n[5]), o.be)),
+                    muls(v3(s.n[6], s.n[7], s.n[8]), o.ga));
+  const V3 pl = add(add(muls(v3(s.l[0], s.l[1], s.l[2]), o.al), muls(v3(s.l[3], s.l[4], s.l[5]), o.be)),
+                    muls(v3(s.l[6], s.l[7], s.l[8]), o.ga));
+  const V3 ph = add(add(muls(v3(s.h[0], s.h[1], s.h[2]), o.al), muls(v3(s.h[3], s.h[4], s.h[5]), o.be)),
+                    muls(v3(s.h[6], s.h[7], s.h[8]), o.ga));
+  const V3 c_l = v3(m.ks[0], m.ks[1], m.ks[2]);
+  const V3 c_r = v3(diffuse.x * m.kd[0], diffuse.y * m.kd[1], diffuse.z * m.kd[2]);
+  const V3 c_a = v3(ambient.x * m.ka[0], ambient.y * m.ka[1], ambient.z * m.ka[2]);
+  const float dln = dot(pl, pn);
+  const float mx = (dln > 0.0f) ? dln : 0.0f;  // 0.0_f32.max(x); the sign of a zero is irrelevant below
+  const float sfac = 1.0f - mx;
+  const V3 inner = add(c_a, muls(c_l, sfac));
+  const V3 t1 = v3(c_r.x * inner.x, c_r.y * inner.y, c_r.z * inner.z);
+  const float h2 = powi_f(dot(ph, pn), m.power);
+  const V3 col = add(t1, muls(c_l, h2));
+  const V3 sc = muls(col, 255.0f);
+  return f2u8(sc.z) | (f2u8(sc.y) << 8) | (f2u8(sc.x) << 16) | (255u << 24);
+}
+
+__global__ __launch_bounds__(256) void k_shade(const DevView* __restrict__ views, uint32_t n_views, uint32_t n_pixels,
+                                               const uint32_t* __restrict__ vis, const SetupRec* __restrict__ setup,
+                                               const ShadeRec* __restrict__ shade,
+                                               const DevMaterial* __restrict__ mats, Status* __restrict__ status) {
+  const uint32_t g = xcd_remap(blockIdx.x, gridDim.x) * 256u + threadIdx.x;
+  if (g >= n_pixels) return;
+  const DevView& vw = views[find_view_px(views, n_views, g)];
+  const uint32_t p = g - vw.px_base;
+  const uint32_t W = (uint32_t)vw.fr.W, H = (uint32_t)vw.fr.H;
+  const uint32_t y = p / W, x = p - y * W;
+  const uint32_t id = vis[g];
+  uint32_t bgra;
+  float depth;
+  if (id == kNone) {
+    const uint8_t* c = vw.fr.clear_bgra;
+    bgra = (uint32_t)c[0] | ((uint32_t)c[1] << 8) | ((uint32_t)c[2] << 16) | ((uint32_t)c[3] << 24);
+    depth = vw.fr.clear_depth;
+  } else {
+    const SetupRec r = setup[id];
+    const FragOut o = bary(r, (float)x, (float)y);
+    const ShadeRec& s = shade[id];
+    bgra = shade_pixel(s, mats[s.material], o, status);
+    depth = o.d;
+  }
+  vw.out_bgra[(H - 1u - y) * W + x] = bgra;  // draw_pixel_coord row flip, canvas.rs:955
+  vw.out_depth[p] = depth;
+  if (vw.out_winner) vw.out_winner[p] = id;
+}
+
+
+// ============================================================ launch helpers
+static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+template <int TS, int QPW>
+static void launch_raster(const PipeArgs& a, hipStream_t s) {
+  constexpr int NT = 64 * (TS / 8) * (TS / 8) / QPW;
+  hipLaunchKernelGGL((k_raster<TS, QPW>), dim3(a.n_tiles), dim3(NT), 0, s, a.views, a.n_views, a.ranges, a.vals[0],
+                     a.setup, a.shade, a.mats, a.vis, a.status);
+}
+
+bool raster_config_supported(int TS, int waves) {
+  if (TS != 8 && TS != 16 && TS != 32) return false;
+  const int nq = (TS / 8) * (TS / 8);
+  if (waves < 1 || nq % waves) return false;
+  const int qpw = nq / waves;
+  return qpw == 1 || qpw == 2 || qpw == 4 || qpw == 8 || qpw == 16;
+}
+
+void enqueue_raster(const PipeArgs& a, hipStream_t s) {
+  const int qpw = (a.TS / 8) * (a.TS / 8) / a.waves;
+  if (a.TS == 8) launch_raster<8, 1>(a, s);
+  else if (a.TS == 16) {
+    if (qpw == 1) launch_raster<16, 1>(a, s);
+    else if (qpw == 2) launch_raster<16, 2>(a, s);
+    else launch_raster<16, 4>(a, s);
+  } else {
+    if (qpw == 1) launch_raster<32, 1>(a, s);
+    else if (qpw == 2) launch_raster<32, 2>(a, s);
+    else if (qpw == 4) launch_raster<32, 4>(a, s);
+    else if (qpw == 8) launch_raster<32, 8>(a, s);
+    else launch_raster<32, 16>(a, s);
+  }
+}
+
+void enqueue_shade(const PipeArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_shade, dim3(cdiv(a.n_pixels, 256)), dim3(256), 0, s, a.views, a.n_views, a.n_pixels, a.vis,
+                     a.setup, a.shade, a.mats, a.status);
+}
+
+}  // namespace dh
