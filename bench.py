@@ -82,6 +82,8 @@ def main():
     ap.add_argument("--waves", type=int, default=4)
     ap.add_argument("--n-tri", type=int, default=1_000_000)
     ap.add_argument("--cpu-sample", type=int, default=400_000, help="0 disables the CPU baseline")
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="use --tile/--waves instead of the autotuner's choice")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); the path shards as described "
@@ -97,6 +99,12 @@ def main():
     W, H = fr.W, fr.H
     r = Renderer(0, tile=args.tile, waves=args.waves, profiling=True)
     meshes = r.mesh_handles(scene)
+    tune_table = None
+    if args.autotune:
+        from draw_amd.autotune import autotune
+        log("autotuning tile size / waves per tile ...")
+        (args.tile, args.waves), tune_table = autotune(r, lambda: r.render_device(scene, fr, meshes))
+        log(f"autotune: {tune_table} -> {args.tile}x{args.waves}")
     log("uploaded; warmup ...")
     for _ in range(args.warmup):
         r.render_device(scene, fr, meshes)
@@ -124,7 +132,7 @@ def main():
         "config": {"workload": "plain GPU path, 1920x1080, seeded 1M-triangle soup (20% near-plane straddlers, "
                                "log-normal area median 40 px), 2048x2048 RGBA8 texture, one frame per dh_render",
                    "width": W, "height": H, "triangles": scene.triangle_count, "tile": args.tile,
-                   "waves_per_tile": args.waves},
+                   "waves_per_tile": args.waves, "autotune_ms": tune_table},
         "tris_per_s": round(scene.triangle_count * args.steps / total, 1),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 4),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 4)},

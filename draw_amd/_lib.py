@@ -52,6 +52,7 @@ SIGNATURES = {
     "dh_last_error": [_P],
     "dh_set_tiling": [_P, C.c_int32, C.c_int32],
     "dh_set_profiling": [_P, C.c_int32],
+    "dh_set_bin_order": [_P, C.c_int32],
     "dh_texture_create": [_P, _P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(_H)],
     "dh_material_create": [_P, C.POINTER(DhMaterialDesc), C.POINTER(_H)],
     "dh_object_create": [_P, _P, C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, C.POINTER(_H)],

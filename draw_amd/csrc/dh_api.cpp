@@ -102,6 +102,8 @@ struct dh_ctx_s {
   DevBuf<uint32_t> keys0, keys1, vals0, vals1;
   DevBuf<uint32_t> hist, partials;
   DevBuf<uint2> ranges;
+  DevBuf<uint32_t> tile_cnt, tile_start;
+  bool ordered_bins = true;
   DevBuf<uint32_t> vis;
   DevBuf<dh::Status> status;
   DevBuf<dh::DevDraw> draws;
@@ -179,9 +181,11 @@ int ensure_scratch(dh_ctx c, uint32_t n_in, uint32_t cap_tri, uint32_t cap_pairs
   HIPCHK(c, c->keys1.ensure(cap_pairs));
   HIPCHK(c, c->vals0.ensure(cap_pairs));
   HIPCHK(c, c->vals1.ensure(cap_pairs));
-  HIPCHK(c, c->hist.ensure(256 * 512));
+  HIPCHK(c, c->hist.ensure(dh::kRadixHistEntries));
   HIPCHK(c, c->partials.ensure(1024));
   HIPCHK(c, c->ranges.ensure(n_tiles));
+  HIPCHK(c, c->tile_cnt.ensure(2ull * n_tiles));
+  HIPCHK(c, c->tile_start.ensure(n_tiles));
   HIPCHK(c, c->vis.ensure(n_pixels));
   HIPCHK(c, c->status.ensure(1));
   HIPCHK(c, c->draws.ensure(n_draws));
@@ -241,6 +245,7 @@ extern "C" int dh_ctx_destroy(dh_ctx c) {
   c->bincnt.release(); c->binoff.release(); c->big.release();
   c->keys0.release(); c->keys1.release(); c->vals0.release(); c->vals1.release();
   c->hist.release(); c->partials.release(); c->ranges.release(); c->vis.release();
+  c->tile_cnt.release(); c->tile_start.release();
   c->status.release(); c->draws.release(); c->views.release();
   if (c->white_rgb) hipFree(c->white_rgb);
   if (c->h_status) hipHostFree(c->h_status);
@@ -258,6 +263,12 @@ extern "C" int dh_set_tiling(dh_ctx c, int32_t tile, int32_t waves) {
   if (!dh::raster_config_supported(tile, waves)) return fail(c, DH_E_ARG, "unsupported tile/waves");
   c->TS = tile;
   c->waves = waves;
+  return DH_OK;
+}
+
+extern "C" int dh_set_bin_order(dh_ctx c, int32_t ordered) {
+  if (!c) return DH_E_ARG;
+  c->ordered_bins = ordered != 0;
   return DH_OK;
 }
 
@@ -536,6 +547,10 @@ extern "C" int dh_render(dh_ctx c, const dh_frame_params* fr, const dh_handle* m
     a.bincnt = c->bincnt.p;
     a.binoff = c->binoff.p;
     a.big = c->big.p;
+    a.ordered_bins = c->ordered_bins;
+    a.tile_cnt = c->tile_cnt.p;
+    a.tile_cur = c->tile_cnt.p + n_tiles;
+    a.tile_start = c->tile_start.p;
     a.keys[0] = c->keys0.p;
     a.keys[1] = c->keys1.p;
     a.vals[0] = c->vals0.p;

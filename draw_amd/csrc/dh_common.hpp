@@ -26,9 +26,6 @@ enum : uint32_t {
   kUvSafe = 32u,                       // every covered fragment's texel offset is in range
 };
 
-// Triangles with more bin entries than this are emitted by whole workgroups.
-constexpr uint32_t kBigEntries = 64;
-
 // Status words written by the kernels and read back once per frame.
 struct Status {
   uint32_t n_out;     // clipped triangles emitted (may exceed capacity)
@@ -36,7 +33,7 @@ struct Status {
   uint32_t overflow;  // bit0: triangle records, bit1: bin entries
   uint32_t error;     // bit0: texture slice out of range (scene/mod.rs:167 panic)
   uint32_t frags;     // covered fragments submitted to the visibility test
-  uint32_t n_big;     // triangles on the big-triangle bin list
+  uint32_t n_big;     // triangles on the wide-triangle bin list
   uint32_t pad[2];
 };
 

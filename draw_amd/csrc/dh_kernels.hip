@@ -546,166 +546,279 @@ __device__ inline void spans_row(const Spans& g, int TS, int ty, int* lo, int* h
   *hi = h;
 }
 
-__global__ __launch_bounds__(256) void k_bin_count(const SetupRec* __restrict__ setup, Status* status,
-                                                   uint32_t cap_tri, int TS, uint32_t* __restrict__ bincnt,
-                                                   uint32_t* __restrict__ big) {
+// Work split: a triangle whose tile range is more than kWideSpan tiles tall
+// or wide is binned by a whole wave (lanes over rows, then over entries); the
+// rest by one thread each (at most kWideSpan^2 entries).  The split only
+// balances work.
+constexpr int kWideSpan = 4;
+
+__device__ inline bool bin_is_wide(const Spans& g) {
+  return g.cy1 - g.cy0 + 1 > kWideSpan || g.cx1 - g.cx0 + 1 > kWideSpan;
+}
+
+__device__ inline uint64_t lanemask_lt_u64(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
+
+// Calls f(e, ty, tx) for every tile of the triangle, e numbering the tiles in
+// row-major order; returns the tile count.
+template <class F>
+__device__ inline uint32_t thread_tiles(const Spans& g, int TS, F f) {
+  uint32_t e = 0;
+  for (int ty = g.cy0; ty <= g.cy1; ty++) {
+    int lo, hi;
+    spans_row(g, TS, ty, &lo, &hi);
+    for (int tx = lo; tx <= hi; tx++, e++) f(e, ty, tx);
+  }
+  return e;
+}
+
+// Wave version: lanes take tile rows, a wave scan places each row's span and
+// the entries are spread over the lanes.  s_lo/s_off: 64-entry LDS scratch of
+// this wave.  Returns the tile count (wave-uniform).
+template <class F>
+__device__ inline uint32_t wave_tiles(const Spans& g, int TS, int lane, int* s_lo, uint32_t* s_off, F f) {
+  uint32_t o = 0;
+  for (int row0 = g.cy0; row0 <= g.cy1; row0 += 64) {
+    const int ty = row0 + lane;
+    int lo = 0, hi = -1;
+    if (ty <= g.cy1) spans_row(g, TS, ty, &lo, &hi);
+    const uint32_t c = hi >= lo ? (uint32_t)(hi - lo + 1) : 0u;
+    uint32_t x = c;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = (uint32_t)__shfl_up((int)x, d, 64);
+      if (lane >= d) x += y;
+    }
+    const uint32_t total = (uint32_t)__shfl((int)x, 63, 64);
+    s_lo[lane] = lo;
+    s_off[lane] = x - c;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    for (uint32_t e = (uint32_t)lane; e < total; e += 64) {
+      // the row of entry e: the last row whose offset is <= e (rows without
+      // entries share their successor's offset, so the last one has entries)
+      int k = 0;
+#pragma unroll
+      for (int step = 32; step >= 1; step >>= 1)
+        if (s_off[k + step] <= e) k += step;
+      f(o + e, row0 + k, s_lo[k] + (int)(e - s_off[k]));
+    }
+    o += total;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+  return o;
+}
+
+// Pass 1, one thread per clipped triangle: flag wide triangles (wflag, one
+// byte each) and count the tiles of the others -- per tile with atomics
+// (unordered bins) or per triangle (ordered bins).
+template <bool ORDERED>
+__global__ __launch_bounds__(256) void k_bin_count(const SetupRec* __restrict__ setup, const DevView* __restrict__ views,
+                                                   const Status* status, uint32_t cap_tri, int TS,
+                                                   uint32_t* __restrict__ bincnt, uint32_t* __restrict__ tile_cnt,
+                                                   uint8_t* __restrict__ wflag) {
   const uint32_t n = min(status->n_out, cap_tri);
   for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
     const SetupRec r = setup[i];
     uint32_t c = 0;
+    bool is_wide = false;
     if (!(r.flags & kDrop)) {
       Spans g;
       spans_init(r, TS, g);
-      for (int ty = g.cy0; ty <= g.cy1; ty++) {
-        int lo, hi;
-        spans_row(g, TS, ty, &lo, &hi);
-        if (hi >= lo) c += (uint32_t)(hi - lo + 1);
+      is_wide = bin_is_wide(g);
+      if (!is_wide) {
+        if (ORDERED) {
+          c = thread_tiles(g, TS, [](uint32_t, int, int) {});
+        } else {
+          const DevView& vw = views[r.view];
+          thread_tiles(g, TS, [&](uint32_t, int ty, int tx) {
+            atomicAdd(&tile_cnt[vw.tile_base + (uint32_t)ty * vw.ntx + (uint32_t)tx], 1u);
+          });
+        }
       }
     }
-    bincnt[i] = c;
-    if (c > kBigEntries) big[atomicAdd(&status->n_big, 1u)] = i;  // list order is irrelevant: offsets are fixed
+    wflag[i] = is_wide ? 1u : 0u;
+    if (ORDERED && !is_wide) bincnt[i] = c;
   }
 }
 
-// Small triangles: one thread writes its entries at its scanned offset.
+// Visits the flagged (wide) triangles one wave at a time: each wave reads the
+// flags of 64 consecutive triangles and processes the set ones in order.
+template <class F>
+__device__ inline void for_each_wide(const uint8_t* __restrict__ wflag, uint32_t n, int lane, F f) {
+  const uint32_t gw = (blockIdx.x * 256u + threadIdx.x) >> 6, nwaves = gridDim.x * 4u;
+  for (uint32_t base = gw * 64u; base < n; base += nwaves * 64u) {
+    const uint32_t i = base + (uint32_t)lane;
+    uint64_t m = __ballot(i < n && wflag[i] != 0);
+    while (m) {
+      const uint32_t k = base + (uint32_t)__builtin_ctzll(m);
+      m &= m - 1;
+      f(k);
+    }
+  }
+}
+
+template <bool ORDERED>
+__global__ __launch_bounds__(256) void k_bin_count_wide(const SetupRec* __restrict__ setup,
+                                                        const DevView* __restrict__ views, const Status* status,
+                                                        uint32_t cap_tri, int TS, const uint8_t* __restrict__ wflag,
+                                                        uint32_t* __restrict__ bincnt, uint32_t* __restrict__ tile_cnt) {
+  __shared__ int s_lo[4][64];
+  __shared__ uint32_t s_off[4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t n = min(status->n_out, cap_tri);
+  for_each_wide(wflag, n, lane, [&](uint32_t i) {
+    const SetupRec r = setup[i];
+    const DevView& vw = views[r.view];
+    Spans g;
+    spans_init(r, TS, g);
+    const uint32_t c = wave_tiles(g, TS, lane, s_lo[wave], s_off[wave], [&](uint32_t, int ty, int tx) {
+      if (!ORDERED) atomicAdd(&tile_cnt[vw.tile_base + (uint32_t)ty * vw.ntx + (uint32_t)tx], 1u);
+    });
+    if (ORDERED && lane == 0) bincnt[i] = c;
+  });
+}
+
+// Pass 2.  Ordered: (tile key, triangle) pairs at the triangle's scanned
+// offset, for the stable radix sort.  Unordered: the triangle id goes straight
+// into its tile's list at tile_start + atomicAdd(cursor).
+template <bool ORDERED>
 __global__ __launch_bounds__(256) void k_bin_emit(const SetupRec* __restrict__ setup, const DevView* __restrict__ views,
                                                   const Status* status, uint32_t cap_tri, int TS,
-                                                  const uint32_t* __restrict__ bincnt,
-                                                  const uint32_t* __restrict__ binoff, uint32_t cap_pairs,
+                                                  const uint8_t* __restrict__ wflag,
+                                                  const uint32_t* __restrict__ binoff,
+                                                  const uint32_t* __restrict__ tile_start,
+                                                  uint32_t* __restrict__ tile_cur, uint32_t cap_pairs,
                                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   const uint32_t n = min(status->n_out, cap_tri);
   for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
-    if (bincnt[i] > kBigEntries) continue;
+    if (wflag[i]) continue;
     const SetupRec r = setup[i];
     if (r.flags & kDrop) continue;
-    const DevView& vw = views[r.view];
-    uint32_t o = binoff[i];
     Spans g;
     spans_init(r, TS, g);
-    for (int ty = g.cy0; ty <= g.cy1; ty++) {
-      int lo, hi;
-      spans_row(g, TS, ty, &lo, &hi);
-      for (int tx = lo; tx <= hi; tx++, o++) {
-        if (o < cap_pairs) {
-          keys[o] = vw.tile_base + (uint32_t)ty * vw.ntx + (uint32_t)tx;
-          vals[o] = i;
-        }
+    const DevView& vw = views[r.view];
+    const uint32_t o = ORDERED ? binoff[i] : 0u;
+    thread_tiles(g, TS, [&](uint32_t e, int ty, int tx) {
+      const uint32_t key = vw.tile_base + (uint32_t)ty * vw.ntx + (uint32_t)tx;
+      const uint32_t pos = ORDERED ? o + e : tile_start[key] + atomicAdd(&tile_cur[key], 1u);
+      if (pos < cap_pairs) {
+        if (ORDERED) keys[pos] = key;
+        vals[pos] = i;
       }
-    }
+    });
   }
 }
 
-// Big triangles: one workgroup per triangle; threads take tile rows, a block
-// scan places each row's span, and the entries are spread over all threads.
-__global__ __launch_bounds__(256) void k_bin_emit_big(const SetupRec* __restrict__ setup,
-                                                      const DevView* __restrict__ views, const Status* status,
-                                                      const uint32_t* __restrict__ big, int TS,
-                                                      const uint32_t* __restrict__ binoff, uint32_t cap_pairs,
-                                                      uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-  __shared__ int s_lo[256];
-  __shared__ uint32_t s_off[256];
-  __shared__ uint32_t s_tmp[4];
-  const uint32_t nb = status->n_big;
-  for (uint32_t j = blockIdx.x; j < nb; j += gridDim.x) {
-    const uint32_t i = big[j];
+template <bool ORDERED>
+__global__ __launch_bounds__(256) void k_bin_emit_wide(const SetupRec* __restrict__ setup,
+                                                       const DevView* __restrict__ views, const Status* status,
+                                                       uint32_t cap_tri, const uint8_t* __restrict__ wflag, int TS,
+                                                       const uint32_t* __restrict__ binoff,
+                                                       const uint32_t* __restrict__ tile_start,
+                                                       uint32_t* __restrict__ tile_cur, uint32_t cap_pairs,
+                                                       uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  __shared__ int s_lo[4][64];
+  __shared__ uint32_t s_off[4][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t n = min(status->n_out, cap_tri);
+  for_each_wide(wflag, n, lane, [&](uint32_t i) {
     const SetupRec r = setup[i];
     const DevView& vw = views[r.view];
     Spans g;
     spans_init(r, TS, g);
-    uint32_t o = binoff[i];
-    for (int row0 = g.cy0; row0 <= g.cy1; row0 += 256) {
-      const int ty = row0 + (int)threadIdx.x;
-      int lo = 0, hi = -1;
-      if (ty <= g.cy1) spans_row(g, TS, ty, &lo, &hi);
-      const uint32_t c = hi >= lo ? (uint32_t)(hi - lo + 1) : 0u;
-      uint32_t total;
-      const uint32_t ex = block_exclusive_scan256(c, s_tmp, &total);
-      s_lo[threadIdx.x] = lo;
-      s_off[threadIdx.x] = ex;
-      __syncthreads();
-      for (uint32_t e = threadIdx.x; e < total; e += 256) {
-        // the row of entry e: the last row whose offset is <= e (rows without
-        // entries share their successor's offset, so the last one has entries)
-        int k = 0;
-#pragma unroll
-        for (int step = 128; step >= 1; step >>= 1)
-          if (s_off[k + step] <= e) k += step;
-        const uint32_t oo = o + e;
-        if (oo < cap_pairs) {
-          keys[oo] = vw.tile_base + (uint32_t)(row0 + k) * vw.ntx + (uint32_t)(s_lo[k] + (int)(e - s_off[k]));
-          vals[oo] = i;
-        }
+    const uint32_t o = ORDERED ? binoff[i] : 0u;
+    wave_tiles(g, TS, lane, s_lo[wave], s_off[wave], [&](uint32_t e, int ty, int tx) {
+      const uint32_t key = vw.tile_base + (uint32_t)ty * vw.ntx + (uint32_t)tx;
+      const uint32_t pos = ORDERED ? o + e : tile_start[key] + atomicAdd(&tile_cur[key], 1u);
+      if (pos < cap_pairs) {
+        if (ORDERED) keys[pos] = key;
+        vals[pos] = i;
       }
-      o += total;
-      __syncthreads();
-    }
+    });
+  });
+}
+
+__global__ __launch_bounds__(256) void k_ranges_from_counts(const uint32_t* __restrict__ tile_start,
+                                                            const uint32_t* __restrict__ tile_cnt, uint32_t n_tiles,
+                                                            uint32_t cap_pairs, uint2* __restrict__ ranges) {
+  for (uint32_t t = blockIdx.x * 256u + threadIdx.x; t < n_tiles; t += gridDim.x * 256u) {
+    const uint32_t s0 = min(tile_start[t], cap_pairs);
+    ranges[t] = make_uint2(s0, min(s0 + tile_cnt[t], cap_pairs));
   }
 }
 
 // ================================================================ radix sort
-// Stable LSD radix sort of (tile key, triangle id) pairs, 8-bit digits.  Block
-// g owns one contiguous chunk of the input and ranks its items in input order
-// (wave ballot match + per-wave digit counts), so equal keys keep the
-// triangle order.
-constexpr int kRadixG = 512;
+// Stable LSD radix sort of (tile key, triangle id) pairs with BITS-bit digits
+// (one pass when the tile key fits in 11 bits, else 8-bit passes).  Each wave
+// owns one contiguous sub-chunk of the input; k_radix_hist counts its digits,
+// the scan over (digit, wave) turns the counts into output bases, and
+// k_radix_scatter ranks the wave's items in input order (ballot match) and
+// advances its own LDS bases -- no block barrier inside the scatter.  Equal
+// keys keep the triangle order.
+constexpr int kRadixG = 256;               // blocks of 4 waves
+constexpr int kRadixW = kRadixG * 4;       // waves
+static_assert(2048u * kRadixW == kRadixHistEntries, "histogram size");
 
-__device__ inline uint32_t radix_chunk(uint32_t n) { return (((n + kRadixG - 1u) / kRadixG) + 255u) & ~255u; }
+__device__ inline uint32_t radix_chunk(uint32_t n) { return (((n + kRadixW - 1u) / kRadixW) + 63u) & ~63u; }
 
+template <int BITS>
 __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* __restrict__ keys, const uint32_t* n_ptr,
                                                     uint32_t cap, int shift, uint32_t* __restrict__ hist) {
-  __shared__ uint32_t h[256];
+  constexpr uint32_t ND = 1u << BITS;
+  __shared__ uint32_t h[4][ND];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t n = min(*n_ptr, cap);
   const uint32_t C = radix_chunk(n);
-  const uint32_t b0 = blockIdx.x * C, b1 = min(b0 + C, n);
-  h[threadIdx.x] = 0;
+  const uint32_t gw = blockIdx.x * 4u + (uint32_t)wave;
+  const uint32_t b0 = min(gw * C, n), b1 = min(b0 + C, n);
+  for (uint32_t d = (uint32_t)tid; d < ND; d += 256)
+#pragma unroll
+    for (int w = 0; w < 4; w++) h[w][d] = 0;
   __syncthreads();
-  for (uint32_t i = b0 + threadIdx.x; i < b1; i += 256) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+  for (uint32_t i = b0 + (uint32_t)lane; i < b1; i += 64) atomicAdd(&h[wave][(keys[i] >> shift) & (ND - 1u)], 1u);
   __syncthreads();
-  hist[threadIdx.x * kRadixG + blockIdx.x] = h[threadIdx.x];
+  for (uint32_t d = (uint32_t)tid; d < ND; d += 256)
+#pragma unroll
+    for (int w = 0; w < 4; w++) hist[d * kRadixW + blockIdx.x * 4u + (uint32_t)w] = h[w][d];
 }
 
+template <int BITS>
 __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                        uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                        const uint32_t* n_ptr, uint32_t cap, int shift,
                                                        const uint32_t* __restrict__ hist_off) {
-  __shared__ uint32_t running[256];
-  __shared__ uint32_t wcnt[4][256];
+  constexpr uint32_t ND = 1u << BITS;
+  __shared__ uint32_t run[4][ND];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t n = min(*n_ptr, cap);
   const uint32_t C = radix_chunk(n);
-  const uint32_t b0 = blockIdx.x * C, b1 = min(b0 + C, n);
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  running[tid] = hist_off[tid * kRadixG + blockIdx.x];
-#pragma unroll
-  for (int w = 0; w < 4; w++) wcnt[w][tid] = 0;
-  __syncthreads();
+  const uint32_t gw = blockIdx.x * 4u + (uint32_t)wave;
+  const uint32_t b0 = min(gw * C, n), b1 = min(b0 + C, n);
+  for (uint32_t d = (uint32_t)lane; d < ND; d += 64) run[wave][d] = hist_off[d * kRadixW + gw];
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  for (uint32_t tb = b0; tb < b1; tb += 256) {
-    const uint32_t i = tb + (uint32_t)tid;
+  for (uint32_t tb = b0; tb < b1; tb += 64) {
+    const uint32_t i = tb + (uint32_t)lane;
     const bool valid = i < b1;
     const uint32_t key = valid ? kin[i] : 0u;
     const uint32_t val = valid ? vin[i] : 0u;
-    const uint32_t dgt = (key >> shift) & 255u;
+    const uint32_t dgt = (key >> shift) & (ND - 1u);
     uint64_t peers = __ballot(valid);
 #pragma unroll
-    for (int bit = 0; bit < 8; bit++) {
+    for (int bit = 0; bit < BITS; bit++) {
       const bool on = (dgt >> bit) & 1u;
       const uint64_t m = __ballot(on);
       peers &= on ? m : ~m;
     }
     const uint32_t rank = (uint32_t)__popcll(peers & lt);
-    if (valid && rank == 0) wcnt[wave][dgt] = (uint32_t)__popcll(peers);
-    __syncthreads();
+    uint32_t base = 0;
+    if (valid) base = run[wave][dgt];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     if (valid) {
-      uint32_t pos = running[dgt] + rank;
-      for (int w = 0; w < wave; w++) pos += wcnt[w][dgt];
-      kout[pos] = key;
-      vout[pos] = val;
+      kout[base + rank] = key;
+      vout[base + rank] = val;
+      if (rank == 0) run[wave][dgt] = base + (uint32_t)__popcll(peers);
     }
-    __syncthreads();
-    running[tid] += wcnt[0][tid] + wcnt[1][tid] + wcnt[2][tid] + wcnt[3][tid];
-#pragma unroll
-    for (int w = 0; w < 4; w++) wcnt[w][tid] = 0;
-    __syncthreads();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   }
 }
 
@@ -741,33 +854,64 @@ void enqueue_frame(const PipeArgs& a, hipStream_t s) {
                        a.blk, a.mats, a.setup, a.shade, a.cap_tri, a.status);
   }
   if (a.ev[1]) (void)hipEventRecord(a.ev[1], s);
-  // 4-6 binning
+  // 4-8 binning
   const uint32_t gb = std::min<uint32_t>(cdiv(a.cap_tri, 256), 4096u);
-  hipLaunchKernelGGL(k_bin_count, dim3(gb), dim3(256), 0, s, a.setup, a.status, a.cap_tri, a.TS, a.bincnt, a.big);
-  scan_u32(a.bincnt, a.binoff, &a.status->n_out, 0, a.cap_tri, a.partials, &a.status->n_pairs, a.cap_pairs,
-           &a.status->overflow, 2u, s);
-  hipLaunchKernelGGL(k_bin_emit, dim3(gb), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri, a.TS, a.bincnt,
-                     a.binoff, a.cap_pairs, a.keys[0], a.vals[0]);
-  hipLaunchKernelGGL(k_bin_emit_big, dim3(1024), dim3(256), 0, s, a.setup, a.views, a.status, a.big, a.TS, a.binoff,
-                     a.cap_pairs, a.keys[0], a.vals[0]);
-  // 7 stable radix sort by tile key
-  int cur = 0;
-  for (int shift = 0; shift < a.key_bits; shift += 8) {
-    hipLaunchKernelGGL(k_radix_hist, dim3(kRadixG), dim3(256), 0, s, a.keys[cur], &a.status->n_pairs, a.cap_pairs,
-                       shift, a.hist);
-    scan_u32(a.hist, a.hist, nullptr, 256u * kRadixG, 256u * kRadixG, a.partials, nullptr, 0, nullptr, 0, s);
-    hipLaunchKernelGGL(k_radix_scatter, dim3(kRadixG), dim3(256), 0, s, a.keys[cur], a.vals[cur], a.keys[cur ^ 1],
-                       a.vals[cur ^ 1], &a.status->n_pairs, a.cap_pairs, shift, a.hist);
-    cur ^= 1;
-  }
-  const uint32_t* skeys = a.keys[cur];
+  uint8_t* wflag = reinterpret_cast<uint8_t*>(a.big);
   PipeArgs b = a;
-  b.vals[0] = a.vals[cur];
-  // 8 tile ranges
-  hipLaunchKernelGGL(k_fill_u32, dim3(std::min<uint32_t>(cdiv(2ull * a.n_tiles, 256), 2048u)), dim3(256), 0, s,
-                     reinterpret_cast<uint32_t*>(a.ranges), 0u, 2u * a.n_tiles);
-  hipLaunchKernelGGL(k_tile_ranges, dim3(std::min<uint32_t>(cdiv(a.cap_pairs, 256), 4096u)), dim3(256), 0, s, skeys,
-                     &a.status->n_pairs, a.cap_pairs, a.ranges);
+  if (a.ordered_bins) {
+    hipLaunchKernelGGL(k_bin_count<true>, dim3(gb), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri, a.TS,
+                       a.bincnt, a.tile_cnt, wflag);
+    hipLaunchKernelGGL(k_bin_count_wide<true>, dim3(1024), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri,
+                       a.TS, wflag, a.bincnt, a.tile_cnt);
+    scan_u32(a.bincnt, a.binoff, &a.status->n_out, 0, a.cap_tri, a.partials, &a.status->n_pairs, a.cap_pairs,
+             &a.status->overflow, 2u, s);
+    hipLaunchKernelGGL(k_bin_emit<true>, dim3(gb), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri, a.TS,
+                       wflag, a.binoff, a.tile_start, a.tile_cur, a.cap_pairs, a.keys[0], a.vals[0]);
+    hipLaunchKernelGGL(k_bin_emit_wide<true>, dim3(1024), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri,
+                       wflag, a.TS, a.binoff, a.tile_start, a.tile_cur, a.cap_pairs, a.keys[0], a.vals[0]);
+    // stable radix sort by tile key (one 11-bit pass when the key fits, else
+    // 8-bit passes), then the tile ranges
+    int cur = 0;
+    const bool one_pass = a.key_bits <= 11;
+    const int step = one_pass ? 11 : 8;
+    for (int shift = 0; shift < a.key_bits; shift += step) {
+      const uint32_t nh = (one_pass ? 2048u : 256u) * kRadixW;
+      if (one_pass)
+        hipLaunchKernelGGL(k_radix_hist<11>, dim3(kRadixG), dim3(256), 0, s, a.keys[cur], &a.status->n_pairs,
+                           a.cap_pairs, shift, a.hist);
+      else
+        hipLaunchKernelGGL(k_radix_hist<8>, dim3(kRadixG), dim3(256), 0, s, a.keys[cur], &a.status->n_pairs,
+                           a.cap_pairs, shift, a.hist);
+      scan_u32(a.hist, a.hist, nullptr, nh, nh, a.partials, nullptr, 0, nullptr, 0, s);
+      if (one_pass)
+        hipLaunchKernelGGL(k_radix_scatter<11>, dim3(kRadixG), dim3(256), 0, s, a.keys[cur], a.vals[cur],
+                           a.keys[cur ^ 1], a.vals[cur ^ 1], &a.status->n_pairs, a.cap_pairs, shift, a.hist);
+      else
+        hipLaunchKernelGGL(k_radix_scatter<8>, dim3(kRadixG), dim3(256), 0, s, a.keys[cur], a.vals[cur],
+                           a.keys[cur ^ 1], a.vals[cur ^ 1], &a.status->n_pairs, a.cap_pairs, shift, a.hist);
+      cur ^= 1;
+    }
+    b.vals[0] = a.vals[cur];
+    hipLaunchKernelGGL(k_fill_u32, dim3(std::min<uint32_t>(cdiv(2ull * a.n_tiles, 256), 2048u)), dim3(256), 0, s,
+                       reinterpret_cast<uint32_t*>(a.ranges), 0u, 2u * a.n_tiles);
+    hipLaunchKernelGGL(k_tile_ranges, dim3(std::min<uint32_t>(cdiv(a.cap_pairs, 256), 4096u)), dim3(256), 0, s,
+                       a.keys[cur], &a.status->n_pairs, a.cap_pairs, a.ranges);
+  } else {
+    hipLaunchKernelGGL(k_fill_u32, dim3(std::min<uint32_t>(cdiv(2ull * a.n_tiles, 256), 2048u)), dim3(256), 0, s,
+                       a.tile_cnt, 0u, 2u * a.n_tiles);  // tile_cnt and tile_cur are adjacent
+    hipLaunchKernelGGL(k_bin_count<false>, dim3(gb), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri, a.TS,
+                       a.bincnt, a.tile_cnt, wflag);
+    hipLaunchKernelGGL(k_bin_count_wide<false>, dim3(1024), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri,
+                       a.TS, wflag, a.bincnt, a.tile_cnt);
+    scan_u32(a.tile_cnt, a.tile_start, nullptr, a.n_tiles, a.n_tiles, a.partials, &a.status->n_pairs, a.cap_pairs,
+             &a.status->overflow, 2u, s);
+    hipLaunchKernelGGL(k_bin_emit<false>, dim3(gb), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri, a.TS,
+                       wflag, a.binoff, a.tile_start, a.tile_cur, a.cap_pairs, a.keys[0], a.vals[0]);
+    hipLaunchKernelGGL(k_bin_emit_wide<false>, dim3(1024), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri,
+                       wflag, a.TS, a.binoff, a.tile_start, a.tile_cur, a.cap_pairs, a.keys[0], a.vals[0]);
+    hipLaunchKernelGGL(k_ranges_from_counts, dim3(std::min<uint32_t>(cdiv(a.n_tiles, 256), 2048u)), dim3(256), 0, s,
+                       a.tile_start, a.tile_cnt, a.n_tiles, a.cap_pairs, a.ranges);
+  }
   if (a.ev[2]) (void)hipEventRecord(a.ev[2], s);
   // 9 raster, 10 shade (dh_raster.hip)
   enqueue_raster(b, s);

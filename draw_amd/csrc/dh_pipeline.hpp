@@ -5,6 +5,8 @@
 
 namespace dh {
 
+constexpr uint32_t kRadixHistEntries = 2048u * 1024u;  // (up to 11-bit) digits x radix-sort waves
+
 struct PipeArgs {
   // inputs
   const DevDraw* draws;
@@ -26,7 +28,11 @@ struct PipeArgs {
   uint32_t cap_tri;
   uint32_t* bincnt;    // [cap_tri]
   uint32_t* binoff;    // [cap_tri]
-  uint32_t* big;       // [cap_tri] triangles with more than kBigEntries bin entries
+  uint32_t* big;       // [cap_tri] used as one byte per triangle: 1 = wide, binned by a wave
+  bool ordered_bins;   // stable radix sort (per-tile lists in submission order)
+  uint32_t* tile_cnt;  // [2 * n_tiles] per-tile counts, then per-tile cursors
+  uint32_t* tile_cur;  // = tile_cnt + n_tiles
+  uint32_t* tile_start;  // [n_tiles] exclusive scan of tile_cnt
   uint32_t* keys[2];   // [cap_pairs] each
   uint32_t* vals[2];
   uint32_t cap_pairs;

@@ -77,6 +77,7 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
   constexpr int QCAP = 256;  // fragment queue per wave
   __shared__ SetupRec s_rec[CH];
   __shared__ uint4 s_hdr[CH];  // scan rectangle x, y; flags; depth-bound key
+  __shared__ float4 s_pre[CH][3];  // per edge: s*A, s*B, s*(C-D), |C|+|D| with s = sign(f)
   __shared__ uint32_t s_id[CH];
   __shared__ unsigned long long s_vis[TS * TS];
   __shared__ uint32_t s_q[NW][QCAP];
@@ -155,6 +156,14 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
 #pragma unroll
       for (int q = 0; q < 6; q++) dst[q] = w[q];
       s_hdr[i] = make_uint4(w[5].x, w[5].y, w[5].z, depth_key(__uint_as_float(w[4].w)));
+      const float fs[3] = {__uint_as_float(w[3].x), __uint_as_float(w[3].y), __uint_as_float(w[3].z)};
+#pragma unroll
+      for (int e = 0; e < 3; e++) {
+        const float sg = fs[e] > 0.0f ? 1.0f : -1.0f;
+        const float A = __uint_as_float(w[e].x), B = __uint_as_float(w[e].y);
+        const float C = __uint_as_float(w[e].z), D = __uint_as_float(w[e].w);
+        s_pre[i][e] = make_float4(sg * A, sg * B, sg * (C - D), fabsf(C) + fabsf(D));
+      }
     }
     __syncthreads();
     for (int g0 = 0; g0 < n; g0 += 64) {
@@ -164,6 +173,23 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
         hit = !((int)(h.x >> 16) < wx0 || (int)(h.x & 0xFFFFu) > wx1 || (int)(h.y >> 16) < wy0 ||
                 (int)(h.y & 0xFFFFu) > wy1) &&
               !(h.w > wmax);
+        if (hit && (h.z & kFast)) {
+          // Conservative edge test over the wave's rectangle: a pixel can be
+          // covered only if s*e >= 0 on every edge (s = sign of its
+          // normalizer).  With val the float value of s*(A x* + B y* + C - D)
+          // at the corner maximising it and S = |A| xmax + |B| ymax + |C| + |D|,
+          // every pixel p of the rectangle has s*e(p) <= val + 8.01 u S
+          // (u = 2^-24), so val + 2^-20 S + 1 < 0 rules the edge out.
+#pragma unroll
+          for (int e = 0; e < 3; e++) {
+            const float4 q = s_pre[g0 + lane][e];
+            const float xs = q.x > 0.0f ? (float)wx1 : (float)wx0;
+            const float ys = q.y > 0.0f ? (float)wy1 : (float)wy0;
+            const float val = (q.x * xs + q.y * ys) + q.z;
+            const float S = (fabsf(q.x) * (float)wx1 + fabsf(q.y) * (float)wy1) + q.w;
+            if (val + (9.5367431640625e-07f * S + 1.0f) < 0.0f) hit = false;
+          }
+        }
       }
       uint64_t hm = __ballot(hit);
       while (hm) {

@@ -70,6 +70,9 @@ class Renderer:
     def set_tiling(self, tile: int, waves: int):
         self._check(self.L.dh_set_tiling(self.ctx, int(tile), int(waves)), "dh_set_tiling")
 
+    def set_bin_order(self, ordered: bool):
+        self._check(self.L.dh_set_bin_order(self.ctx, int(bool(ordered))), "dh_set_bin_order")
+
     def set_profiling(self, on: bool):
         self._check(self.L.dh_set_profiling(self.ctx, int(bool(on))), "dh_set_profiling")
 

@@ -118,6 +118,11 @@ const char* dh_last_error(dh_ctx ctx);
 /* Raster tuning: tile size 8/16/32 px and waves per tile (1..16, divides
  * (tile/8)^2).  Defaults 16 and 4. */
 int dh_set_tiling(dh_ctx ctx, int32_t tile, int32_t waves_per_tile);
+/* Bin order: 1 (default) sorts each tile's triangle list stably so that it
+ * keeps the submission order; 0 builds the lists with atomics, in no
+ * particular order.  The frame is the same either way: each pixel keeps the
+ * minimum of (depth, submission index). */
+int dh_set_bin_order(dh_ctx ctx, int32_t ordered);
 
 /* ---- resources (Scene::add_obj, scene/mod.rs:788-801; Object::new,
  *      object.rs:34-71; TextureMap::new, scene/mod.rs:113-126) ---------- */
