@@ -73,6 +73,68 @@ def cpu_baseline(scene, n_sample: int, p_cov_full: int) -> dict:
     }
 
 
+def bench_batched(reps: int) -> dict:
+    """256 views (8 seeded meshes of 1k-500k triangles x 32 cameras, resolutions
+    uniform over 256x256 .. 1920x1080, 1024x1024 textures): the plain path
+    (one dh_render per view, each complete before the next) against one
+    batched launch replayed from a hipGraph.  Same views, same outputs."""
+    from draw_amd import workloads
+    from draw_amd.autotune import autotune
+    from draw_amd.render import Renderer, frame_uniforms
+
+    views = workloads.batched_scenes(seed=4)
+    items = [(s, frame_uniforms(s)) for s in views]
+    r = Renderer(0, profiling=True)
+    prepared = r.batch_views(items)
+    meshes = [r.mesh_handles(s) for s in views]
+    log("batched: autotune ...")
+    best, table = autotune(r, lambda: r.render_batch(items, prepared=prepared), frames=2)
+    # plain path: one frame per launch
+    for (s, fr), hs in zip(items[:8], meshes[:8]):
+        r.render_device(s, fr, hs)
+    lat = []
+    t0 = time.perf_counter()
+    for (s, fr), hs in zip(items, meshes):
+        a = time.perf_counter()
+        r.render_device(s, fr, hs)
+        lat.append(time.perf_counter() - a)
+    plain_s = time.perf_counter() - t0
+    # batched: one launch for all views, replayed from a captured graph
+    for _ in range(2):
+        r.render_batch(items, graph=True, prepared=prepared)
+    blat = []
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        a = time.perf_counter()
+        cvs = r.render_batch(items, graph=True, prepared=prepared)
+        blat.append(time.perf_counter() - a)
+    batch_s = (time.perf_counter() - t0) / reps
+    covered = 0
+    for cv, s in zip(cvs, views):
+        _, _, w = r.read(cv, s.width, s.height)
+        covered += int((w != 0xFFFFFFFF).sum())
+    tris = sum(s.triangle_count for s in views)
+    px = sum(s.width * s.height for s in views)
+    r.close()
+    plain_mpix = covered / plain_s / 1e6
+    batch_mpix = covered / batch_s / 1e6
+    return {
+        "workload": "256 views: 8 meshes (tori, UV spheres, soups; 1k-500k triangles log-uniform) x 32 random "
+                    "cameras, resolutions uniform over 256x256/640x480/1280x720/1920x1080, 1024x1024 textures",
+        "views": len(views), "triangles": tris, "pixels": px, "covered_px": covered,
+        "tile": best[0], "waves_per_tile": best[1], "autotune_ms": table,
+        "plain": {"shaded_mpix_per_s": round(plain_mpix, 1), "tris_per_s": round(tris / plain_s, 1),
+                  "frame_latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 4),
+                                       "p99": round(float(np.percentile(lat, 99)) * 1e3, 4)},
+                  "all_views_ms": round(plain_s * 1e3, 3)},
+        "batched": {"shaded_mpix_per_s": round(batch_mpix, 1), "tris_per_s": round(tris / batch_s, 1),
+                    "batch_latency_ms": {"p50": round(float(np.percentile(blat, 50)) * 1e3, 4),
+                                         "p99": round(float(np.percentile(blat, 99)) * 1e3, 4)},
+                    "all_views_ms": round(batch_s * 1e3, 3), "graph": True},
+        "batched_over_plain": round(batch_mpix / plain_mpix, 3),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +144,7 @@ def main():
     ap.add_argument("--waves", type=int, default=4)
     ap.add_argument("--n-tri", type=int, default=1_000_000)
     ap.add_argument("--cpu-sample", type=int, default=400_000, help="0 disables the CPU baseline")
+    ap.add_argument("--batched-reps", type=int, default=5, help="timed batched launches; 0 skips the batched run")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="use --tile/--waves instead of the autotuner's choice")
     args = ap.parse_args()
@@ -149,10 +212,12 @@ def main():
                             "unit": "GB/s", "frac": round(raster_gbs / HBM_PEAK_GBS, 4) if raster_gbs else None,
                             "algorithmic_bytes": rb},
     }
+    r.close()
+    if args.batched_reps > 0:
+        out["batched"] = bench_batched(args.batched_reps)
     if args.cpu_sample > 0:
         log(f"CPU baseline on {args.cpu_sample} triangles ...")
         out["cpu_baseline"] = cpu_baseline(scene, min(args.cpu_sample, scene.triangle_count), p_cov)
-    r.close()
     print(json.dumps(out), flush=True)
 
 

@@ -35,6 +35,14 @@ class DhMaterialDesc(C.Structure):
                 ("power", C.c_int32), ("map_kd", C.c_uint64), ("map_ka", C.c_uint64)]
 
 
+class DhView(C.Structure):
+    _fields_ = [("fr", DhFrameParams), ("canvas", C.c_uint64), ("meshes", C.POINTER(C.c_uint64)),
+                ("n_meshes", C.c_uint32), ("pad", C.c_uint32)]
+
+
+DH_BATCH_GRAPH = 1
+
+
 class DhStats(C.Structure):
     _fields_ = [("tris_in", C.c_uint64), ("tris_out", C.c_uint64), ("bin_entries", C.c_uint64),
                 ("frags", C.c_uint64), ("retries", C.c_uint64), ("ms_geometry", C.c_float),
@@ -60,6 +68,7 @@ SIGNATURES = {
     "dh_canvas_create": [_P, C.c_int32, C.c_int32, C.POINTER(_H)],
     "dh_release": [_P, _H],
     "dh_render": [_P, C.POINTER(DhFrameParams), C.POINTER(_H), C.c_uint32, _H],
+    "dh_render_batch": [_P, C.POINTER(DhView), C.c_uint32, C.c_uint32],
     "dh_canvas_read": [_P, _H, _P, _P, _P],
     "dh_canvas_device_ptrs": [_P, _H, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)],
     "dh_get_stats": [_P, C.POINTER(DhStats)],

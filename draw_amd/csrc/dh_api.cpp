@@ -114,6 +114,10 @@ struct dh_ctx_s {
   std::vector<dh::DevDraw> h_draws;
   std::vector<dh::DevView> h_views;
   dh_stats stats = {};
+  // captured batch launch (DH_BATCH_GRAPH)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<uint64_t> graph_key;
 };
 
 namespace {
@@ -228,6 +232,8 @@ extern "C" int dh_ctx_destroy(dh_ctx c) {
   if (!c) return DH_E_ARG;
   hipSetDevice(c->device);
   if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) hipGraphDestroy(c->graph);
   for (auto& kv : c->tex) hipFree(kv.second.d);
   for (auto& kv : c->objs) {
     hipFree(kv.second.pos);
@@ -445,73 +451,148 @@ extern "C" int dh_release(dh_ctx c, dh_handle h) {
 }
 
 // ---------------------------------------------------------------- rendering
-extern "C" int dh_render(dh_ctx c, const dh_frame_params* fr, const dh_handle* meshes, uint32_t n_meshes,
-                         dh_handle canvas) {
-  if (!c || !fr || (n_meshes && !meshes)) return DH_E_ARG;
-  if (fr->W <= 0 || fr->H <= 0) return fail(c, DH_E_ARG, "bad frame size");
-  if (handle_kind(canvas) != kCanvas) return fail(c, DH_E_HANDLE, "not a canvas handle");
-  auto ct = c->canvases.find(canvas & ((1ull << 56) - 1));
-  if (ct == c->canvases.end()) return fail(c, DH_E_HANDLE, "unknown canvas");
-  const CanvasRes& cv = ct->second;
-  if (cv.W != fr->W || cv.H != fr->H) return fail(c, DH_E_ARG, "canvas size differs from frame size");
-  HIPCHK(c, hipSetDevice(c->device));
+namespace {
 
-  // draw list in submission order (Scene::render's object/mesh loops)
+// Draw and view tables of a batch; key = everything a captured graph depends on.
+int build_tables(dh_ctx c, const dh_view* views, uint32_t n_views, uint64_t* n_in_out, uint32_t* n_tiles_out,
+                 uint32_t* n_pixels_out, std::vector<uint64_t>* key) {
   c->h_draws.clear();
-  uint64_t n_in = 0;
-  for (uint32_t i = 0; i < n_meshes; i++) {
-    if (handle_kind(meshes[i]) != kMesh) return fail(c, DH_E_HANDLE, "not a mesh handle");
-    auto mt = c->meshes.find(meshes[i] & ((1ull << 56) - 1));
-    if (mt == c->meshes.end()) return fail(c, DH_E_HANDLE, "unknown mesh");
-    const MeshRes& m = mt->second;
-    auto ot = c->objs.find(m.obj & ((1ull << 56) - 1));
-    if (ot == c->objs.end()) return fail(c, DH_E_HANDLE, "mesh object was released");
-    uint32_t slot = 0;
-    float alpha = 1.0f;
-    if (m.mat) {
-      auto it = c->mats.find(m.mat & ((1ull << 56) - 1));
-      if (it == c->mats.end()) return fail(c, DH_E_HANDLE, "mesh material was released");
-      slot = it->second.slot;
-      alpha = it->second.alpha;
+  c->h_views.clear();
+  key->clear();
+  key->push_back(n_views);
+  key->push_back((uint64_t)c->TS << 32 | (uint64_t)c->waves << 1 | (c->ordered_bins ? 1u : 0u));
+  uint64_t n_in = 0, n_tiles = 0, n_pixels = 0;
+  for (uint32_t v = 0; v < n_views; v++) {
+    const dh_view& vq = views[v];
+    const dh_frame_params* fr = &vq.fr;
+    if (fr->W <= 0 || fr->H <= 0 || (vq.n_meshes && !vq.meshes)) return fail(c, DH_E_ARG, "bad view");
+    if (handle_kind(vq.canvas) != kCanvas) return fail(c, DH_E_HANDLE, "not a canvas handle");
+    auto ct = c->canvases.find(vq.canvas & ((1ull << 56) - 1));
+    if (ct == c->canvases.end()) return fail(c, DH_E_HANDLE, "unknown canvas");
+    const CanvasRes& cv = ct->second;
+    if (cv.W != fr->W || cv.H != fr->H) return fail(c, DH_E_ARG, "canvas size differs from frame size");
+    dh::DevView dv;
+    memset(&dv, 0, sizeof(dv));
+    dv.fr = *fr;
+    dv.ntx = (uint32_t)((fr->W + c->TS - 1) / c->TS);
+    dv.nty = (uint32_t)((fr->H + c->TS - 1) / c->TS);
+    dv.tile_base = (uint32_t)n_tiles;
+    dv.px_base = (uint32_t)n_pixels;
+    dv.out_bgra = cv.bgra;
+    dv.out_depth = cv.depth;
+    dv.out_winner = cv.winner;
+    dv.tri_begin = (uint32_t)n_in;
+    dv.id_base = 0;
+    key->push_back((uint64_t)fr->W << 32 | (uint32_t)fr->H);
+    key->push_back(vq.canvas);
+    key->push_back(vq.n_meshes);
+    // meshes in submission order (Scene::render's object/mesh loops)
+    for (uint32_t i = 0; i < vq.n_meshes; i++) {
+      key->push_back(vq.meshes[i]);
+      if (handle_kind(vq.meshes[i]) != kMesh) return fail(c, DH_E_HANDLE, "not a mesh handle");
+      auto mt = c->meshes.find(vq.meshes[i] & ((1ull << 56) - 1));
+      if (mt == c->meshes.end()) return fail(c, DH_E_HANDLE, "unknown mesh");
+      const MeshRes& m = mt->second;
+      auto ot = c->objs.find(m.obj & ((1ull << 56) - 1));
+      if (ot == c->objs.end()) return fail(c, DH_E_HANDLE, "mesh object was released");
+      uint32_t slot = 0;
+      float alpha = 1.0f;
+      if (m.mat) {
+        auto it = c->mats.find(m.mat & ((1ull << 56) - 1));
+        if (it == c->mats.end()) return fail(c, DH_E_HANDLE, "mesh material was released");
+        slot = it->second.slot;
+        alpha = it->second.alpha;
+      }
+      if (alpha < 1.0f)
+        return fail(c, DH_E_UNSUPPORTED, "transparent meshes (alpha < 1) are not rendered by dh_render");
+      if (m.n_tri == 0) continue;
+      dh::DevDraw d;
+      d.pos = ot->second.pos;
+      d.nrm = ot->second.nrm;
+      d.uv = ot->second.uv;
+      d.tri = m.tri;
+      d.n_tri = m.n_tri;
+      d.tri_begin = (uint32_t)n_in;
+      d.material = slot;
+      d.view = v;
+      c->h_draws.push_back(d);
+      n_in += m.n_tri;
     }
-    if (alpha < 1.0f) return fail(c, DH_E_UNSUPPORTED, "transparent meshes (alpha < 1) are not rendered by dh_render");
-    if (m.n_tri == 0) continue;
-    dh::DevDraw d;
-    d.pos = ot->second.pos;
-    d.nrm = ot->second.nrm;
-    d.uv = ot->second.uv;
-    d.tri = m.tri;
-    d.n_tri = m.n_tri;
-    d.tri_begin = (uint32_t)n_in;
-    d.material = slot;
-    d.view = 0;
-    c->h_draws.push_back(d);
-    n_in += m.n_tri;
+    c->h_views.push_back(dv);
+    n_tiles += (uint64_t)dv.ntx * dv.nty;
+    n_pixels += (uint64_t)fr->W * (uint64_t)fr->H;
   }
-  if (n_in > 0x7FFFFFFFull) return fail(c, DH_E_ARG, "too many triangles in one frame");
+  if (n_in > 0x7FFFFFFFull) return fail(c, DH_E_ARG, "too many triangles in one launch");
+  if (n_tiles > 0x7FFFFFFFull || n_pixels > 0x7FFFFFFFull) return fail(c, DH_E_ARG, "too many pixels in one launch");
   if (c->h_draws.empty()) {  // keep the draw table non-empty
     dh::DevDraw d;
     memset(&d, 0, sizeof(d));
     c->h_draws.push_back(d);
   }
-  int rc = upload_materials(c);
+  *n_in_out = n_in;
+  *n_tiles_out = (uint32_t)n_tiles;
+  *n_pixels_out = (uint32_t)n_pixels;
+  return DH_OK;
+}
+
+dh::PipeArgs make_args(dh_ctx c, uint32_t n_in, uint32_t n_tiles, uint32_t n_pixels, bool events) {
+  dh::PipeArgs a;
+  memset(&a, 0, sizeof(a));
+  a.draws = c->draws.p;
+  a.n_draws = (uint32_t)c->h_draws.size();
+  a.n_tri_in = n_in;
+  a.views = c->views.p;
+  a.n_views = (uint32_t)c->h_views.size();
+  a.n_tiles = n_tiles;
+  a.n_pixels = n_pixels;
+  a.key_bits = key_bits_for(n_tiles);
+  a.mats = c->mat_dev.p;
+  a.TS = c->TS;
+  a.waves = c->waves;
+  a.cnt = c->cnt.p;
+  a.blk = c->blk.p;
+  a.setup = c->setup.p;
+  a.shade = c->shade.p;
+  a.cap_tri = c->cap_tri;
+  a.bincnt = c->bincnt.p;
+  a.binoff = c->binoff.p;
+  a.big = c->big.p;
+  a.ordered_bins = c->ordered_bins;
+  a.tile_cnt = c->tile_cnt.p;
+  a.tile_cur = c->tile_cnt.p + n_tiles;
+  a.tile_start = c->tile_start.p;
+  a.keys[0] = c->keys0.p;
+  a.keys[1] = c->keys1.p;
+  a.vals[0] = c->vals0.p;
+  a.vals[1] = c->vals1.p;
+  a.cap_pairs = c->cap_pairs;
+  a.hist = c->hist.p;
+  a.partials = c->partials.p;
+  a.ranges = c->ranges.p;
+  a.vis = c->vis.p;
+  a.status = c->status.p;
+  for (int i = 0; i < 5; i++) a.ev[i] = events ? c->ev[i] : nullptr;
+  return a;
+}
+
+void drop_graph(dh_ctx c) {
+  if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+  c->graph_key.clear();
+}
+
+int render_views(dh_ctx c, const dh_view* views, uint32_t n_views, bool use_graph) {
+  HIPCHK(c, hipSetDevice(c->device));
+  uint64_t n_in = 0;
+  uint32_t n_tiles = 0, n_pixels = 0;
+  std::vector<uint64_t> key;
+  int rc = build_tables(c, views, n_views, &n_in, &n_tiles, &n_pixels, &key);
   if (rc) return rc;
-
-  const int TS = c->TS;
-  dh::DevView v;
-  memset(&v, 0, sizeof(v));
-  v.fr = *fr;
-  v.ntx = (uint32_t)((fr->W + TS - 1) / TS);
-  v.nty = (uint32_t)((fr->H + TS - 1) / TS);
-  v.tile_base = 0;
-  v.px_base = 0;
-  v.out_bgra = cv.bgra;
-  v.out_depth = cv.depth;
-  v.out_winner = cv.winner;
-  c->h_views.assign(1, v);
-  const uint32_t n_tiles = v.ntx * v.nty;
-  const uint32_t n_pixels = (uint32_t)fr->W * (uint32_t)fr->H;
-
+  rc = upload_materials(c);
+  if (rc) return rc;
+  key.push_back((uint64_t)(uintptr_t)c->mat_dev.p);
   uint64_t want_tri = std::max<uint64_t>(c->cap_tri, n_in + n_in / 4 + 1024);
   uint64_t want_pairs = std::max<uint64_t>(c->cap_pairs, 4 * want_tri + n_tiles);
   c->stats = dh_stats{};
@@ -520,50 +601,29 @@ extern "C" int dh_render(dh_ctx c, const dh_frame_params* fr, const dh_handle* m
     want_tri = std::min<uint64_t>(want_tri, 0xFFFFFFF0ull);
     want_pairs = std::min<uint64_t>(want_pairs, 0xFFFFFFF0ull);
     rc = ensure_scratch(c, (uint32_t)n_in, (uint32_t)want_tri, (uint32_t)want_pairs, n_tiles, n_pixels,
-                        (uint32_t)c->h_draws.size(), 1);
+                        (uint32_t)c->h_draws.size(), (uint32_t)c->h_views.size());
     if (rc) return rc;
     HIPCHK(c, hipMemcpyAsync(c->draws.p, c->h_draws.data(), c->h_draws.size() * sizeof(dh::DevDraw),
                              hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->views.p, c->h_views.data(), sizeof(dh::DevView), hipMemcpyHostToDevice,
-                             c->stream));
-    dh::PipeArgs a;
-    memset(&a, 0, sizeof(a));
-    a.draws = c->draws.p;
-    a.n_draws = (uint32_t)c->h_draws.size();
-    a.n_tri_in = (uint32_t)n_in;
-    a.views = c->views.p;
-    a.n_views = 1;
-    a.n_tiles = n_tiles;
-    a.n_pixels = n_pixels;
-    a.key_bits = key_bits_for(n_tiles);
-    a.mats = c->mat_dev.p;
-    a.TS = TS;
-    a.waves = c->waves;
-    a.cnt = c->cnt.p;
-    a.blk = c->blk.p;
-    a.setup = c->setup.p;
-    a.shade = c->shade.p;
-    a.cap_tri = c->cap_tri;
-    a.bincnt = c->bincnt.p;
-    a.binoff = c->binoff.p;
-    a.big = c->big.p;
-    a.ordered_bins = c->ordered_bins;
-    a.tile_cnt = c->tile_cnt.p;
-    a.tile_cur = c->tile_cnt.p + n_tiles;
-    a.tile_start = c->tile_start.p;
-    a.keys[0] = c->keys0.p;
-    a.keys[1] = c->keys1.p;
-    a.vals[0] = c->vals0.p;
-    a.vals[1] = c->vals1.p;
-    a.cap_pairs = c->cap_pairs;
-    a.hist = c->hist.p;
-    a.partials = c->partials.p;
-    a.ranges = c->ranges.p;
-    a.vis = c->vis.p;
-    a.status = c->status.p;
-    for (int i = 0; i < 5; i++) a.ev[i] = c->profiling ? c->ev[i] : nullptr;
-    dh::enqueue_frame(a, c->stream);
-    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipMemcpyAsync(c->views.p, c->h_views.data(), c->h_views.size() * sizeof(dh::DevView),
+                             hipMemcpyHostToDevice, c->stream));
+    // everything a captured launch sequence bakes in
+    std::vector<uint64_t> gkey = key;
+    const void* ptrs[] = {c->cnt.p, c->blk.p, c->setup.p, c->shade.p, c->bincnt.p, c->binoff.p, c->big.p,
+                          c->keys0.p, c->keys1.p, c->vals0.p, c->vals1.p, c->hist.p, c->partials.p, c->ranges.p,
+                          c->tile_cnt.p, c->tile_start.p, c->vis.p, c->status.p, c->draws.p, c->views.p};
+    for (const void* p : ptrs) gkey.push_back((uint64_t)(uintptr_t)p);
+    gkey.push_back((uint64_t)c->cap_tri << 32 | c->cap_pairs);
+    gkey.push_back(n_in);
+    const bool replay = use_graph && c->graph_exec && gkey == c->graph_key;
+    if (replay) {
+      if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
+      HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
+      if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+    } else {
+      dh::enqueue_frame(make_args(c, (uint32_t)n_in, n_tiles, n_pixels, c->profiling), c->stream);
+      HIPCHK(c, hipGetLastError());
+    }
     HIPCHK(c, hipMemcpyAsync(c->h_status, c->status.p, sizeof(dh::Status), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const dh::Status st = *c->h_status;
@@ -576,21 +636,57 @@ extern "C" int dh_render(dh_ctx c, const dh_frame_params* fr, const dh_handle* m
       if (st.overflow & 2u) want_pairs = (uint64_t)st.n_pairs + st.n_pairs / 8 + 4096;
       // bin entries are only exact once the triangle records fit
       if (st.overflow & 1u) want_pairs = std::max<uint64_t>(want_pairs, 4 * want_tri + n_tiles);
+      drop_graph(c);
       continue;
     }
     if (c->profiling) {
-      float ms[4];
-      for (int i = 0; i < 4; i++) hipEventElapsedTime(&ms[i], c->ev[i], c->ev[i + 1]);
-      c->stats.ms_geometry = ms[0];
-      c->stats.ms_binning = ms[1];
-      c->stats.ms_raster = ms[2];
-      c->stats.ms_shade = ms[3];
+      if (replay) {
+        c->stats.ms_geometry = c->stats.ms_binning = c->stats.ms_raster = c->stats.ms_shade = 0.0f;
+      } else {
+        float ms[4];
+        for (int i = 0; i < 4; i++) hipEventElapsedTime(&ms[i], c->ev[i], c->ev[i + 1]);
+        c->stats.ms_geometry = ms[0];
+        c->stats.ms_binning = ms[1];
+        c->stats.ms_raster = ms[2];
+        c->stats.ms_shade = ms[3];
+      }
       hipEventElapsedTime(&c->stats.ms_frame, c->ev[0], c->ev[4]);
+    }
+    if (use_graph && !replay) {
+      // capture the same launch sequence (no stage events) for the next call
+      drop_graph(c);
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      dh::enqueue_frame(make_args(c, (uint32_t)n_in, n_tiles, n_pixels, false), c->stream);
+      hipGraph_t g = nullptr;
+      HIPCHK(c, hipStreamEndCapture(c->stream, &g));
+      c->graph = g;
+      HIPCHK(c, hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
+      c->graph_key = gkey;
     }
     if (st.error & 1u) return fail(c, DH_E_TEXCOORD, "texture coordinate out of range (get_rgb_slice would panic)");
     return DH_OK;
   }
   return fail(c, DH_E_CAPACITY, "scratch capacity did not converge");
+}
+
+}  // namespace
+
+extern "C" int dh_render(dh_ctx c, const dh_frame_params* fr, const dh_handle* meshes, uint32_t n_meshes,
+                         dh_handle canvas) {
+  if (!c || !fr || (n_meshes && !meshes)) return DH_E_ARG;
+  dh_view v;
+  memset(&v, 0, sizeof(v));
+  v.fr = *fr;
+  v.canvas = canvas;
+  v.meshes = meshes;
+  v.n_meshes = n_meshes;
+  return render_views(c, &v, 1, false);
+}
+
+extern "C" int dh_render_batch(dh_ctx c, const dh_view* views, uint32_t n_views, uint32_t flags) {
+  if (!c || (n_views && !views)) return DH_E_ARG;
+  if (n_views == 0) return DH_OK;
+  return render_views(c, views, n_views, (flags & DH_BATCH_GRAPH) != 0);
 }
 
 extern "C" int dh_canvas_read(dh_ctx c, dh_handle canvas, uint8_t* bgra, float* depth, uint32_t* winner) {

@@ -70,6 +70,8 @@ struct DevView {
   uint32_t* out_bgra;     // canvas frame, W*H BGRA8 words, rows top-down
   float* out_depth;       // canvas depth, W*H, rows bottom-up
   uint32_t* out_winner;   // canvas winner ids (may be null), rows bottom-up
+  uint32_t tri_begin;     // first input triangle of this view
+  uint32_t id_base;       // first clipped-triangle id of this view (set on the device)
 };
 
 // Raster record of one clipped triangle: the edge-function constants of the

@@ -394,6 +394,27 @@ __global__ __launch_bounds__(256) void k_geom_emit(const DevDraw* __restrict__ d
   });
 }
 
+// First clipped-triangle id of each view: the exclusive prefix of the clip
+// counts at the view's first input triangle (blk_off holds the scanned block
+// sums).  Shading subtracts it, so a view's winner ids do not depend on the
+// other views of a batch.
+__global__ void k_view_id_base(DevView* __restrict__ views, uint32_t n_views, uint32_t n_in,
+                               const uint8_t* __restrict__ cnt, const uint32_t* __restrict__ blk_off,
+                               const Status* __restrict__ status) {
+  const uint32_t v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n_views) return;
+  const uint32_t t0 = views[v].tri_begin;
+  uint32_t base;
+  if (t0 >= n_in) {
+    base = status->n_out;
+  } else {
+    const uint32_t b = t0 >> 8;
+    base = blk_off[b];
+    for (uint32_t t = b << 8; t < t0; t++) base += cnt[t];
+  }
+  views[v].id_base = base;
+}
+
 // ====================================================================== scan
 // Generic exclusive scan of u32 with the element count possibly on the device.
 constexpr int kScanG = 1024;  // partial sums per scan
@@ -853,6 +874,9 @@ void enqueue_frame(const PipeArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_geom_emit, dim3(n_blk), dim3(256), 0, s, a.draws, a.n_draws, a.views, a.n_tri_in, a.cnt,
                        a.blk, a.mats, a.setup, a.shade, a.cap_tri, a.status);
   }
+  if (a.n_views > 1)
+    hipLaunchKernelGGL(k_view_id_base, dim3(cdiv(a.n_views, 256)), dim3(256), 0, s, const_cast<DevView*>(a.views),
+                       a.n_views, a.n_tri_in, a.cnt, a.blk, a.status);
   if (a.ev[1]) (void)hipEventRecord(a.ev[1], s);
   // 4-8 binning
   const uint32_t gb = std::min<uint32_t>(cdiv(a.cap_tri, 256), 4096u);

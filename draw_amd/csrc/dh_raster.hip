@@ -329,7 +329,7 @@ __global__ __launch_bounds__(256) void k_shade(const DevView* __restrict__ views
   }
   vw.out_bgra[(H - 1u - y) * W + x] = bgra;  // draw_pixel_coord row flip, canvas.rs:955
   vw.out_depth[p] = depth;
-  if (vw.out_winner) vw.out_winner[p] = id;
+  if (vw.out_winner) vw.out_winner[p] = (id == kNone) ? kNone : id - vw.id_base;
 }
 
 

@@ -146,6 +146,40 @@ class Renderer:
         self._check(self.L.dh_render(self.ctx, C.byref(fr), arr, len(hs), cv), "dh_render")
         return cv
 
+    def canvas_slot(self, slot, W: int, H: int) -> int:
+        """A device canvas owned by one batch slot (views of a batch need
+        distinct canvases even when their sizes agree)."""
+        key = ("slot", slot, W, H)
+        if key not in self._canvas:
+            h = C.c_uint64()
+            self._check(self.L.dh_canvas_create(self.ctx, int(W), int(H), C.byref(h)), "dh_canvas_create")
+            self._canvas[key] = h.value
+        return self._canvas[key]
+
+    def batch_views(self, items, slot_base: int = 0):
+        """items: [(scene, frame or None)] -> (dh_view array, canvas handles)."""
+        views = (_lib.DhView * max(1, len(items)))()
+        canvases, keep = [], []
+        for k, (scene, frame) in enumerate(items):
+            fr = frame if frame is not None else frame_uniforms(scene)
+            hs = self.mesh_handles(scene)
+            arr = (C.c_uint64 * max(1, len(hs)))(*hs)
+            keep.append(arr)
+            cv = self.canvas_slot(slot_base + k, fr.W, fr.H)
+            canvases.append(cv)
+            views[k] = _lib.DhView(fr, cv, C.cast(arr, C.POINTER(C.c_uint64)), len(hs), 0)
+        return views, canvases, keep
+
+    def render_batch(self, items, graph: bool = False, prepared=None):
+        """Batched Scene::render of [(scene, frame or None)] in one pipeline
+        pass (dh_render_batch); returns the canvas handles, one per view.
+        ``prepared`` = the result of batch_views(items) to skip re-marshalling."""
+        views, canvases, keep = prepared if prepared is not None else self.batch_views(items)
+        flags = _lib.DH_BATCH_GRAPH if graph else 0
+        self._check(self.L.dh_render_batch(self.ctx, views, len(canvases), flags), "dh_render_batch")
+        del keep
+        return canvases
+
     def read(self, canvas_handle: int, W: int, H: int):
         bgra = np.empty((H, W, 4), np.uint8)
         depth = np.empty((H, W), np.float32)

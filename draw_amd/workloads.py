@@ -349,6 +349,24 @@ def mesh_pool(seed: int, n_meshes: int, tri_lo: int, tri_hi: int, tex_size: int 
     return out
 
 
+def batched_scenes(seed: int = 4, n_meshes: int = 8, cams_per_mesh: int = 32, tri_lo: int = 1000,
+                   tri_hi: int = 500_000, tex_size: int = 1024, resolutions=RESOLUTIONS) -> List[Scene]:
+    """Batched workload: ``n_meshes`` seeded meshes (tori, UV spheres and
+    soups with log-uniform triangle counts in [tri_lo, tri_hi], each with its
+    own tex_size^2 RGBA8 texture), each seen by ``cams_per_mesh`` random
+    cameras at a resolution drawn uniformly from ``resolutions``.  Returns one
+    Scene per view, views of a mesh sharing the same Object."""
+    rng = _rng(seed)
+    meshes = mesh_pool(seed + 1000, n_meshes, tri_lo, tri_hi, tex_size)
+    views: List[Scene] = []
+    for obj in meshes:
+        for _ in range(cams_per_mesh):
+            W, H = resolutions[int(rng.integers(len(resolutions)))]
+            cam = random_camera(rng)
+            views.append(Scene(W, H, cam, (0.0, 6.0, 6.0), [obj]))
+    return views
+
+
 def _ball_soup(n: int, radius: float, tex: Texture, rng: np.random.Generator) -> Object:
     """Soup of small triangles with centroids uniform in a ball."""
     c = rng.standard_normal((n, 3))

@@ -149,6 +149,27 @@ int dh_release(dh_ctx ctx, dh_handle h);
  * returns when the frame is complete on the device. */
 int dh_render(dh_ctx ctx, const dh_frame_params* fr, const dh_handle* meshes, uint32_t n_meshes,
               dh_handle canvas);
+/* One view of a batch: uniforms, target canvas (same size as fr), meshes in
+ * submission order. */
+typedef struct {
+    dh_frame_params fr;
+    dh_handle canvas;
+    const dh_handle* meshes;
+    uint32_t n_meshes;
+    uint32_t pad;
+} dh_view;
+
+enum { DH_BATCH_GRAPH = 1u };
+
+/* Batched Scene::render: n_views independent frames (each exactly what
+ * dh_render would produce for it alone, winner ids included) through one
+ * pipeline pass -- every kernel launches once over all views' triangles, tiles
+ * and pixels.  DH_BATCH_GRAPH captures that launch sequence in a hipGraph and
+ * replays it while the batch keeps the same structure (views, sizes, canvases,
+ * meshes); uniforms may change between replays.  No counterpart in the
+ * reference, which renders one frame per call (app/mod.rs:96-235). */
+int dh_render_batch(dh_ctx ctx, const dh_view* views, uint32_t n_views, uint32_t flags);
+
 /* Copy a canvas to host memory: bgra W*H*4 bytes, rows top-down
  * (Canvas::as_bytes_slice, canvas.rs:974-978); depth W*H floats, rows
  * bottom-up (canvas.rs:415); winner W*H ids, rows bottom-up, 0xFFFFFFFF =
