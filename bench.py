@@ -8,7 +8,8 @@ complete on the device before the next (one-frame-per-launch mode).  Inputs
 
 Prints ONE JSON line: shaded Mpix/s (pixels with a depth-test winner per
 second) as ``value``, plus triangles/s, p50/p99 frame latency, per-stage device
-times, the shade kernel's roofline (and the raster's), and the CPU oracle
+times, the rooflines of the raster (dominant) and shade kernels, the batched mode
+against the plain path on 256 views, and the CPU oracle
 timed on a bounded prefix of the same workload on this host.
 """
 from __future__ import annotations
@@ -32,23 +33,57 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def shade_bytes(winner: np.ndarray) -> tuple:
-    """Algorithmic HBM bytes of one k_shade launch: per pixel 4 B winner id read
-    + 4 B BGRA8 + 4 B depth + 4 B winner-id writes; per covered pixel 4 B diffuse
-    texel (the ambient map is 1x1); per distinct visible triangle its 96 B raster
-    record + 144 B shading record, read once."""
-    P = winner.size
-    cov = winner != 0xFFFFFFFF
-    p_cov = int(cov.sum())
-    t_vis = int(np.unique(winner[cov]).size)
-    return 16 * P + 4 * p_cov + 240 * t_vis, p_cov, t_vis
+def visible_triangles(winner: np.ndarray) -> tuple:
+    """(covered pixels, distinct winning triangles) of one canvas."""
+    w = winner.ravel()
+    cov = w[w != 0xFFFFFFFF]
+    if cov.size == 0:
+        return 0, 0
+    return int(cov.size), int(np.count_nonzero(np.bincount(cov)))
 
 
-def raster_bytes(P: int, entries: int) -> int:
-    """Algorithmic HBM bytes of one k_raster launch: per bin entry its 4 B
-    triangle id + 96 B raster record, per pixel the 4 B winner id written, per
-    tile the 8 B list range."""
-    return 100 * entries + 4 * P
+def shade_bytes(P: int, p_cov: int, t_vis: int) -> dict:
+    """Algorithmic HBM bytes of one k_shade launch: per pixel a 4 B winner id
+    read and a 4 B BGRA8 write; per covered pixel the 4 B diffuse texel (the
+    ambient map is 1x1); per visible triangle 3 vertices x 14 f32 (normal,
+    light, halfway, uv, depth).  ``record`` is what this implementation moves:
+    it also writes f32 depth and the winner id per pixel and reads a 96 B raster
+    record plus a 144 B shading record per visible triangle."""
+    return {"algorithmic": 8 * P + 4 * p_cov + 168 * t_vis, "record": 16 * P + 4 * p_cov + 240 * t_vis}
+
+
+def raster_bytes(P: int, entries: int) -> dict:
+    """Algorithmic HBM bytes of one k_raster launch: per pixel an f32 depth and
+    a u32 id written; per bin entry a 4 B id and a 48 B setup record read (6
+    snapped coordinates, 3 depths, 3 normalizers).  ``record``: this
+    implementation writes only the 4 B winner per pixel and reads a 96 B record
+    per entry."""
+    return {"algorithmic": 8 * P + 52 * entries, "record": 4 * P + 100 * entries}
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of ``kernel`` on this workload from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE per dispatch, each
+    scaled by the same counter's reading for a device copy of known size), or
+    None when that summary has no line for the kernel."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_soup1m_1080p.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 4 and cells[0] == f"`{kernel}`":
+            try:
+                return int(float(cells[3]))
+            except ValueError:
+                return None
+    return None
+
+
+def roofline(kernel: str, nbytes: dict, ms: float, traffic=None) -> dict:
+    gbs = nbytes["algorithmic"] / (ms * 1e-3) / 1e9 if ms > 0 else None
+    return {"kernel": kernel, "bound": "hbm", "achieved": round(gbs, 1) if gbs else None, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4) if gbs else None, "traffic": traffic,
+            "algorithmic_bytes": nbytes["algorithmic"], "record_bytes": nbytes["record"], "kernel_ms": round(ms, 4)}
 
 
 def cpu_baseline(scene, n_sample: int, p_cov_full: int) -> dict:
@@ -109,12 +144,18 @@ def bench_batched(reps: int) -> dict:
         cvs = r.render_batch(items, graph=True, prepared=prepared)
         blat.append(time.perf_counter() - a)
     batch_s = (time.perf_counter() - t0) / reps
-    covered = 0
+    # one eager batch with per-stage HIP events (the graph replay records only the whole launch)
+    r.render_batch(items, graph=False, prepared=prepared)
+    st = r.stats()
+    covered = t_vis = 0
     for cv, s in zip(cvs, views):
         _, _, w = r.read(cv, s.width, s.height)
-        covered += int((w != 0xFFFFFFFF).sum())
+        pc, tv = visible_triangles(w)
+        covered += pc
+        t_vis += tv
     tris = sum(s.triangle_count for s in views)
     px = sum(s.width * s.height for s in views)
+    rast_kernel = f"k_raster<{best[0]}, {(best[0] // 8) ** 2 // best[1]}>"
     r.close()
     plain_mpix = covered / plain_s / 1e6
     batch_mpix = covered / batch_s / 1e6
@@ -132,6 +173,10 @@ def bench_batched(reps: int) -> dict:
                                          "p99": round(float(np.percentile(blat, 99)) * 1e3, 4)},
                     "all_views_ms": round(batch_s * 1e3, 3), "graph": True},
         "batched_over_plain": round(batch_mpix / plain_mpix, 3),
+        "stage_ms_eager": {k: round(st[k], 4) for k in ("ms_geometry", "ms_binning", "ms_raster", "ms_shade",
+                                                        "ms_frame")},
+        "roofline_shade": roofline("k_shade", shade_bytes(px, covered, t_vis), st["ms_shade"]),
+        "roofline_raster": roofline(rast_kernel, raster_bytes(px, int(st["bin_entries"])), st["ms_raster"]),
     }
 
 
@@ -181,13 +226,14 @@ def main():
         st_acc.append(r.stats())
     total = time.perf_counter() - t0
     _, _, winner = r.read(cv, W, H)
-    sb, p_cov, t_vis = shade_bytes(winner)
+    p_cov, t_vis = visible_triangles(winner)
     st = {k: float(np.mean([s[k] for s in st_acc])) for k in st_acc[0]}
     frame_ms = total / args.steps * 1e3
     value = p_cov * args.steps / total / 1e6
-    shade_gbs = sb / (st["ms_shade"] * 1e-3) / 1e9 if st["ms_shade"] > 0 else None
-    rb = raster_bytes(W * H, int(st["bin_entries"]))
-    raster_gbs = rb / (st["ms_raster"] * 1e-3) / 1e9 if st["ms_raster"] > 0 else None
+    rast_kernel = f"k_raster<{args.tile}, {(args.tile // 8) ** 2 // args.waves}>"
+    roof_raster = roofline(rast_kernel, raster_bytes(W * H, int(st["bin_entries"])), st["ms_raster"],
+                           measured_traffic(rast_kernel))
+    roof_shade = roofline("k_shade", shade_bytes(W * H, p_cov, t_vis), st["ms_shade"], measured_traffic("k_shade"))
     out = {
         "metric": "shaded Mpix/s", "value": round(value, 2), "unit": "Mpix/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(frame_ms, 4),
@@ -202,15 +248,9 @@ def main():
         "stage_ms": {k: round(st[k], 4) for k in ("ms_geometry", "ms_binning", "ms_raster", "ms_shade", "ms_frame")},
         "frame": {"covered_px": p_cov, "visible_tris": t_vis, "clipped_tris": int(st["tris_out"]),
                   "bin_entries": int(st["bin_entries"]), "fragments": int(st["frags"]),
-                  "depth_complexity": round(st["frags"] / (W * H), 2)},
-        "roofline": {"kernel": "k_shade", "bound": "hbm", "achieved": round(shade_gbs, 1) if shade_gbs else None,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(shade_gbs / HBM_PEAK_GBS, 4) if shade_gbs else None,
-                     "traffic": None, "algorithmic_bytes": sb},
-        "roofline_raster": {"kernel": "k_raster", "bound": "hbm",
-                            "achieved": round(raster_gbs, 1) if raster_gbs else None, "peak": HBM_PEAK_GBS,
-                            "unit": "GB/s", "frac": round(raster_gbs / HBM_PEAK_GBS, 4) if raster_gbs else None,
-                            "algorithmic_bytes": rb},
+                  "frags_per_px": round(st["frags"] / (W * H), 2)},
+        "roofline": roof_raster,
+        "roofline_shade": roof_shade,
     }
     r.close()
     if args.batched_reps > 0:

@@ -89,7 +89,8 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
   const int W = vw.fr.W, H = vw.fr.H;
   const float clear_depth = vw.fr.clear_depth;
   const int tx0 = tx * TS, ty0 = ty * TS;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: keeps the wave's geometry scalar
   const uint2 rg = ranges[tile];
 
   for (int p = tid; p < TS * TS; p += NT) s_vis[p] = ~0ull;
@@ -104,11 +105,17 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
   const int wy1 = wy0 + (QPW < QS ? 1 : QPW / QS) * 8 - 1;
   int lp[QPW];        // local index of the lane's pixel in each of its quadrants
   uint32_t zk[QPW];   // recorded depth key at that pixel (0xFFFFFFFF: none yet)
+  float dxf[QPW], dyf[QPW];  // the pixel's offset from the wave rectangle's origin
+  bool onscr[QPW];           // the pixel lies on the screen
 #pragma unroll
   for (int qq = 0; qq < QPW; qq++) {
     const int Q = wq0 + qq;
     lp[qq] = ((Q / QS) * 8 + ly) * TS + (Q % QS) * 8 + lx;
     zk[qq] = 0xFFFFFFFFu;
+    const int px = tx0 + (Q % QS) * 8 + lx, py = ty0 + (Q / QS) * 8 + ly;
+    dxf[qq] = (float)(px - wx0);
+    dyf[qq] = (float)(py - wy0);
+    onscr[qq] = px < W && py < H;
   }
   uint32_t wmax = 0xFFFFFFFFu;  // largest recorded depth key over the wave's pixels
   uint32_t qn = 0;
@@ -167,9 +174,11 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
     }
     __syncthreads();
     for (int g0 = 0; g0 < n; g0 += 64) {
-      bool hit = false;
+      bool hit = false, wexact = false;
+      float pA[3] = {0.0f, 0.0f, 0.0f}, pB[3] = {0.0f, 0.0f, 0.0f}, pE[3] = {0.0f, 0.0f, 0.0f};
+      uint4 h = make_uint4(0u, 0u, 0u, 0u);
       if (g0 + lane < n) {
-        const uint4 h = s_hdr[g0 + lane];
+        h = s_hdr[g0 + lane];
         hit = !((int)(h.x >> 16) < wx0 || (int)(h.x & 0xFFFFu) > wx1 || (int)(h.y >> 16) < wy0 ||
                 (int)(h.y & 0xFFFFu) > wy1) &&
               !(h.w > wmax);
@@ -180,6 +189,17 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
           // at the corner maximising it and S = |A| xmax + |B| ymax + |C| + |D|,
           // every pixel p of the rectangle has s*e(p) <= val + 8.01 u S
           // (u = 2^-24), so val + 2^-20 S + 1 < 0 rules the edge out.
+          //
+          // When S < 2^24 - 216 on every edge (the float S is within 3 units
+          // of the exact one), every product and partial sum of the edge
+          // function over the rectangle is an integer of magnitude <= S, so
+          // s*e(p) - t = E + sA*(px - wx0) + sB*(py - wy0) is exact in two
+          // fma steps, E = s*e(wx0, wy0) - t with t = 0 for a tie edge and
+          // 1 otherwise.  covered_fast on the edge <=> s*e >= t <=> that
+          // value >= 0.  Pixels outside the triangle's bounding box need no
+          // separate test: a covered pixel is inside the snapped triangle,
+          // hence inside its bounding box unless it is off the screen.
+          wexact = true;
 #pragma unroll
           for (int e = 0; e < 3; e++) {
             const float4 q = s_pre[g0 + lane][e];
@@ -188,21 +208,56 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
             const float val = (q.x * xs + q.y * ys) + q.z;
             const float S = (fabsf(q.x) * (float)wx1 + fabsf(q.y) * (float)wy1) + q.w;
             if (val + (9.5367431640625e-07f * S + 1.0f) < 0.0f) hit = false;
+            if (!(S < 16777000.0f)) wexact = false;
+            pA[e] = q.x;
+            pB[e] = q.y;
+            pE[e] = ((q.x * (float)wx0 + q.y * (float)wy0) + q.z) - ((h.z & (kTieA << e)) ? 0.0f : 1.0f);
           }
         }
       }
       uint64_t hm = __ballot(hit);
+      const uint64_t em = __ballot(hit && wexact);
       while (hm) {
-        const int k = g0 + (int)__builtin_ctzll(hm);
+        const int kl = (int)__builtin_ctzll(hm);
+        const int k = g0 + kl;
         hm &= hm - 1;
         const SetupRec& r = s_rec[k];
-        const uint4 h = s_hdr[k];
-        const int rx0 = (int)__builtin_amdgcn_readfirstlane(h.x & 0xFFFFu);
-        const int rx1 = (int)__builtin_amdgcn_readfirstlane(h.x >> 16);
-        const int ry0 = (int)__builtin_amdgcn_readfirstlane(h.y & 0xFFFFu);
-        const int ry1 = (int)__builtin_amdgcn_readfirstlane(h.y >> 16);
-        const uint32_t flags = __builtin_amdgcn_readfirstlane(h.z);
-        const uint32_t zkey = __builtin_amdgcn_readfirstlane(h.w);
+        // the record's header, from the registers of the lane that pre-filtered it
+        const uint32_t hx = (uint32_t)__builtin_amdgcn_readlane((int)h.x, kl);
+        const uint32_t hy = (uint32_t)__builtin_amdgcn_readlane((int)h.y, kl);
+        const int rx0 = (int)(hx & 0xFFFFu), rx1 = (int)(hx >> 16);
+        const int ry0 = (int)(hy & 0xFFFFu), ry1 = (int)(hy >> 16);
+        const uint32_t flags = (uint32_t)__builtin_amdgcn_readlane((int)h.z, kl);
+        const uint32_t zkey = (uint32_t)__builtin_amdgcn_readlane((int)h.w, kl);
+        if ((em >> kl) & 1ull) {
+          float A[3], B[3], E[3];
+#pragma unroll
+          for (int e = 0; e < 3; e++) {
+            A[e] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pA[e]), kl));
+            B[e] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pB[e]), kl));
+            E[e] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pE[e]), kl));
+          }
+#pragma unroll
+          for (int qq = 0; qq < QPW; qq++) {
+            const int Q = wq0 + qq;
+            const int qx0 = tx0 + (Q % QS) * 8, qy0 = ty0 + (Q / QS) * 8;
+            if (rx1 < qx0 || rx0 > qx0 + 7 || ry1 < qy0 || ry0 > qy0 + 7) continue;
+            const float w0 = fmaf(A[0], dxf[qq], fmaf(B[0], dyf[qq], E[0]));
+            const float w1 = fmaf(A[1], dxf[qq], fmaf(B[1], dyf[qq], E[1]));
+            const float w2 = fmaf(A[2], dxf[qq], fmaf(B[2], dyf[qq], E[2]));
+            const bool cov = fminf(fminf(w0, w1), w2) >= 0.0f && onscr[qq] && !(zkey > zk[qq]);
+            const uint64_t m = __ballot(cov);
+            if (m) {
+              if (cov) {
+                const uint32_t pos = qn + (uint32_t)__popcll(m & lanemask_lt);
+                s_q[wave][pos] = ((uint32_t)k << 16) | (uint32_t)lp[qq];
+              }
+              qn += (uint32_t)__popcll(m);
+              if (qn > QCAP - 64) flush();
+            }
+          }
+          continue;
+        }
 #pragma unroll
         for (int qq = 0; qq < QPW; qq++) {
           const int Q = wq0 + qq;
