@@ -180,6 +180,84 @@ def bench_batched(reps: int) -> dict:
     }
 
 
+def bench_serving(seconds: float, rate_hz: float, n_distinct: int = 512) -> dict:
+    """Config 5: a device-resident pool of 64 seeded meshes (1k-2M triangles,
+    log-uniform, 1024^2 textures) and requests (mesh, random camera,
+    resolution 854x480 p=.4 / 1280x720 .3 / 1920x1080 .25 / 3840x2160 .05).
+    Three measurements over the same ``n_distinct`` requests:
+      plain     one dh_render per request, back to back;
+      capacity  a burst of 2 x n_distinct jobs through the server at once;
+      stream    open-loop Poisson arrivals at ``rate_hz`` for ``seconds``
+                (requests cycled), HBM budget 200 GB, p50/p99 latency from
+                submit to launch completion.
+    Shaded pixels per request come from one untimed render of each."""
+    from draw_amd import workloads
+    from draw_amd.render import Renderer, frame_uniforms
+    from draw_amd.serve import Server, poisson_load
+
+    log("serving: generating mesh pool ...")
+    pool, scenes = workloads.serving_requests(seed=5, n_requests=n_distinct)
+    r = Renderer(0, tile=32, waves=2, profiling=False)
+    reqs = [(frame_uniforms(s), r.mesh_handles(s)) for s in scenes]
+    pool_tris = int(sum(m.triangles.shape[0] for o in pool for m in o.meshes))
+    log("serving: plain path ...")
+    for fr, hs in reqs[:16]:
+        r.render_device(None, fr, hs)
+    t0 = time.perf_counter()
+    for fr, hs in reqs:
+        r.render_device(None, fr, hs)
+    plain_s = time.perf_counter() - t0
+    covered = []
+    for fr, hs in reqs:
+        cv = r.render_device(None, fr, hs)
+        _, _, w = r.read(cv, fr.W, fr.H)
+        covered.append(int(np.count_nonzero(w != 0xFFFFFFFF)))
+    cov_total = sum(covered)
+    tris_total = sum(s.triangle_count for s in scenes)
+    srv = Server(r, budget_bytes=200 << 30, workers=2, max_views=64)
+    log("serving: capacity burst ...")
+    for fr, hs in reqs[:64]:
+        srv.poll(srv.submit(fr, hs, autorelease=True), wait=True)
+    t0 = time.perf_counter()
+    tickets = [srv.submit(fr, hs, autorelease=True) for _ in range(2) for fr, hs in reqs]
+    burst_lat = []
+    for t in tickets:
+        res = srv.poll(t, wait=True)
+        burst_lat.append(res.latency_ms)
+        srv.release(t)
+    burst_s = time.perf_counter() - t0
+    st0 = srv.stats()
+    log(f"serving: Poisson {rate_hz:g}/s for {seconds:g} s ...")
+    out = poisson_load(srv, reqs, rate_hz=rate_hz, seconds=seconds, seed=6)
+    st1 = srv.stats()
+    srv.close()
+    r.close()
+    # served jobs cycle through the distinct requests in submission order
+    served_cov = sum(covered[(res.ticket - st0["submitted"] - 1) % len(reqs)] for res, _ in out["results"])
+    span = out["span_s"]
+    return {
+        "workload": "64-mesh resident pool (1k-2M triangles log-uniform, 1024^2 RGBA8 textures), random cameras, "
+                    "resolutions 854x480/1280x720/1920x1080/3840x2160 with p=.4/.3/.25/.05",
+        "pool_triangles": pool_tris, "distinct_requests": len(reqs),
+        "mean_tris_per_request": round(tris_total / len(reqs), 1),
+        "mean_shaded_px_per_request": round(cov_total / len(reqs), 1),
+        "plain": {"shaded_mpix_per_s": round(cov_total / plain_s / 1e6, 1),
+                  "requests_per_s": round(len(reqs) / plain_s, 1), "tris_per_s": round(tris_total / plain_s, 1)},
+        "capacity": {"shaded_mpix_per_s": round(2 * cov_total / burst_s / 1e6, 1),
+                     "requests_per_s": round(2 * len(reqs) / burst_s, 1),
+                     "tris_per_s": round(2 * tris_total / burst_s, 1),
+                     "latency_ms": {"p50": round(float(np.percentile(burst_lat, 50)), 3),
+                                    "p99": round(float(np.percentile(burst_lat, 99)), 3)}},
+        "capacity_over_plain": round(plain_s * 2 / burst_s, 3),
+        "stream": {"rate_hz": rate_hz, "seconds": seconds, "requests": out["requests"],
+                   "completed": out["completed"], "failed": out["failed"], "late_submits": out["late_submits"],
+                   "latency_ms": out["latency_ms"], "mean_batch_views": out["mean_batch_views"],
+                   "shaded_mpix_per_s": round(served_cov / span / 1e6, 1) if span > 0 else None,
+                   "launches": st1["launches"] - st0["launches"], "hbm_budget_bytes": 200 << 30,
+                   "hbm_peak_bytes": st1["bytes_peak"]},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -190,6 +268,9 @@ def main():
     ap.add_argument("--n-tri", type=int, default=1_000_000)
     ap.add_argument("--cpu-sample", type=int, default=400_000, help="0 disables the CPU baseline")
     ap.add_argument("--batched-reps", type=int, default=5, help="timed batched launches; 0 skips the batched run")
+    ap.add_argument("--serve-seconds", type=float, default=20.0,
+                    help="length of the Poisson serving stream (the serving config names 60 s); 0 skips serving")
+    ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate of the serving stream")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="use --tile/--waves instead of the autotuner's choice")
     args = ap.parse_args()
@@ -255,6 +336,8 @@ def main():
     r.close()
     if args.batched_reps > 0:
         out["batched"] = bench_batched(args.batched_reps)
+    if args.serve_seconds > 0:
+        out["serving"] = bench_serving(args.serve_seconds, args.serve_rate)
     if args.cpu_sample > 0:
         log(f"CPU baseline on {args.cpu_sample} triangles ...")
         out["cpu_baseline"] = cpu_baseline(scene, min(args.cpu_sample, scene.triangle_count), p_cov)

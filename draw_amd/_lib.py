@@ -43,6 +43,31 @@ class DhView(C.Structure):
 DH_BATCH_GRAPH = 1
 
 
+class DhServerConfig(C.Structure):
+    _fields_ = [("budget_bytes", C.c_uint64), ("workers", C.c_uint32), ("max_views", C.c_uint32),
+                ("max_tris", C.c_uint64)]
+
+
+class DhJob(C.Structure):
+    _fields_ = [("fr", DhFrameParams), ("meshes", C.POINTER(C.c_uint64)), ("n_meshes", C.c_uint32),
+                ("flags", C.c_uint32)]
+
+
+class DhJobInfo(C.Structure):
+    _fields_ = [("status", C.c_int32), ("batch_views", C.c_uint32), ("canvas", C.c_uint64),
+                ("t_submit_ms", C.c_double), ("t_admit_ms", C.c_double), ("t_done_ms", C.c_double)]
+
+
+class DhServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("launches", C.c_uint64), ("bytes_in_use", C.c_uint64), ("bytes_peak", C.c_uint64),
+                ("queued", C.c_uint64)]
+
+
+DH_JOB_AUTORELEASE = 1
+DH_JOB_DONE, DH_JOB_QUEUED, DH_JOB_RUNNING = 0, 1, 2
+
+
 class DhStats(C.Structure):
     _fields_ = [("tris_in", C.c_uint64), ("tris_out", C.c_uint64), ("bin_entries", C.c_uint64),
                 ("frags", C.c_uint64), ("retries", C.c_uint64), ("ms_geometry", C.c_float),
@@ -56,6 +81,7 @@ _H = C.c_uint64
 SIGNATURES = {
     "dh_frame_uniforms": [C.POINTER(DhCamera), C.c_int32, C.c_int32, C.POINTER(C.c_float), C.POINTER(DhFrameParams)],
     "dh_ctx_create": [C.c_int32, C.POINTER(_P)],
+    "dh_ctx_create_shared": [_P, C.POINTER(_P)],
     "dh_ctx_destroy": [_P],
     "dh_last_error": [_P],
     "dh_set_tiling": [_P, C.c_int32, C.c_int32],
@@ -72,6 +98,12 @@ SIGNATURES = {
     "dh_canvas_read": [_P, _H, _P, _P, _P],
     "dh_canvas_device_ptrs": [_P, _H, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)],
     "dh_get_stats": [_P, C.POINTER(DhStats)],
+    "dh_srv_create": [_P, C.POINTER(DhServerConfig), C.POINTER(_P)],
+    "dh_srv_submit": [_P, C.POINTER(DhJob), C.POINTER(C.c_uint64)],
+    "dh_srv_poll": [_P, C.c_uint64, C.c_int32, C.POINTER(DhJobInfo)],
+    "dh_srv_release": [_P, C.c_uint64],
+    "dh_srv_stats": [_P, C.POINTER(DhServerStats)],
+    "dh_srv_destroy": [_P],
 }
 
 _lib = None

@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <unordered_map>
@@ -72,12 +73,11 @@ struct DevBuf {
   }
 };
 
-}  // namespace
-
-struct dh_ctx_s {
+// Device-resident resources.  One pool may serve several contexts (streams)
+// on the same device (dh_ctx_create_shared); every access holds mu.
+struct ResPool {
+  std::mutex mu;
   int device = 0;
-  hipStream_t stream = nullptr;
-  std::string err;
   uint64_t next_id = 1;
   std::unordered_map<uint64_t, TexRes> tex;
   std::unordered_map<uint64_t, MatRes> mats;
@@ -89,6 +89,16 @@ struct dh_ctx_s {
   DevBuf<dh::DevMaterial> mat_dev;
   bool mat_dirty = true;
   uint8_t* white_rgb = nullptr;  // TextureMap::default
+  ~ResPool();
+};
+
+}  // namespace
+
+struct dh_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::shared_ptr<ResPool> res;
   // raster configuration
   int TS = 16, waves = 4;
   bool profiling = false;
@@ -145,12 +155,12 @@ int key_bits_for(uint32_t n) {
 }
 
 int upload_materials(dh_ctx c) {
-  if (!c->mat_dirty) return DH_OK;
-  HIPCHK(c, c->mat_dev.ensure(c->mat_host.size()));
-  HIPCHK(c, hipMemcpyAsync(c->mat_dev.p, c->mat_host.data(), c->mat_host.size() * sizeof(dh::DevMaterial),
+  if (!c->res->mat_dirty) return DH_OK;
+  HIPCHK(c, c->res->mat_dev.ensure(c->res->mat_host.size()));
+  HIPCHK(c, hipMemcpyAsync(c->res->mat_dev.p, c->res->mat_host.data(), c->res->mat_host.size() * sizeof(dh::DevMaterial),
                            hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  c->mat_dirty = false;
+  c->res->mat_dirty = false;
   return DH_OK;
 }
 
@@ -202,30 +212,75 @@ int ensure_scratch(dh_ctx c, uint32_t n_in, uint32_t cap_tri, uint32_t cap_pairs
 }  // namespace
 
 // ------------------------------------------------------------------ context
+ResPool::~ResPool() {
+  hipSetDevice(device);
+  for (auto& kv : tex) hipFree(kv.second.d);
+  for (auto& kv : objs) {
+    hipFree(kv.second.pos);
+    hipFree(kv.second.nrm);
+    hipFree(kv.second.uv);
+  }
+  for (auto& kv : meshes) hipFree(kv.second.tri);
+  for (auto& kv : canvases) {
+    hipFree(kv.second.bgra);
+    hipFree(kv.second.depth);
+    hipFree(kv.second.winner);
+  }
+  mat_dev.release();
+  if (white_rgb) hipFree(white_rgb);
+}
+
+namespace {
+
+// A context on `device` with its own stream, events and scratch; resources
+// from `pool`, or a new pool when null.
+int ctx_new(int32_t device, std::shared_ptr<ResPool> pool, dh_ctx* out) {
+  dh_ctx c = new dh_ctx_s();
+  c->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_status), sizeof(dh::Status));
+  for (int i = 0; i < 5 && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
+  if (e == hipSuccess && !pool) {
+    pool = std::make_shared<ResPool>();
+    pool->device = device;
+    e = hipMalloc(reinterpret_cast<void**>(&pool->white_rgb), 4);
+    if (e == hipSuccess) {
+      const uint8_t w[4] = {255, 255, 255, 255};  // Color::White.as_slice(), scene/mod.rs:130
+      e = hipMemcpy(pool->white_rgb, w, 4, hipMemcpyHostToDevice);
+    }
+    if (e == hipSuccess) pool->mat_host.push_back(default_material(pool->white_rgb));
+  }
+  c->res = pool;
+  if (e != hipSuccess) {
+    dh_ctx_destroy(c);
+    return DH_E_HIP;
+  }
+  *out = c;
+  return DH_OK;
+}
+
+}  // namespace
+
 extern "C" int dh_ctx_create(int32_t device, dh_ctx* out) {
   if (!out) return DH_E_ARG;
   *out = nullptr;
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return DH_E_UNSUPPORTED;
   if (device < 0 || device >= n) return DH_E_ARG;
-  dh_ctx c = new dh_ctx_s();
-  c->device = device;
-  hipError_t e = hipSetDevice(device);
-  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_status), sizeof(dh::Status));
-  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&c->white_rgb), 4);
-  if (e == hipSuccess) {
-    const uint8_t w[4] = {255, 255, 255, 255};  // Color::White.as_slice(), scene/mod.rs:130
-    e = hipMemcpy(c->white_rgb, w, 4, hipMemcpyHostToDevice);
+  return ctx_new(device, nullptr, out);
+}
+
+extern "C" int dh_ctx_create_shared(dh_ctx parent, dh_ctx* out) {
+  if (!parent || !out) return DH_E_ARG;
+  *out = nullptr;
+  const int rc = ctx_new(parent->device, parent->res, out);
+  if (rc == DH_OK) {
+    (*out)->TS = parent->TS;
+    (*out)->waves = parent->waves;
+    (*out)->ordered_bins = parent->ordered_bins;
   }
-  for (int i = 0; i < 5 && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
-  if (e != hipSuccess) {
-    dh_ctx_destroy(c);
-    return DH_E_HIP;
-  }
-  c->mat_host.push_back(default_material(c->white_rgb));
-  *out = c;
-  return DH_OK;
+  return rc;
 }
 
 extern "C" int dh_ctx_destroy(dh_ctx c) {
@@ -234,30 +289,17 @@ extern "C" int dh_ctx_destroy(dh_ctx c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
   if (c->graph) hipGraphDestroy(c->graph);
-  for (auto& kv : c->tex) hipFree(kv.second.d);
-  for (auto& kv : c->objs) {
-    hipFree(kv.second.pos);
-    hipFree(kv.second.nrm);
-    hipFree(kv.second.uv);
-  }
-  for (auto& kv : c->meshes) hipFree(kv.second.tri);
-  for (auto& kv : c->canvases) {
-    hipFree(kv.second.bgra);
-    hipFree(kv.second.depth);
-    hipFree(kv.second.winner);
-  }
-  c->mat_dev.release();
   c->cnt.release(); c->blk.release(); c->setup.release(); c->shade.release();
   c->bincnt.release(); c->binoff.release(); c->big.release();
   c->keys0.release(); c->keys1.release(); c->vals0.release(); c->vals1.release();
   c->hist.release(); c->partials.release(); c->ranges.release(); c->vis.release();
   c->tile_cnt.release(); c->tile_start.release();
   c->status.release(); c->draws.release(); c->views.release();
-  if (c->white_rgb) hipFree(c->white_rgb);
   if (c->h_status) hipHostFree(c->h_status);
   for (int i = 0; i < 5; i++)
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
   if (c->stream) hipStreamDestroy(c->stream);
+  c->res.reset();  // the pool goes with its last context
   delete c;
   return DH_OK;
 }
@@ -287,6 +329,7 @@ extern "C" int dh_set_profiling(dh_ctx c, int32_t enable) {
 // ---------------------------------------------------------------- resources
 extern "C" int dh_texture_create(dh_ctx c, const uint8_t* img, int32_t w, int32_t h, int32_t comp, dh_handle* out) {
   if (!c || !img || !out || w <= 0 || h <= 0 || (comp != 3 && comp != 4)) return DH_E_ARG;
+  std::lock_guard<std::mutex> lk(c->res->mu);
   HIPCHK(c, hipSetDevice(c->device));
   TexRes t;
   t.w = w;
@@ -295,15 +338,16 @@ extern "C" int dh_texture_create(dh_ctx c, const uint8_t* img, int32_t w, int32_
   const size_t bytes = (size_t)w * (size_t)h * (size_t)comp;
   HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&t.d), bytes + 4));  // +4: 32-bit texel loads
   HIPCHK(c, hipMemcpy(t.d, img, bytes, hipMemcpyHostToDevice));
-  const uint64_t id = c->next_id++;
-  c->tex[id] = t;
+  const uint64_t id = c->res->next_id++;
+  c->res->tex[id] = t;
   *out = make_handle(kTex, id);
   return DH_OK;
 }
 
 extern "C" int dh_material_create(dh_ctx c, const dh_material_desc* d, dh_handle* out) {
   if (!c || !d || !out || d->power < 0) return DH_E_ARG;
-  dh::DevMaterial m = default_material(c->white_rgb);
+  std::lock_guard<std::mutex> lk(c->res->mu);
+  dh::DevMaterial m = default_material(c->res->white_rgb);
   for (int i = 0; i < 3; i++) {
     m.ka[i] = d->ka[i];
     m.kd[i] = d->kd[i];
@@ -315,8 +359,8 @@ extern "C" int dh_material_create(dh_ctx c, const dh_material_desc* d, dh_handle
   for (int k = 0; k < 2; k++) {
     if (!maps[k]) continue;
     if (handle_kind(maps[k]) != kTex) return fail(c, DH_E_HANDLE, "material map is not a texture");
-    auto it = c->tex.find(maps[k] & ((1ull << 56) - 1));
-    if (it == c->tex.end()) return fail(c, DH_E_HANDLE, "unknown texture");
+    auto it = c->res->tex.find(maps[k] & ((1ull << 56) - 1));
+    if (it == c->res->tex.end()) return fail(c, DH_E_HANDLE, "unknown texture");
     if (k == 0) {
       m.kd_img = it->second.d; m.kd_w = it->second.w; m.kd_h = it->second.h; m.kd_comp = it->second.comp;
     } else {
@@ -324,12 +368,12 @@ extern "C" int dh_material_create(dh_ctx c, const dh_material_desc* d, dh_handle
     }
   }
   MatRes r;
-  r.slot = (uint32_t)c->mat_host.size();
+  r.slot = (uint32_t)c->res->mat_host.size();
   r.alpha = d->alpha;
-  c->mat_host.push_back(m);
-  c->mat_dirty = true;
-  const uint64_t id = c->next_id++;
-  c->mats[id] = r;
+  c->res->mat_host.push_back(m);
+  c->res->mat_dirty = true;
+  const uint64_t id = c->res->next_id++;
+  c->res->mats[id] = r;
   *out = make_handle(kMat, id);
   return DH_OK;
 }
@@ -337,6 +381,7 @@ extern "C" int dh_material_create(dh_ctx c, const dh_material_desc* d, dh_handle
 extern "C" int dh_object_create(dh_ctx c, const float* pos, uint32_t n_pos, const float* nrm, uint32_t n_nrm,
                                 const float* uv, uint32_t n_uv, dh_handle* out) {
   if (!c || !out || (n_pos && !pos) || (n_nrm && !nrm) || (n_uv && !uv)) return DH_E_ARG;
+  std::lock_guard<std::mutex> lk(c->res->mu);
   HIPCHK(c, hipSetDevice(c->device));
   ObjRes o;
   o.n_pos = n_pos;
@@ -348,8 +393,8 @@ extern "C" int dh_object_create(dh_ctx c, const float* pos, uint32_t n_pos, cons
   if (n_pos) HIPCHK(c, hipMemcpy(o.pos, pos, 12ull * n_pos, hipMemcpyHostToDevice));
   if (n_nrm) HIPCHK(c, hipMemcpy(o.nrm, nrm, 12ull * n_nrm, hipMemcpyHostToDevice));
   if (n_uv) HIPCHK(c, hipMemcpy(o.uv, uv, 8ull * n_uv, hipMemcpyHostToDevice));
-  const uint64_t id = c->next_id++;
-  c->objs[id] = o;
+  const uint64_t id = c->res->next_id++;
+  c->res->objs[id] = o;
   *out = make_handle(kObj, id);
   return DH_OK;
 }
@@ -357,10 +402,11 @@ extern "C" int dh_object_create(dh_ctx c, const float* pos, uint32_t n_pos, cons
 extern "C" int dh_mesh_create(dh_ctx c, dh_handle object, const uint32_t* tri, uint32_t n_tri, dh_handle material,
                               dh_handle* out) {
   if (!c || !out || (n_tri && !tri)) return DH_E_ARG;
+  std::lock_guard<std::mutex> lk(c->res->mu);
   if (handle_kind(object) != kObj) return fail(c, DH_E_HANDLE, "not an object handle");
-  auto ot = c->objs.find(object & ((1ull << 56) - 1));
-  if (ot == c->objs.end()) return fail(c, DH_E_HANDLE, "unknown object");
-  if (material && (handle_kind(material) != kMat || !c->mats.count(material & ((1ull << 56) - 1))))
+  auto ot = c->res->objs.find(object & ((1ull << 56) - 1));
+  if (ot == c->res->objs.end()) return fail(c, DH_E_HANDLE, "unknown object");
+  if (material && (handle_kind(material) != kMat || !c->res->mats.count(material & ((1ull << 56) - 1))))
     return fail(c, DH_E_HANDLE, "unknown material");
   const ObjRes& o = ot->second;
   // IndexedMesh::vec3_list_from_indexed asserts (scene/mesh.rs:46-48)
@@ -381,14 +427,15 @@ extern "C" int dh_mesh_create(dh_ctx c, dh_handle object, const uint32_t* tri, u
   m.mat = material;
   HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&m.tri), std::max<size_t>(1, 9ull * n_tri) * 4));
   if (n_tri) HIPCHK(c, hipMemcpy(m.tri, tri, 36ull * n_tri, hipMemcpyHostToDevice));
-  const uint64_t id = c->next_id++;
-  c->meshes[id] = m;
+  const uint64_t id = c->res->next_id++;
+  c->res->meshes[id] = m;
   *out = make_handle(kMesh, id);
   return DH_OK;
 }
 
 extern "C" int dh_canvas_create(dh_ctx c, int32_t W, int32_t H, dh_handle* out) {
   if (!c || !out || W <= 0 || H <= 0 || W > 65535 || H > 65535) return DH_E_ARG;
+  std::lock_guard<std::mutex> lk(c->res->mu);
   HIPCHK(c, hipSetDevice(c->device));
   CanvasRes cv;
   cv.W = W;
@@ -397,8 +444,8 @@ extern "C" int dh_canvas_create(dh_ctx c, int32_t W, int32_t H, dh_handle* out) 
   HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&cv.bgra), n * 4));
   HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&cv.depth), n * 4));
   HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&cv.winner), n * 4));
-  const uint64_t id = c->next_id++;
-  c->canvases[id] = cv;
+  const uint64_t id = c->res->next_id++;
+  c->res->canvases[id] = cv;
   *out = make_handle(kCanvas, id);
   return DH_OK;
 }
@@ -406,44 +453,45 @@ extern "C" int dh_canvas_create(dh_ctx c, int32_t W, int32_t H, dh_handle* out) 
 extern "C" int dh_release(dh_ctx c, dh_handle h) {
   if (!c) return DH_E_ARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  std::lock_guard<std::mutex> lk(c->res->mu);
   const uint64_t id = h & ((1ull << 56) - 1);
   switch (handle_kind(h)) {
     case kTex: {
-      auto it = c->tex.find(id);
-      if (it == c->tex.end()) break;
+      auto it = c->res->tex.find(id);
+      if (it == c->res->tex.end()) break;
       hipFree(it->second.d);
-      c->tex.erase(it);
+      c->res->tex.erase(it);
       return DH_OK;
     }
     case kMat: {
-      auto it = c->mats.find(id);
-      if (it == c->mats.end()) break;
-      c->mats.erase(it);  // the slot stays in the table
+      auto it = c->res->mats.find(id);
+      if (it == c->res->mats.end()) break;
+      c->res->mats.erase(it);  // the slot stays in the table
       return DH_OK;
     }
     case kObj: {
-      auto it = c->objs.find(id);
-      if (it == c->objs.end()) break;
+      auto it = c->res->objs.find(id);
+      if (it == c->res->objs.end()) break;
       hipFree(it->second.pos);
       hipFree(it->second.nrm);
       hipFree(it->second.uv);
-      c->objs.erase(it);
+      c->res->objs.erase(it);
       return DH_OK;
     }
     case kMesh: {
-      auto it = c->meshes.find(id);
-      if (it == c->meshes.end()) break;
+      auto it = c->res->meshes.find(id);
+      if (it == c->res->meshes.end()) break;
       hipFree(it->second.tri);
-      c->meshes.erase(it);
+      c->res->meshes.erase(it);
       return DH_OK;
     }
     case kCanvas: {
-      auto it = c->canvases.find(id);
-      if (it == c->canvases.end()) break;
+      auto it = c->res->canvases.find(id);
+      if (it == c->res->canvases.end()) break;
       hipFree(it->second.bgra);
       hipFree(it->second.depth);
       hipFree(it->second.winner);
-      c->canvases.erase(it);
+      c->res->canvases.erase(it);
       return DH_OK;
     }
   }
@@ -467,8 +515,8 @@ int build_tables(dh_ctx c, const dh_view* views, uint32_t n_views, uint64_t* n_i
     const dh_frame_params* fr = &vq.fr;
     if (fr->W <= 0 || fr->H <= 0 || (vq.n_meshes && !vq.meshes)) return fail(c, DH_E_ARG, "bad view");
     if (handle_kind(vq.canvas) != kCanvas) return fail(c, DH_E_HANDLE, "not a canvas handle");
-    auto ct = c->canvases.find(vq.canvas & ((1ull << 56) - 1));
-    if (ct == c->canvases.end()) return fail(c, DH_E_HANDLE, "unknown canvas");
+    auto ct = c->res->canvases.find(vq.canvas & ((1ull << 56) - 1));
+    if (ct == c->res->canvases.end()) return fail(c, DH_E_HANDLE, "unknown canvas");
     const CanvasRes& cv = ct->second;
     if (cv.W != fr->W || cv.H != fr->H) return fail(c, DH_E_ARG, "canvas size differs from frame size");
     dh::DevView dv;
@@ -490,16 +538,16 @@ int build_tables(dh_ctx c, const dh_view* views, uint32_t n_views, uint64_t* n_i
     for (uint32_t i = 0; i < vq.n_meshes; i++) {
       key->push_back(vq.meshes[i]);
       if (handle_kind(vq.meshes[i]) != kMesh) return fail(c, DH_E_HANDLE, "not a mesh handle");
-      auto mt = c->meshes.find(vq.meshes[i] & ((1ull << 56) - 1));
-      if (mt == c->meshes.end()) return fail(c, DH_E_HANDLE, "unknown mesh");
+      auto mt = c->res->meshes.find(vq.meshes[i] & ((1ull << 56) - 1));
+      if (mt == c->res->meshes.end()) return fail(c, DH_E_HANDLE, "unknown mesh");
       const MeshRes& m = mt->second;
-      auto ot = c->objs.find(m.obj & ((1ull << 56) - 1));
-      if (ot == c->objs.end()) return fail(c, DH_E_HANDLE, "mesh object was released");
+      auto ot = c->res->objs.find(m.obj & ((1ull << 56) - 1));
+      if (ot == c->res->objs.end()) return fail(c, DH_E_HANDLE, "mesh object was released");
       uint32_t slot = 0;
       float alpha = 1.0f;
       if (m.mat) {
-        auto it = c->mats.find(m.mat & ((1ull << 56) - 1));
-        if (it == c->mats.end()) return fail(c, DH_E_HANDLE, "mesh material was released");
+        auto it = c->res->mats.find(m.mat & ((1ull << 56) - 1));
+        if (it == c->res->mats.end()) return fail(c, DH_E_HANDLE, "mesh material was released");
         slot = it->second.slot;
         alpha = it->second.alpha;
       }
@@ -535,7 +583,8 @@ int build_tables(dh_ctx c, const dh_view* views, uint32_t n_views, uint64_t* n_i
   return DH_OK;
 }
 
-dh::PipeArgs make_args(dh_ctx c, uint32_t n_in, uint32_t n_tiles, uint32_t n_pixels, bool events) {
+dh::PipeArgs make_args(dh_ctx c, const dh::DevMaterial* mats, uint32_t n_in, uint32_t n_tiles, uint32_t n_pixels,
+                       bool events) {
   dh::PipeArgs a;
   memset(&a, 0, sizeof(a));
   a.draws = c->draws.p;
@@ -546,7 +595,7 @@ dh::PipeArgs make_args(dh_ctx c, uint32_t n_in, uint32_t n_tiles, uint32_t n_pix
   a.n_tiles = n_tiles;
   a.n_pixels = n_pixels;
   a.key_bits = key_bits_for(n_tiles);
-  a.mats = c->mat_dev.p;
+  a.mats = mats;
   a.TS = c->TS;
   a.waves = c->waves;
   a.cnt = c->cnt.p;
@@ -588,11 +637,17 @@ int render_views(dh_ctx c, const dh_view* views, uint32_t n_views, bool use_grap
   uint64_t n_in = 0;
   uint32_t n_tiles = 0, n_pixels = 0;
   std::vector<uint64_t> key;
-  int rc = build_tables(c, views, n_views, &n_in, &n_tiles, &n_pixels, &key);
-  if (rc) return rc;
-  rc = upload_materials(c);
-  if (rc) return rc;
-  key.push_back((uint64_t)(uintptr_t)c->mat_dev.p);
+  int rc;
+  dh::DevMaterial* mats = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(c->res->mu);
+    rc = build_tables(c, views, n_views, &n_in, &n_tiles, &n_pixels, &key);
+    if (rc) return rc;
+    rc = upload_materials(c);
+    if (rc) return rc;
+    mats = c->res->mat_dev.p;
+  }
+  key.push_back((uint64_t)(uintptr_t)mats);
   uint64_t want_tri = std::max<uint64_t>(c->cap_tri, n_in + n_in / 4 + 1024);
   uint64_t want_pairs = std::max<uint64_t>(c->cap_pairs, 4 * want_tri + n_tiles);
   c->stats = dh_stats{};
@@ -621,7 +676,7 @@ int render_views(dh_ctx c, const dh_view* views, uint32_t n_views, bool use_grap
       HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
       if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
     } else {
-      dh::enqueue_frame(make_args(c, (uint32_t)n_in, n_tiles, n_pixels, c->profiling), c->stream);
+      dh::enqueue_frame(make_args(c, mats, (uint32_t)n_in, n_tiles, n_pixels, c->profiling), c->stream);
       HIPCHK(c, hipGetLastError());
     }
     HIPCHK(c, hipMemcpyAsync(c->h_status, c->status.p, sizeof(dh::Status), hipMemcpyDeviceToHost, c->stream));
@@ -656,7 +711,7 @@ int render_views(dh_ctx c, const dh_view* views, uint32_t n_views, bool use_grap
       // capture the same launch sequence (no stage events) for the next call
       drop_graph(c);
       HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      dh::enqueue_frame(make_args(c, (uint32_t)n_in, n_tiles, n_pixels, false), c->stream);
+      dh::enqueue_frame(make_args(c, mats, (uint32_t)n_in, n_tiles, n_pixels, false), c->stream);
       hipGraph_t g = nullptr;
       HIPCHK(c, hipStreamEndCapture(c->stream, &g));
       c->graph = g;
@@ -689,12 +744,23 @@ extern "C" int dh_render_batch(dh_ctx c, const dh_view* views, uint32_t n_views,
   return render_views(c, views, n_views, (flags & DH_BATCH_GRAPH) != 0);
 }
 
+namespace {
+// The canvas behind a handle (copied out under the pool lock).
+int find_canvas(dh_ctx c, dh_handle canvas, CanvasRes* out) {
+  if (handle_kind(canvas) != kCanvas) return fail(c, DH_E_HANDLE, "not a canvas handle");
+  std::lock_guard<std::mutex> lk(c->res->mu);
+  auto ct = c->res->canvases.find(canvas & ((1ull << 56) - 1));
+  if (ct == c->res->canvases.end()) return fail(c, DH_E_HANDLE, "unknown canvas");
+  *out = ct->second;
+  return DH_OK;
+}
+}  // namespace
+
 extern "C" int dh_canvas_read(dh_ctx c, dh_handle canvas, uint8_t* bgra, float* depth, uint32_t* winner) {
   if (!c) return DH_E_ARG;
-  if (handle_kind(canvas) != kCanvas) return fail(c, DH_E_HANDLE, "not a canvas handle");
-  auto ct = c->canvases.find(canvas & ((1ull << 56) - 1));
-  if (ct == c->canvases.end()) return fail(c, DH_E_HANDLE, "unknown canvas");
-  const CanvasRes& cv = ct->second;
+  CanvasRes cv;
+  const int rc = find_canvas(c, canvas, &cv);
+  if (rc) return rc;
   const size_t n = (size_t)cv.W * (size_t)cv.H;
   HIPCHK(c, hipSetDevice(c->device));
   if (bgra) HIPCHK(c, hipMemcpyAsync(bgra, cv.bgra, n * 4, hipMemcpyDeviceToHost, c->stream));
@@ -706,11 +772,12 @@ extern "C" int dh_canvas_read(dh_ctx c, dh_handle canvas, uint8_t* bgra, float* 
 
 extern "C" int dh_canvas_device_ptrs(dh_ctx c, dh_handle canvas, void** bgra, void** depth, void** winner) {
   if (!c) return DH_E_ARG;
-  auto ct = c->canvases.find(canvas & ((1ull << 56) - 1));
-  if (handle_kind(canvas) != kCanvas || ct == c->canvases.end()) return fail(c, DH_E_HANDLE, "unknown canvas");
-  if (bgra) *bgra = ct->second.bgra;
-  if (depth) *depth = ct->second.depth;
-  if (winner) *winner = ct->second.winner;
+  CanvasRes cv;
+  const int rc = find_canvas(c, canvas, &cv);
+  if (rc) return rc;
+  if (bgra) *bgra = cv.bgra;
+  if (depth) *depth = cv.depth;
+  if (winner) *winner = cv.winner;
   return DH_OK;
 }
 
@@ -719,3 +786,55 @@ extern "C" int dh_get_stats(dh_ctx c, dh_stats* out) {
   *out = c->stats;
   return DH_OK;
 }
+
+// --------------------------------------------------------- internal helpers
+namespace dh {
+
+uint64_t ctx_scratch_bytes(dh_ctx c) {
+  uint64_t b = 0;
+  b += c->cnt.n + 4 * c->blk.n + sizeof(SetupRec) * c->setup.n + sizeof(ShadeRec) * c->shade.n;
+  b += 4 * (c->bincnt.n + c->binoff.n + c->big.n + c->keys0.n + c->keys1.n + c->vals0.n + c->vals1.n);
+  b += 4 * (c->hist.n + c->partials.n + c->tile_cnt.n + c->tile_start.n + c->vis.n) + 8 * c->ranges.n;
+  b += sizeof(Status) * c->status.n + sizeof(DevDraw) * c->draws.n + sizeof(DevView) * c->views.n;
+  return b;
+}
+
+// Scratch a frame of n_in input triangles, n_tiles tiles and n_pixels pixels
+// asks for on its first attempt (the sizing render_views starts from).
+uint64_t scratch_estimate(uint64_t n_in, uint64_t n_tiles, uint64_t n_pixels, uint64_t n_draws,
+                          uint64_t n_views) {
+  const uint64_t tri = n_in + n_in / 4 + 1024, pairs = 4 * tri + n_tiles;
+  return n_in + 4 * (n_in / 256 + 1) + (sizeof(SetupRec) + sizeof(ShadeRec) + 12) * tri + 16 * pairs +
+         4ull * kRadixHistEntries + 4096 + 20 * n_tiles + 4 * n_pixels + sizeof(DevDraw) * n_draws +
+         sizeof(DevView) * n_views + 256;
+}
+
+int ctx_tile_size(dh_ctx c) { return c->TS; }
+
+// Triangle count of a mesh handle (DH_E_HANDLE if unknown).
+int mesh_triangles(dh_ctx c, dh_handle mesh, uint32_t* n_tri) {
+  if (handle_kind(mesh) != kMesh) return fail(c, DH_E_HANDLE, "not a mesh handle");
+  std::lock_guard<std::mutex> lk(c->res->mu);
+  auto it = c->res->meshes.find(mesh & ((1ull << 56) - 1));
+  if (it == c->res->meshes.end()) return fail(c, DH_E_HANDLE, "unknown mesh");
+  *n_tri = it->second.n_tri;
+  return DH_OK;
+}
+
+}  // namespace dh
+
+namespace dh {
+// Free every scratch buffer of a context (they are regrown by the next frame).
+void ctx_release_scratch(dh_ctx c) {
+  hipSetDevice(c->device);
+  hipStreamSynchronize(c->stream);
+  drop_graph(c);
+  c->cnt.release(); c->blk.release(); c->setup.release(); c->shade.release();
+  c->bincnt.release(); c->binoff.release(); c->big.release();
+  c->keys0.release(); c->keys1.release(); c->vals0.release(); c->vals1.release();
+  c->hist.release(); c->partials.release(); c->ranges.release(); c->vis.release();
+  c->tile_cnt.release(); c->tile_start.release();
+  c->status.release(); c->draws.release(); c->views.release();
+  c->cap_tri = c->cap_pairs = 0;
+}
+}  // namespace dh

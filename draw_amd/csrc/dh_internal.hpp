@@ -1,0 +1,14 @@
+// dh_internal.hpp -- library-internal helpers shared by dh_api.cpp and the
+// render server (dh_server.cpp).  Not part of the C ABI.
+#pragma once
+#include <stdint.h>
+
+#include "../../include/drawhip.h"
+
+namespace dh {
+uint64_t ctx_scratch_bytes(dh_ctx c);
+uint64_t scratch_estimate(uint64_t n_in, uint64_t n_tiles, uint64_t n_pixels, uint64_t n_draws, uint64_t n_views);
+int ctx_tile_size(dh_ctx c);
+int mesh_triangles(dh_ctx c, dh_handle mesh, uint32_t* n_tri);
+void ctx_release_scratch(dh_ctx c);
+}  // namespace dh

@@ -367,6 +367,30 @@ def batched_scenes(seed: int = 4, n_meshes: int = 8, cams_per_mesh: int = 32, tr
     return views
 
 
+SERVING_RESOLUTIONS = [((854, 480), 0.4), ((1280, 720), 0.3), ((1920, 1080), 0.25), ((3840, 2160), 0.05)]
+
+
+def serving_requests(seed: int = 5, n_requests: int = 4096, n_meshes: int = 64, tri_lo: int = 1000,
+                     tri_hi: int = 2_000_000, tex_size: int = 1024, resolutions=SERVING_RESOLUTIONS,
+                     pool: List[Object] | None = None) -> Tuple[List[Object], List[Scene]]:
+    """Serving workload: a pool of ``n_meshes`` seeded meshes (log-uniform
+    triangle counts in [tri_lo, tri_hi]) and ``n_requests`` requests, each a
+    mesh drawn uniformly from the pool, a random camera and a resolution drawn
+    with the given probabilities.  Returns (pool, one Scene per request)."""
+    rng = _rng(seed)
+    if pool is None:
+        pool = mesh_pool(seed + 2000, n_meshes, tri_lo, tri_hi, tex_size)
+    sizes = [r for r, _ in resolutions]
+    p = np.array([q for _, q in resolutions], np.float64)
+    p /= p.sum()
+    reqs: List[Scene] = []
+    for _ in range(n_requests):
+        obj = pool[int(rng.integers(len(pool)))]
+        W, H = sizes[int(rng.choice(len(sizes), p=p))]
+        reqs.append(Scene(W, H, random_camera(rng), (0.0, 6.0, 6.0), [obj]))
+    return pool, reqs
+
+
 def _ball_soup(n: int, radius: float, tex: Texture, rng: np.random.Generator) -> Object:
     """Soup of small triangles with centroids uniform in a ball."""
     c = rng.standard_normal((n, 3))

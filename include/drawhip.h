@@ -112,6 +112,12 @@ int dh_frame_uniforms(const dh_camera* cam, int32_t W, int32_t H, const float li
 
 /* ---- context --------------------------------------------------------- */
 int dh_ctx_create(int32_t device, dh_ctx* out);
+/* A second context (own HIP stream and scratch) on the parent's device that
+ * shares the parent's resources: handles created through either are valid in
+ * both, so frames can render concurrently from one resident mesh pool.  The
+ * resources live until the last context sharing them is destroyed.  No
+ * counterpart in the reference (one canvas, one thread). */
+int dh_ctx_create_shared(dh_ctx parent, dh_ctx* out);
 int dh_ctx_destroy(dh_ctx ctx);
 /* Human-readable text of the last error on this context (never NULL). */
 const char* dh_last_error(dh_ctx ctx);
@@ -180,6 +186,71 @@ int dh_canvas_device_ptrs(dh_ctx ctx, dh_handle canvas, void** bgra, void** dept
 int dh_get_stats(dh_ctx ctx, dh_stats* out);
 /* Enable (1) or disable (0) per-stage HIP event timing of dh_render. */
 int dh_set_profiling(dh_ctx ctx, int32_t enable);
+
+/* ---- serving ---------------------------------------------------------- */
+/* A render server with continuous batching.  No counterpart in the reference,
+ * whose app loop renders one frame per iteration (app/mod.rs:96-235); each job
+ * is one Scene::render of the given meshes and uniforms, and its result is
+ * exactly what dh_render produces for it alone.
+ *
+ * Worker threads, each with its own stream and scratch (contexts sharing the
+ * server context's resources), take queued jobs in arrival order and render
+ * as many as fit in one dh_render_batch launch.  A job is admitted only when
+ * its canvas and the scratch of the launch fit the HBM budget together with
+ * every canvas and scratch buffer already held; otherwise it waits until
+ * memory is released.  Jobs that arrive while launches are on the device join
+ * the next launch of whichever worker frees up first, so launches of
+ * different workers overlap on the device. */
+typedef struct dh_server_s* dh_server;
+
+typedef struct {
+    uint64_t budget_bytes;  /* HBM cap for job canvases + worker scratch; 0 = no cap   */
+    uint32_t workers;       /* concurrent launches (streams), >= 1; 0 = 2               */
+    uint32_t max_views;     /* jobs per launch; 0 = 64                                  */
+    uint64_t max_tris;      /* input triangles per launch; 0 = 16M                      */
+} dh_server_config;
+
+enum { DH_JOB_AUTORELEASE = 1u };  /* recycle the canvas as soon as the job is done */
+
+typedef struct {
+    dh_frame_params fr;
+    const dh_handle* meshes;  /* copied by dh_srv_submit                              */
+    uint32_t n_meshes;
+    uint32_t flags;           /* DH_JOB_*                                             */
+} dh_job;
+
+enum { DH_JOB_QUEUED = 1, DH_JOB_RUNNING = 2, DH_JOB_DONE = 0 };
+
+typedef struct {
+    int32_t status;           /* DH_JOB_QUEUED/RUNNING/DONE or a negative DH_E_* code  */
+    uint32_t batch_views;     /* jobs in the launch this job ran in                   */
+    dh_handle canvas;         /* result canvas (resource of the server context) when
+                                 DONE; valid until dh_srv_release                      */
+    double t_submit_ms;       /* server clock: ms since dh_srv_create                  */
+    double t_admit_ms;
+    double t_done_ms;         /* the job's launch completed on the device              */
+} dh_job_info;
+
+typedef struct {
+    uint64_t submitted, completed, failed, launches;
+    uint64_t bytes_in_use;    /* canvases (held and pooled) + worker scratch now        */
+    uint64_t bytes_peak;
+    uint64_t queued;          /* jobs waiting for admission now                        */
+} dh_server_stats;
+
+/* ctx: the context whose resources (meshes, textures, materials) jobs use; it
+ * must outlive the server.  Its tiling applies to every worker. */
+int dh_srv_create(dh_ctx ctx, const dh_server_config* cfg, dh_server* out);
+/* Queue a job.  DH_E_BUDGET if its canvas plus the scratch of a launch of it
+ * alone exceed the budget; DH_E_HANDLE for an unknown mesh. */
+int dh_srv_submit(dh_server srv, const dh_job* job, uint64_t* ticket);
+/* State of a job; wait != 0 blocks until it is done or failed. */
+int dh_srv_poll(dh_server srv, uint64_t ticket, int32_t wait, dh_job_info* out);
+/* Forget a finished job and return its canvas to the server's pool. */
+int dh_srv_release(dh_server srv, uint64_t ticket);
+int dh_srv_stats(dh_server srv, dh_server_stats* out);
+/* Finish every queued and running job, then stop the workers. */
+int dh_srv_destroy(dh_server srv);
 
 #ifdef __cplusplus
 }

@@ -4,6 +4,7 @@ gen_transformation_matrix + gen_view_planes) equal the oracle's bit for bit."""
 import ctypes as C
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -34,6 +35,32 @@ def test_structs_match_header_sizes():
     assert C.sizeof(_lib.DhFrameParams) == 208
     assert C.sizeof(_lib.DhFrameParams) == C.sizeof(O.OrFrame)
     assert C.sizeof(_lib.DhCamera) == 36
+
+
+def test_struct_layouts_match_c_compiler(tmp_path):
+    """sizeof/offsetof of every ABI struct as gcc lays out include/drawhip.h
+    equal the ctypes mirrors."""
+    structs = {"dh_frame_params": _lib.DhFrameParams, "dh_camera": _lib.DhCamera,
+               "dh_material_desc": _lib.DhMaterialDesc, "dh_stats": _lib.DhStats, "dh_view": _lib.DhView,
+               "dh_server_config": _lib.DhServerConfig, "dh_job": _lib.DhJob, "dh_job_info": _lib.DhJobInfo,
+               "dh_server_stats": _lib.DhServerStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "drawhip.h"', "int main(void) {"]
+    for cname, ct in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in ct._fields_:
+            cf = {"near_": "near_", "far_": "far_"}.get(fname, fname)
+            lines.append(f'  printf("{cname} {fname} %zu\\n", offsetof({cname}, {cf}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for cname, ct in structs.items():
+        assert got[(cname, "size")] == C.sizeof(ct), cname
+        for fname, _ in ct._fields_:
+            assert got[(cname, fname)] == getattr(ct, fname).offset, (cname, fname)
 
 
 def test_frame_uniforms_bit_exact_vs_oracle():

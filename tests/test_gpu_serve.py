@@ -1,0 +1,84 @@
+"""Serving: the continuous-batching server (dh_srv_*).  Every job's result in
+a mixed, memory-limited stream of jobs must equal the same job rendered alone
+(through the server and through dh_render) bit for bit, and the oracle."""
+import numpy as np
+import pytest
+
+from draw_amd import workloads
+from draw_amd._lib import DrawHipError
+from draw_amd.render import frame_uniforms
+from draw_amd.serve import Server, poisson_load
+from parity import assert_same, oracle
+
+pytestmark = pytest.mark.gpu
+
+SMALL_RES = [((96, 64), 0.3), ((160, 120), 0.3), ((320, 240), 0.3), ((200, 150), 0.1)]
+
+
+def _requests(renderer, seed, n):
+    pool, scenes = workloads.serving_requests(seed=seed, n_requests=n, n_meshes=6, tri_lo=300, tri_hi=30_000,
+                                              tex_size=128, resolutions=SMALL_RES)
+    return scenes, [(frame_uniforms(s), renderer.mesh_handles(s)) for s in scenes]
+
+
+def _alone_need(scene):
+    # canvas + a generous bound on one job's scratch
+    return 16 * scene.width * scene.height + 450 * scene.triangle_count + (12 << 20)
+
+
+def test_mix_equals_alone_under_memory_pressure(renderer):
+    scenes, reqs = _requests(renderer, seed=31, n=40)
+    budget = 3 * max(_alone_need(s) for s in scenes)
+    srv = Server(renderer, budget_bytes=budget, workers=2, max_views=16)
+    try:
+        tickets = [srv.submit(fr, hs) for fr, hs in reqs]           # a burst: all queued at once
+        mixed = []
+        for t, s in zip(tickets, scenes):
+            res = srv.poll(t, wait=True)
+            assert res.status == 0
+            mixed.append(srv.read(res, s.width, s.height))
+            srv.release(t)
+        st = srv.stats()
+        assert st["completed"] == len(scenes) and st["failed"] == 0
+        assert st["launches"] < len(scenes), "jobs were not batched"
+        assert st["bytes_peak"] <= budget
+        # the same jobs one at a time through the server, and through dh_render
+        for k, ((fr, hs), s) in enumerate(zip(reqs, scenes)):
+            t = srv.submit(fr, hs)
+            res = srv.poll(t, wait=True)
+            assert res.status == 0 and res.batch_views == 1
+            alone = srv.read(res, s.width, s.height)
+            srv.release(t)
+            c, w = renderer.render(s, frame=fr)
+            f, d, wm = mixed[k]
+            for a, b in zip(alone, mixed[k]):
+                assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), f"job {k}: mix differs from alone"
+            assert np.array_equal(c.frame, f) and np.array_equal(c.depth.view(np.uint32), d.view(np.uint32))
+            assert np.array_equal(w, wm)
+            if k % 8 == 0:
+                assert_same(f, d, wm, oracle(s), label=f"served job {k}")
+    finally:
+        srv.close()
+
+
+def test_job_over_budget_is_rejected(renderer):
+    scenes, reqs = _requests(renderer, seed=32, n=2)
+    srv = Server(renderer, budget_bytes=1 << 20, workers=1)
+    try:
+        with pytest.raises(DrawHipError, match="DH_E_BUDGET"):
+            srv.submit(*reqs[0])
+    finally:
+        srv.close()
+
+
+def test_poisson_stream_completes(renderer):
+    scenes, reqs = _requests(renderer, seed=33, n=64)
+    srv = Server(renderer, budget_bytes=512 << 20, workers=2)
+    try:
+        out = poisson_load(srv, reqs, rate_hz=400.0, seconds=0.5, seed=1)
+        assert out["completed"] == out["requests"] > 50 and out["failed"] == 0
+        assert 0 < out["latency_ms"]["p50"] <= out["latency_ms"]["p99"] < 5000
+        st = srv.stats()
+        assert st["queued"] == 0 and st["completed"] == out["requests"]
+    finally:
+        srv.close()
