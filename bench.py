@@ -71,7 +71,7 @@ def measured_traffic(kernel: str):
         return None
     for line in open(path):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) >= 4 and cells[0] == f"`{kernel}`":
+        if len(cells) >= 4 and cells[0].strip("`").replace("dh::", "") == kernel:
             try:
                 return int(float(cells[3]))
             except ValueError:
@@ -258,6 +258,21 @@ def bench_serving(seconds: float, rate_hz: float, n_distinct: int = 512) -> dict
     }
 
 
+def copy_calibration(mib: int) -> dict:
+    """A plain device-to-device copy of ``mib`` MiB, three times: the known
+    byte count that scales rocprofv3's FETCH_SIZE / WRITE_SIZE readings."""
+    import torch
+    n = mib << 20
+    src = torch.empty(n, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    src.fill_(7)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    return {"copy_bytes": n, "copies": 3}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -273,7 +288,12 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate of the serving stream")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="use --tile/--waves instead of the autotuner's choice")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (calibrates rocprofv3 byte counters) and exit")
     args = ap.parse_args()
+    if args.copy_calibration > 0:
+        print(json.dumps(copy_calibration(args.copy_calibration)), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); the path shards as described "
               f"in DESIGN.md but no multi-GPU run is made", file=sys.stderr)

@@ -97,6 +97,9 @@ SIGNATURES = {
     "dh_render_batch": [_P, C.POINTER(DhView), C.c_uint32, C.c_uint32],
     "dh_canvas_read": [_P, _H, _P, _P, _P],
     "dh_canvas_device_ptrs": [_P, _H, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)],
+    "dh_canvas_read_rgba": [_P, _H, _P],
+    "dh_canvas_export_png": [_P, _H, C.c_char_p],
+    "dh_write_png": [C.c_char_p, _P, C.c_int32, C.c_int32],
     "dh_get_stats": [_P, C.POINTER(DhStats)],
     "dh_srv_create": [_P, C.POINTER(DhServerConfig), C.POINTER(_P)],
     "dh_srv_submit": [_P, C.POINTER(DhJob), C.POINTER(C.c_uint64)],

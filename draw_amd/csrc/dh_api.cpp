@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "dh_common.hpp"
+#include "dh_internal.hpp"
 #include "dh_pipeline.hpp"
 
 namespace {
@@ -118,6 +119,7 @@ struct dh_ctx_s {
   DevBuf<dh::Status> status;
   DevBuf<dh::DevDraw> draws;
   DevBuf<dh::DevView> views;
+  DevBuf<uint32_t> export_buf;  // RGBA staging of dh_canvas_read_rgba
   uint32_t cap_tri = 0, cap_pairs = 0;
   // pinned staging
   dh::Status* h_status = nullptr;
@@ -294,7 +296,7 @@ extern "C" int dh_ctx_destroy(dh_ctx c) {
   c->keys0.release(); c->keys1.release(); c->vals0.release(); c->vals1.release();
   c->hist.release(); c->partials.release(); c->ranges.release(); c->vis.release();
   c->tile_cnt.release(); c->tile_start.release();
-  c->status.release(); c->draws.release(); c->views.release();
+  c->status.release(); c->draws.release(); c->views.release(); c->export_buf.release();
   if (c->h_status) hipHostFree(c->h_status);
   for (int i = 0; i < 5; i++)
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
@@ -770,6 +772,32 @@ extern "C" int dh_canvas_read(dh_ctx c, dh_handle canvas, uint8_t* bgra, float* 
   return DH_OK;
 }
 
+extern "C" int dh_canvas_read_rgba(dh_ctx c, dh_handle canvas, uint8_t* rgba) {
+  if (!c || !rgba) return DH_E_ARG;
+  CanvasRes cv;
+  const int rc = find_canvas(c, canvas, &cv);
+  if (rc) return rc;
+  const size_t n = (size_t)cv.W * (size_t)cv.H;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, c->export_buf.ensure(n));
+  HIPCHK(c, dh::enqueue_bgra_to_rgba(cv.bgra, c->export_buf.p, (uint32_t)n, c->stream));
+  HIPCHK(c, hipMemcpyAsync(rgba, c->export_buf.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return DH_OK;
+}
+
+extern "C" int dh_canvas_export_png(dh_ctx c, dh_handle canvas, const char* path) {
+  if (!c || !path) return DH_E_ARG;
+  CanvasRes cv;
+  int rc = find_canvas(c, canvas, &cv);
+  if (rc) return rc;
+  std::vector<uint8_t> rgba((size_t)cv.W * (size_t)cv.H * 4);
+  rc = dh_canvas_read_rgba(c, canvas, rgba.data());
+  if (rc) return rc;
+  rc = dh_write_png(path, rgba.data(), cv.W, cv.H);
+  return rc ? fail(c, rc, std::string("cannot write ") + path) : DH_OK;
+}
+
 extern "C" int dh_canvas_device_ptrs(dh_ctx c, dh_handle canvas, void** bgra, void** depth, void** winner) {
   if (!c) return DH_E_ARG;
   CanvasRes cv;
@@ -834,7 +862,7 @@ void ctx_release_scratch(dh_ctx c) {
   c->keys0.release(); c->keys1.release(); c->vals0.release(); c->vals1.release();
   c->hist.release(); c->partials.release(); c->ranges.release(); c->vis.release();
   c->tile_cnt.release(); c->tile_start.release();
-  c->status.release(); c->draws.release(); c->views.release();
+  c->status.release(); c->draws.release(); c->views.release(); c->export_buf.release();
   c->cap_tri = c->cap_pairs = 0;
 }
 }  // namespace dh

@@ -1,6 +1,7 @@
 // dh_internal.hpp -- library-internal helpers shared by dh_api.cpp and the
 // render server (dh_server.cpp).  Not part of the C ABI.
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 #include "../../include/drawhip.h"
@@ -11,4 +12,5 @@ uint64_t scratch_estimate(uint64_t n_in, uint64_t n_tiles, uint64_t n_pixels, ui
 int ctx_tile_size(dh_ctx c);
 int mesh_triangles(dh_ctx c, dh_handle mesh, uint32_t* n_tri);
 void ctx_release_scratch(dh_ctx c);
+hipError_t enqueue_bgra_to_rgba(const uint32_t* src, uint32_t* dst, uint32_t n, hipStream_t s);
 }  // namespace dh

@@ -188,6 +188,15 @@ class Renderer:
                                           winner.ctypes.data), "dh_canvas_read")
         return bgra, depth, winner
 
+    def read_rgba(self, canvas_handle: int, W: int, H: int) -> np.ndarray:
+        """The frame as App::export_frame_as prepares it: RGBA8, rows top-down."""
+        rgba = np.empty((H, W, 4), np.uint8)
+        self._check(self.L.dh_canvas_read_rgba(self.ctx, canvas_handle, rgba.ctypes.data), "dh_canvas_read_rgba")
+        return rgba
+
+    def export_png(self, canvas_handle: int, path: str):
+        self._check(self.L.dh_canvas_export_png(self.ctx, canvas_handle, str(path).encode()), "dh_canvas_export_png")
+
     def render(self, scene: Scene, canvas: Canvas | None = None, frame: DhFrameParams | None = None):
         """Scene::render(&mut canvas): returns (canvas, winner ids)."""
         fr = frame if frame is not None else frame_uniforms(scene)

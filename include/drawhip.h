@@ -181,6 +181,17 @@ int dh_render_batch(dh_ctx ctx, const dh_view* views, uint32_t n_views, uint32_t
  * bottom-up (canvas.rs:415); winner W*H ids, rows bottom-up, 0xFFFFFFFF =
  * clear.  Any pointer may be NULL. */
 int dh_canvas_read(dh_ctx ctx, dh_handle canvas, uint8_t* bgra, float* depth, uint32_t* winner);
+/* Frame export (App::export_frame_as, app/mod.rs:316-379): the canvas frame
+ * with bytes 0 and 2 of every pixel swapped (BGRA -> RGBA, app/mod.rs:347-351)
+ * into rgba, W*H*4 bytes, rows top-down; the swap runs on the device. */
+int dh_canvas_read_rgba(dh_ctx ctx, dh_handle canvas, uint8_t* rgba);
+/* That RGBA frame written as an 8-bit RGBA PNG file (the reference writes it
+ * with stb_image_write; decoded pixels are the same, the compressed bytes are
+ * not).  JPEG export is not provided. */
+int dh_canvas_export_png(dh_ctx ctx, dh_handle canvas, const char* path);
+/* Host-only PNG writer behind dh_canvas_export_png: W*H RGBA8 pixels, rows
+ * top-down. */
+int dh_write_png(const char* path, const uint8_t* rgba, int32_t W, int32_t H);
 /* Device pointers of a canvas (for zero-copy consumers on the same device). */
 int dh_canvas_device_ptrs(dh_ctx ctx, dh_handle canvas, void** bgra, void** depth, void** winner);
 int dh_get_stats(dh_ctx ctx, dh_stats* out);
