@@ -92,6 +92,7 @@ SIGNATURES = {
     "dh_object_create": [_P, _P, C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, C.POINTER(_H)],
     "dh_mesh_create": [_P, _H, _P, C.c_uint32, _H, C.POINTER(_H)],
     "dh_canvas_create": [_P, C.c_int32, C.c_int32, C.POINTER(_H)],
+    "dh_mesh_read": [_P, _H, _P],
     "dh_release": [_P, _H],
     "dh_render": [_P, C.POINTER(DhFrameParams), C.POINTER(_H), C.c_uint32, _H],
     "dh_render_batch": [_P, C.POINTER(DhView), C.c_uint32, C.c_uint32],

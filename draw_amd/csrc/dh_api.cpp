@@ -120,11 +120,22 @@ struct dh_ctx_s {
   DevBuf<dh::DevDraw> draws;
   DevBuf<dh::DevView> views;
   DevBuf<uint32_t> export_buf;  // RGBA staging of dh_canvas_read_rgba
+  uint64_t scratch_cap = 0;     // 0 = none; else frame scratch may not grow past it (DH_E_BUDGET)
+  uint64_t scratch_need = 0;    // bytes the last frame refused by the cap asked for
   uint32_t cap_tri = 0, cap_pairs = 0;
   // pinned staging
   dh::Status* h_status = nullptr;
   std::vector<dh::DevDraw> h_draws;
   std::vector<dh::DevView> h_views;
+  // transparent pass of a one-view frame
+  std::vector<dh::DevDraw> h_tdraws;
+  std::vector<dh::TransMesh> h_tmeshes;
+  std::vector<uint32_t> h_grp_first;  // first opaque input triangle of each object group
+  uint64_t n_tin = 0, max_tmesh = 0;
+  DevBuf<dh::DevDraw> tdraws;
+  DevBuf<uint32_t> grp;       // [2 * groups]: first input triangle, then first clipped id
+  DevBuf<uint32_t> tperm;     // [9 * max_tmesh]
+  DevBuf<uint32_t> sort_n;    // [1]
   dh_stats stats = {};
   // captured batch launch (DH_BATCH_GRAPH)
   hipGraph_t graph = nullptr;
@@ -183,6 +194,21 @@ dh::DevMaterial default_material(const uint8_t* white) {
   return m;
 }
 
+// Bytes every scratch buffer would hold after ensure_scratch with these sizes.
+uint64_t scratch_bytes_after(dh_ctx c, uint64_t n_in, uint64_t cap_tri, uint64_t cap_pairs, uint64_t n_tiles,
+                             uint64_t n_pixels, uint64_t n_draws, uint64_t n_views) {
+  auto mx = [](uint64_t have, uint64_t want) { return std::max<uint64_t>(have, std::max<uint64_t>(want, 1)); };
+  uint64_t b = mx(c->cnt.n, n_in) + 4 * mx(c->blk.n, (n_in + 255) / 256 + 1);
+  b += (sizeof(dh::SetupRec) * mx(c->setup.n, cap_tri)) + sizeof(dh::ShadeRec) * mx(c->shade.n, cap_tri);
+  b += 4 * (mx(c->bincnt.n, cap_tri) + mx(c->binoff.n, cap_tri) + mx(c->big.n, cap_tri));
+  b += 4 * (mx(c->keys0.n, cap_pairs) + mx(c->keys1.n, cap_pairs) + mx(c->vals0.n, cap_pairs) + mx(c->vals1.n, cap_pairs));
+  b += 4 * (mx(c->hist.n, dh::kRadixHistEntries) + mx(c->partials.n, 1024)) + 8 * mx(c->ranges.n, n_tiles);
+  b += 4 * (mx(c->tile_cnt.n, 2 * n_tiles) + mx(c->tile_start.n, n_tiles) + mx(c->vis.n, n_pixels));
+  b += sizeof(dh::Status) * mx(c->status.n, 2) + sizeof(dh::DevDraw) * mx(c->draws.n, n_draws) +
+       sizeof(dh::DevView) * mx(c->views.n, n_views);
+  return b;
+}
+
 // Grow every scratch buffer to the given capacities.
 int ensure_scratch(dh_ctx c, uint32_t n_in, uint32_t cap_tri, uint32_t cap_pairs, uint32_t n_tiles,
                    uint32_t n_pixels, uint32_t n_draws, uint32_t n_views) {
@@ -203,7 +229,7 @@ int ensure_scratch(dh_ctx c, uint32_t n_in, uint32_t cap_tri, uint32_t cap_pairs
   HIPCHK(c, c->tile_cnt.ensure(2ull * n_tiles));
   HIPCHK(c, c->tile_start.ensure(n_tiles));
   HIPCHK(c, c->vis.ensure(n_pixels));
-  HIPCHK(c, c->status.ensure(1));
+  HIPCHK(c, c->status.ensure(2));  // opaque pass, transparent pass
   HIPCHK(c, c->draws.ensure(n_draws));
   HIPCHK(c, c->views.ensure(n_views));
   c->cap_tri = (uint32_t)std::min<size_t>(c->setup.n, 0xFFFFFFF0u);
@@ -241,7 +267,7 @@ int ctx_new(int32_t device, std::shared_ptr<ResPool> pool, dh_ctx* out) {
   c->device = device;
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_status), sizeof(dh::Status));
+  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_status), 2 * sizeof(dh::Status));
   for (int i = 0; i < 5 && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
   if (e == hipSuccess && !pool) {
     pool = std::make_shared<ResPool>();
@@ -297,6 +323,7 @@ extern "C" int dh_ctx_destroy(dh_ctx c) {
   c->hist.release(); c->partials.release(); c->ranges.release(); c->vis.release();
   c->tile_cnt.release(); c->tile_start.release();
   c->status.release(); c->draws.release(); c->views.release(); c->export_buf.release();
+  c->tdraws.release(); c->grp.release(); c->tperm.release(); c->sort_n.release();
   if (c->h_status) hipHostFree(c->h_status);
   for (int i = 0; i < 5; i++)
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
@@ -435,6 +462,22 @@ extern "C" int dh_mesh_create(dh_ctx c, dh_handle object, const uint32_t* tri, u
   return DH_OK;
 }
 
+extern "C" int dh_mesh_read(dh_ctx c, dh_handle mesh, uint32_t* tri) {
+  if (!c || !tri) return DH_E_ARG;
+  if (handle_kind(mesh) != kMesh) return fail(c, DH_E_HANDLE, "not a mesh handle");
+  MeshRes m;
+  {
+    std::lock_guard<std::mutex> lk(c->res->mu);
+    auto it = c->res->meshes.find(mesh & ((1ull << 56) - 1));
+    if (it == c->res->meshes.end()) return fail(c, DH_E_HANDLE, "unknown mesh");
+    m = it->second;
+  }
+  HIPCHK(c, hipSetDevice(c->device));
+  if (m.n_tri) HIPCHK(c, hipMemcpyAsync(tri, m.tri, 36ull * m.n_tri, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return DH_OK;
+}
+
 extern "C" int dh_canvas_create(dh_ctx c, int32_t W, int32_t H, dh_handle* out) {
   if (!c || !out || W <= 0 || H <= 0 || W > 65535 || H > 65535) return DH_E_ARG;
   std::lock_guard<std::mutex> lk(c->res->mu);
@@ -504,10 +547,16 @@ extern "C" int dh_release(dh_ctx c, dh_handle h) {
 namespace {
 
 // Draw and view tables of a batch; key = everything a captured graph depends on.
-int build_tables(dh_ctx c, const dh_view* views, uint32_t n_views, uint64_t* n_in_out, uint32_t* n_tiles_out,
-                 uint32_t* n_pixels_out, std::vector<uint64_t>* key) {
+int build_tables(dh_ctx c, const dh_view* views, uint32_t n_views, bool allow_transparent, uint64_t* n_in_out,
+                 uint32_t* n_tiles_out, uint32_t* n_pixels_out, std::vector<uint64_t>* key) {
   c->h_draws.clear();
   c->h_views.clear();
+  c->h_tdraws.clear();
+  c->h_tmeshes.clear();
+  c->h_grp_first.clear();
+  c->n_tin = 0;
+  c->max_tmesh = 0;
+  dh_handle last_obj = 0;
   key->clear();
   key->push_back(n_views);
   key->push_back((uint64_t)c->TS << 32 | (uint64_t)c->waves << 1 | (c->ordered_bins ? 1u : 0u));
@@ -553,18 +602,36 @@ int build_tables(dh_ctx c, const dh_view* views, uint32_t n_views, uint64_t* n_i
         slot = it->second.slot;
         alpha = it->second.alpha;
       }
-      if (alpha < 1.0f)
-        return fail(c, DH_E_UNSUPPORTED, "transparent meshes (alpha < 1) are not rendered by dh_render");
+      // Object groups: consecutive meshes of one object, as Scene::render
+      // visits an object's opaque meshes, then its transparent ones.
+      if (i == 0 || m.obj != last_obj) {
+        c->h_grp_first.push_back((uint32_t)n_in);
+        last_obj = m.obj;
+      }
+      const uint32_t group = (uint32_t)c->h_grp_first.size() - 1u;
+      if (alpha < 1.0f && !allow_transparent)
+        return fail(c, DH_E_UNSUPPORTED, "transparent meshes (alpha < 1) need dh_render (one view)");
       if (m.n_tri == 0) continue;
       dh::DevDraw d;
+      memset(&d, 0, sizeof(d));
       d.pos = ot->second.pos;
       d.nrm = ot->second.nrm;
       d.uv = ot->second.uv;
       d.tri = m.tri;
       d.n_tri = m.n_tri;
-      d.tri_begin = (uint32_t)n_in;
       d.material = slot;
       d.view = v;
+      d.group = group;
+      if (alpha < 1.0f) {  // transparent pass: no culling, own submission sequence
+        d.tri_begin = (uint32_t)c->n_tin;
+        d.flags = dh::kDrawNoCull;
+        c->h_tdraws.push_back(d);
+        c->h_tmeshes.push_back(dh::TransMesh{m.tri, m.n_tri, ot->second.pos});
+        c->n_tin += m.n_tri;
+        c->max_tmesh = std::max<uint64_t>(c->max_tmesh, m.n_tri);
+        continue;
+      }
+      d.tri_begin = (uint32_t)n_in;
       c->h_draws.push_back(d);
       n_in += m.n_tri;
     }
@@ -572,7 +639,7 @@ int build_tables(dh_ctx c, const dh_view* views, uint32_t n_views, uint64_t* n_i
     n_tiles += (uint64_t)dv.ntx * dv.nty;
     n_pixels += (uint64_t)fr->W * (uint64_t)fr->H;
   }
-  if (n_in > 0x7FFFFFFFull) return fail(c, DH_E_ARG, "too many triangles in one launch");
+  if (n_in > 0x7FFFFFFFull || c->n_tin > 0x7FFFFFFFull) return fail(c, DH_E_ARG, "too many triangles in one launch");
   if (n_tiles > 0x7FFFFFFFull || n_pixels > 0x7FFFFFFFull) return fail(c, DH_E_ARG, "too many pixels in one launch");
   if (c->h_draws.empty()) {  // keep the draw table non-empty
     dh::DevDraw d;
@@ -634,7 +701,7 @@ void drop_graph(dh_ctx c) {
   c->graph_key.clear();
 }
 
-int render_views(dh_ctx c, const dh_view* views, uint32_t n_views, bool use_graph) {
+int render_views(dh_ctx c, const dh_view* views, uint32_t n_views, bool use_graph, bool allow_transparent) {
   HIPCHK(c, hipSetDevice(c->device));
   uint64_t n_in = 0;
   uint32_t n_tiles = 0, n_pixels = 0;
@@ -643,27 +710,49 @@ int render_views(dh_ctx c, const dh_view* views, uint32_t n_views, bool use_grap
   dh::DevMaterial* mats = nullptr;
   {
     std::lock_guard<std::mutex> lk(c->res->mu);
-    rc = build_tables(c, views, n_views, &n_in, &n_tiles, &n_pixels, &key);
+    rc = build_tables(c, views, n_views, allow_transparent, &n_in, &n_tiles, &n_pixels, &key);
     if (rc) return rc;
     rc = upload_materials(c);
     if (rc) return rc;
     mats = c->res->mat_dev.p;
   }
+  const bool transparent = !c->h_tdraws.empty();
+  if (transparent) use_graph = false;
+  const uint64_t n_max = std::max<uint64_t>(n_in, c->n_tin);
   key.push_back((uint64_t)(uintptr_t)mats);
-  uint64_t want_tri = std::max<uint64_t>(c->cap_tri, n_in + n_in / 4 + 1024);
-  uint64_t want_pairs = std::max<uint64_t>(c->cap_pairs, 4 * want_tri + n_tiles);
+  uint64_t want_tri = std::max<uint64_t>(c->cap_tri, n_max + n_max / 4 + 1024);
+  uint64_t want_pairs = std::max<uint64_t>(std::max<uint64_t>(c->cap_pairs, 4 * want_tri + n_tiles), c->max_tmesh);
   c->stats = dh_stats{};
-  c->stats.tris_in = n_in;
+  c->stats.tris_in = n_in + c->n_tin;
   for (int attempt = 0; attempt < 6; attempt++) {
     want_tri = std::min<uint64_t>(want_tri, 0xFFFFFFF0ull);
     want_pairs = std::min<uint64_t>(want_pairs, 0xFFFFFFF0ull);
-    rc = ensure_scratch(c, (uint32_t)n_in, (uint32_t)want_tri, (uint32_t)want_pairs, n_tiles, n_pixels,
+    if (c->scratch_cap) {
+      const uint64_t need = scratch_bytes_after(c, n_max, want_tri, want_pairs, n_tiles, n_pixels,
+                                                c->h_draws.size(), c->h_views.size());
+      if (need > c->scratch_cap) {
+        c->scratch_need = need;
+        return fail(c, DH_E_BUDGET, "frame scratch would exceed the context's cap");
+      }
+    }
+    rc = ensure_scratch(c, (uint32_t)n_max, (uint32_t)want_tri, (uint32_t)want_pairs, n_tiles, n_pixels,
                         (uint32_t)c->h_draws.size(), (uint32_t)c->h_views.size());
     if (rc) return rc;
     HIPCHK(c, hipMemcpyAsync(c->draws.p, c->h_draws.data(), c->h_draws.size() * sizeof(dh::DevDraw),
                              hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->views.p, c->h_views.data(), c->h_views.size() * sizeof(dh::DevView),
                              hipMemcpyHostToDevice, c->stream));
+    const uint32_t n_groups = (uint32_t)c->h_grp_first.size();
+    if (transparent) {
+      HIPCHK(c, c->tdraws.ensure(c->h_tdraws.size()));
+      HIPCHK(c, c->grp.ensure(2ull * n_groups));
+      HIPCHK(c, c->tperm.ensure(9ull * c->max_tmesh));
+      HIPCHK(c, c->sort_n.ensure(1));
+      HIPCHK(c, hipMemcpyAsync(c->tdraws.p, c->h_tdraws.data(), c->h_tdraws.size() * sizeof(dh::DevDraw),
+                               hipMemcpyHostToDevice, c->stream));
+      HIPCHK(c, hipMemcpyAsync(c->grp.p, c->h_grp_first.data(), n_groups * sizeof(uint32_t), hipMemcpyHostToDevice,
+                               c->stream));
+    }
     // everything a captured launch sequence bakes in
     std::vector<uint64_t> gkey = key;
     const void* ptrs[] = {c->cnt.p, c->blk.p, c->setup.p, c->shade.p, c->bincnt.p, c->binoff.p, c->big.p,
@@ -678,21 +767,50 @@ int render_views(dh_ctx c, const dh_view* views, uint32_t n_views, bool use_grap
       HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
       if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
     } else {
-      dh::enqueue_frame(make_args(c, mats, (uint32_t)n_in, n_tiles, n_pixels, c->profiling), c->stream);
+      dh::PipeArgs oa = make_args(c, mats, (uint32_t)n_in, n_tiles, n_pixels, c->profiling);
+      if (transparent) oa.ev[4] = nullptr;  // the frame ends after the transparent pass
+      dh::enqueue_frame(oa, c->stream);
+      if (transparent) {
+        // transparent pass (scene/mod.rs:1088-1247): sort, geometry without
+        // culling, ordered binning, per-tile blend
+        dh::TransArgs ta;
+        memset(&ta, 0, sizeof(ta));
+        ta.meshes = c->h_tmeshes.data();
+        ta.n_meshes = (uint32_t)c->h_tmeshes.size();
+        for (int k = 0; k < 3; k++) ta.cam[k] = c->h_views[0].fr.cam[k];
+        ta.sort_n = c->sort_n.p;
+        ta.perm_tmp = c->tperm.p;
+        ta.grp_first = c->grp.p;
+        ta.grp_base = c->grp.p + n_groups;
+        ta.n_groups = n_groups;
+        ta.pass = make_args(c, mats, (uint32_t)c->n_tin, n_tiles, n_pixels, false);
+        ta.pass.draws = c->tdraws.p;
+        ta.pass.n_draws = (uint32_t)c->h_tdraws.size();
+        ta.pass.status = c->status.p + 1;
+        ta.pass.ordered_bins = true;  // blending needs every tile list in submission order
+        dh::enqueue_transparent_prep(oa, ta, c->stream);
+        const dh::PipeArgs tb = dh::enqueue_front(ta.pass, c->stream);
+        dh::enqueue_blend(tb, ta, c->stream);
+        if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
+      }
       HIPCHK(c, hipGetLastError());
     }
-    HIPCHK(c, hipMemcpyAsync(c->h_status, c->status.p, sizeof(dh::Status), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->h_status, c->status.p, 2 * sizeof(dh::Status), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    const dh::Status st = *c->h_status;
-    c->stats.tris_out = st.n_out;
-    c->stats.bin_entries = st.n_pairs;
+    const dh::Status st = c->h_status[0];
+    const dh::Status st2 = transparent ? c->h_status[1] : dh::Status{};
+    c->stats.tris_out = (uint64_t)st.n_out + st2.n_out;
+    c->stats.bin_entries = (uint64_t)st.n_pairs + st2.n_pairs;
     c->stats.frags = st.frags;
-    if (st.overflow) {
+    if (st.overflow || st2.overflow) {
       c->stats.retries++;
-      if (st.overflow & 1u) want_tri = (uint64_t)st.n_out + st.n_out / 8 + 1024;
-      if (st.overflow & 2u) want_pairs = (uint64_t)st.n_pairs + st.n_pairs / 8 + 4096;
+      const uint64_t n_out = std::max<uint32_t>(st.n_out, st2.n_out);
+      const uint64_t n_pairs = std::max<uint32_t>(st.n_pairs, st2.n_pairs);
+      if ((st.overflow | st2.overflow) & 1u) want_tri = n_out + n_out / 8 + 1024;
+      if ((st.overflow | st2.overflow) & 2u) want_pairs = n_pairs + n_pairs / 8 + 4096;
       // bin entries are only exact once the triangle records fit
-      if (st.overflow & 1u) want_pairs = std::max<uint64_t>(want_pairs, 4 * want_tri + n_tiles);
+      if ((st.overflow | st2.overflow) & 1u) want_pairs = std::max<uint64_t>(want_pairs, 4 * want_tri + n_tiles);
+      want_pairs = std::max<uint64_t>(want_pairs, c->max_tmesh);
       drop_graph(c);
       continue;
     }
@@ -701,7 +819,8 @@ int render_views(dh_ctx c, const dh_view* views, uint32_t n_views, bool use_grap
         c->stats.ms_geometry = c->stats.ms_binning = c->stats.ms_raster = c->stats.ms_shade = 0.0f;
       } else {
         float ms[4];
-        for (int i = 0; i < 4; i++) hipEventElapsedTime(&ms[i], c->ev[i], c->ev[i + 1]);
+        for (int i = 0; i < 3; i++) hipEventElapsedTime(&ms[i], c->ev[i], c->ev[i + 1]);
+        hipEventElapsedTime(&ms[3], c->ev[3], c->ev[4]);  // shade (+ the transparent pass)
         c->stats.ms_geometry = ms[0];
         c->stats.ms_binning = ms[1];
         c->stats.ms_raster = ms[2];
@@ -720,7 +839,8 @@ int render_views(dh_ctx c, const dh_view* views, uint32_t n_views, bool use_grap
       HIPCHK(c, hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
       c->graph_key = gkey;
     }
-    if (st.error & 1u) return fail(c, DH_E_TEXCOORD, "texture coordinate out of range (get_rgb_slice would panic)");
+    if ((st.error | st2.error) & 1u)
+      return fail(c, DH_E_TEXCOORD, "texture coordinate out of range (get_rgb_slice would panic)");
     return DH_OK;
   }
   return fail(c, DH_E_CAPACITY, "scratch capacity did not converge");
@@ -737,13 +857,13 @@ extern "C" int dh_render(dh_ctx c, const dh_frame_params* fr, const dh_handle* m
   v.canvas = canvas;
   v.meshes = meshes;
   v.n_meshes = n_meshes;
-  return render_views(c, &v, 1, false);
+  return render_views(c, &v, 1, false, true);
 }
 
 extern "C" int dh_render_batch(dh_ctx c, const dh_view* views, uint32_t n_views, uint32_t flags) {
   if (!c || (n_views && !views)) return DH_E_ARG;
   if (n_views == 0) return DH_OK;
-  return render_views(c, views, n_views, (flags & DH_BATCH_GRAPH) != 0);
+  return render_views(c, views, n_views, (flags & DH_BATCH_GRAPH) != 0, false);
 }
 
 namespace {
@@ -832,12 +952,14 @@ uint64_t ctx_scratch_bytes(dh_ctx c) {
 uint64_t scratch_estimate(uint64_t n_in, uint64_t n_tiles, uint64_t n_pixels, uint64_t n_draws,
                           uint64_t n_views) {
   const uint64_t tri = n_in + n_in / 4 + 1024, pairs = 4 * tri + n_tiles;
-  return n_in + 4 * (n_in / 256 + 1) + (sizeof(SetupRec) + sizeof(ShadeRec) + 12) * tri + 16 * pairs +
+  return std::max<uint64_t>(n_in, 1) + 4 * ((n_in + 255) / 256 + 1) + (sizeof(SetupRec) + sizeof(ShadeRec) + 12) * tri + 16 * pairs +
          4ull * kRadixHistEntries + 4096 + 20 * n_tiles + 4 * n_pixels + sizeof(DevDraw) * n_draws +
-         sizeof(DevView) * n_views + 256;
+         sizeof(DevView) * n_views + 2 * sizeof(Status);
 }
 
 int ctx_tile_size(dh_ctx c) { return c->TS; }
+void ctx_set_scratch_cap(dh_ctx c, uint64_t cap) { c->scratch_cap = cap; }
+uint64_t ctx_scratch_need(dh_ctx c) { return c->scratch_need; }
 
 // Triangle count of a mesh handle (DH_E_HANDLE if unknown).
 int mesh_triangles(dh_ctx c, dh_handle mesh, uint32_t* n_tri) {
@@ -863,6 +985,7 @@ void ctx_release_scratch(dh_ctx c) {
   c->hist.release(); c->partials.release(); c->ranges.release(); c->vis.release();
   c->tile_cnt.release(); c->tile_start.release();
   c->status.release(); c->draws.release(); c->views.release(); c->export_buf.release();
+  c->tdraws.release(); c->grp.release(); c->tperm.release(); c->sort_n.release();
   c->cap_tri = c->cap_pairs = 0;
 }
 }  // namespace dh

@@ -50,6 +50,8 @@ struct DevMaterial {
 };
 
 // One draw = one IndexedMesh of one Object, in submission order.
+constexpr uint32_t kDrawNoCull = 1u;
+
 struct DevDraw {
   const float* pos;       // xyz
   const float* nrm;       // xyz
@@ -59,6 +61,8 @@ struct DevDraw {
   uint32_t tri_begin;     // first input-triangle index of this draw
   uint32_t material;      // slot in the material table
   uint32_t view;          // view slot (batched rendering); 0 for one frame
+  uint32_t flags;         // kDrawNoCull: transparent meshes skip back-face culling
+  uint32_t group;         // object group (submission order of Scene::render's object loop)
 };
 
 // Per-view raster geometry (one entry for a single frame).
@@ -90,7 +94,7 @@ static_assert(sizeof(SetupRec) == 96, "SetupRec layout");
 // Shading attributes of one clipped triangle.
 struct alignas(16) ShadeRec {
   float uv[6];            // uA vA uB vB uC vC
-  uint32_t material, pad0;
+  uint32_t material, group;  // group: the draw's object group
   float n[9], l[9], h[9]; // per-vertex normal, light, halfway
   float pad1;
 };

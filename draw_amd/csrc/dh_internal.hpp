@@ -10,6 +10,10 @@ namespace dh {
 uint64_t ctx_scratch_bytes(dh_ctx c);
 uint64_t scratch_estimate(uint64_t n_in, uint64_t n_tiles, uint64_t n_pixels, uint64_t n_draws, uint64_t n_views);
 int ctx_tile_size(dh_ctx c);
+// Frame scratch of the context may not grow past cap bytes (0 = no cap); a
+// frame that would returns DH_E_BUDGET and ctx_scratch_need gives its need.
+void ctx_set_scratch_cap(dh_ctx c, uint64_t cap);
+uint64_t ctx_scratch_need(dh_ctx c);
 int mesh_triangles(dh_ctx c, dh_handle mesh, uint32_t* n_tri);
 void ctx_release_scratch(dh_ctx c);
 hipError_t enqueue_bgra_to_rgba(const uint32_t* src, uint32_t* dst, uint32_t n, hipStream_t s);

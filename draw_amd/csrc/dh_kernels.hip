@@ -166,11 +166,14 @@ __device__ inline int clip_plane(const float* pl, V a, V b, V c, V& o0, V& o1, V
 // (43-90): lateral trivial reject, then near and far full clipping.  Calls
 // emit(k, v0, v1, v2) for each output in order; returns the count.
 template <class V, class Emit>
-__device__ inline int cull_and_clip(const FrameParamsPod& fr, const V& a, const V& b, const V& c, Emit emit) {
-  const V3 cam = v3(fr.cam[0], fr.cam[1], fr.cam[2]);
-  const V3 nrm = cross(sub(b.p, a.p), sub(c.p, b.p));                             // Triangle::calc_normal
-  const V3 ctr = divs(add(add(add(v3(0.0f, 0.0f, 0.0f), a.p), b.p), c.p), 3.0f);  // get_center
-  if (dot(sub(cam, ctr), nrm) <= 0.0f) return 0;
+__device__ inline int cull_and_clip(const FrameParamsPod& fr, const V& a, const V& b, const V& c, bool cull,
+                                    Emit emit) {
+  if (cull) {  // the transparent pass does not cull (scene/mod.rs:1194-1203)
+    const V3 cam = v3(fr.cam[0], fr.cam[1], fr.cam[2]);
+    const V3 nrm = cross(sub(b.p, a.p), sub(c.p, b.p));                             // Triangle::calc_normal
+    const V3 ctr = divs(add(add(add(v3(0.0f, 0.0f, 0.0f), a.p), b.p), c.p), 3.0f);  // get_center
+    if (dot(sub(cam, ctr), nrm) <= 0.0f) return 0;
+  }
   if (!(part_visible(fr.planes[2], a.p, b.p, c.p) && part_visible(fr.planes[3], a.p, b.p, c.p) &&
         part_visible(fr.planes[4], a.p, b.p, c.p) && part_visible(fr.planes[5], a.p, b.p, c.p)))
     return 0;
@@ -208,7 +211,7 @@ __device__ inline void rect_axis(uint64_t a0, uint64_t a1, uint64_t smax, uint32
 // tie-break products (660-666) and the scan rectangle (640-658).  Writes the
 // SetupRec and ShadeRec of one clipped triangle with 16-byte stores.
 __device__ inline void make_setup(const FrameParamsPod& fr, const FV& A, const FV& B, const FV& C, uint32_t material,
-                                  const DevMaterial& mat, uint32_t view, SetupRec* __restrict__ rout,
+                                  const DevMaterial& mat, uint32_t view, uint32_t group, SetupRec* __restrict__ rout,
                                   ShadeRec* __restrict__ sout) {
   const float* M = fr.M;
   float X[3], Y[3];
@@ -314,7 +317,7 @@ __device__ inline void make_setup(const FrameParamsPod& fr, const FV& A, const F
   // shading attributes: uv[6], material, pad, n[9], l[9], h[9], pad
   float4* so = reinterpret_cast<float4*>(sout);
   so[0] = make_float4(A.u, A.v, B.u, B.v);
-  so[1] = make_float4(C.u, C.v, __uint_as_float(material), 0.0f);
+  so[1] = make_float4(C.u, C.v, __uint_as_float(material), __uint_as_float(group));
   so[2] = make_float4(A.n.x, A.n.y, A.n.z, B.n.x);
   so[3] = make_float4(B.n.y, B.n.z, C.n.x, C.n.y);
   so[4] = make_float4(C.n.z, A.l.x, A.l.y, A.l.z);
@@ -339,7 +342,7 @@ __global__ __launch_bounds__(256) void k_geom_count(const DevDraw* __restrict__ 
     a.p = ld3(dr.pos, ti[0]);
     b.p = ld3(dr.pos, ti[1]);
     cc.p = ld3(dr.pos, ti[2]);
-    c = (uint32_t)cull_and_clip(fr, a, b, cc, [](int, const PV&, const PV&, const PV&) {});
+    c = (uint32_t)cull_and_clip(fr, a, b, cc, !(dr.flags & kDrawNoCull), [](int, const PV&, const PV&, const PV&) {});
     cnt[t] = (uint8_t)c;
   }
   uint32_t total;
@@ -383,14 +386,14 @@ __global__ __launch_bounds__(256) void k_geom_emit(const DevDraw* __restrict__ d
     v[k].u = uv[0];
     v[k].v = uv[1];
   }
-  const uint32_t material = dr.material, view = dr.view;
-  cull_and_clip(fr, v[0], v[1], v[2], [&](int k, const FV& A, const FV& B, const FV& C) {
+  const uint32_t material = dr.material, view = dr.view, group = dr.group;
+  cull_and_clip(fr, v[0], v[1], v[2], !(dr.flags & kDrawNoCull), [&](int k, const FV& A, const FV& B, const FV& C) {
     const uint32_t idx = base + (uint32_t)k;
     if (idx >= cap_tri) {
       atomicOr(&status->overflow, 1u);
       return;
     }
-    make_setup(fr, A, B, C, material, mats[material], view, setup + idx, shade + idx);
+    make_setup(fr, A, B, C, material, mats[material], view, group, setup + idx, shade + idx);
   });
 }
 
@@ -857,10 +860,90 @@ __global__ __launch_bounds__(256) void k_tile_ranges(const uint32_t* __restrict_
   }
 }
 
+// ======================================================== transparent pass
+// First opaque clipped-triangle id of each object group (the same prefix as
+// k_view_id_base, at the group's first opaque input triangle).
+__global__ void k_group_base(const uint32_t* __restrict__ grp_first, uint32_t n_groups, uint32_t n_in,
+                             const uint8_t* __restrict__ cnt, const uint32_t* __restrict__ blk_off,
+                             const Status* __restrict__ status, uint32_t* __restrict__ grp_base) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n_groups) return;
+  const uint32_t t0 = grp_first[g];
+  uint32_t base;
+  if (t0 >= n_in) {
+    base = status->n_out;
+  } else {
+    const uint32_t b = t0 >> 8;
+    base = blk_off[b];
+    for (uint32_t t = b << 8; t < t0; t++) base += cnt[t];
+  }
+  grp_base[g] = base;
+}
+
+// Sort keys of a transparent mesh's triangles (scene/mod.rs:1100-1115): the
+// distance from the camera to the centroid (a + b + c) / 3, ordered by
+// f32::total_cmp reversed.  key = ~(order-preserving bits of the distance),
+// so an ascending stable sort gives the descending total order with ties in
+// the current triangle order.
+__global__ __launch_bounds__(256) void k_tsort_keys(const uint32_t* __restrict__ tri, uint32_t n,
+                                                    const float* __restrict__ pos, float cx, float cy, float cz,
+                                                    uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t* t9 = tri + 9ull * i;
+  const V3 a = ld3(pos, t9[0]), b = ld3(pos, t9[1]), c = ld3(pos, t9[2]);
+  const V3 ctr = divs(add(add(a, b), c), 3.0f);
+  const float d = norm(sub(ctr, v3(cx, cy, cz)));  // Vec3::dist, linalg.rs:177-180
+  const uint32_t u = __float_as_uint(d);
+  const uint32_t k = (u & 0x80000000u) ? ~u : (u | 0x80000000u);  // total order as unsigned
+  keys[i] = ~k;
+  vals[i] = i;
+}
+
+// out[i] = tri[order[i]] (9 indices per triangle).
+__global__ __launch_bounds__(256) void k_permute_tris(const uint32_t* __restrict__ tri,
+                                                      const uint32_t* __restrict__ order, uint32_t n,
+                                                      uint32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t* src = tri + 9ull * order[i];
+  uint32_t* dst = out + 9ull * i;
+#pragma unroll
+  for (int k = 0; k < 9; k++) dst[k] = src[k];
+}
+
 // ============================================================= orchestration
 static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
-void enqueue_frame(const PipeArgs& a, hipStream_t s) {
+// Stable LSD radix sort of (keys[0], vals[0]) by the low key_bits bits of the
+// keys, *n_ptr items (at most cap): one 11-bit pass when key_bits <= 11, else
+// 8-bit passes.  Returns the index of the buffer pair holding the result.
+int enqueue_radix_sort(uint32_t* const keys[2], uint32_t* const vals[2], const uint32_t* n_ptr, uint32_t cap,
+                       int key_bits, uint32_t* hist, uint32_t* partials, hipStream_t s) {
+  int cur = 0;
+  const bool one_pass = key_bits <= 11;
+  const int step = one_pass ? 11 : 8;
+  for (int shift = 0; shift < key_bits; shift += step) {
+    const uint32_t nh = (one_pass ? 2048u : 256u) * kRadixW;
+    if (one_pass)
+      hipLaunchKernelGGL(k_radix_hist<11>, dim3(kRadixG), dim3(256), 0, s, keys[cur], n_ptr, cap, shift, hist);
+    else
+      hipLaunchKernelGGL(k_radix_hist<8>, dim3(kRadixG), dim3(256), 0, s, keys[cur], n_ptr, cap, shift, hist);
+    scan_u32(hist, hist, nullptr, nh, nh, partials, nullptr, 0, nullptr, 0, s);
+    if (one_pass)
+      hipLaunchKernelGGL(k_radix_scatter<11>, dim3(kRadixG), dim3(256), 0, s, keys[cur], vals[cur], keys[cur ^ 1],
+                         vals[cur ^ 1], n_ptr, cap, shift, hist);
+    else
+      hipLaunchKernelGGL(k_radix_scatter<8>, dim3(kRadixG), dim3(256), 0, s, keys[cur], vals[cur], keys[cur ^ 1],
+                         vals[cur ^ 1], n_ptr, cap, shift, hist);
+    cur ^= 1;
+  }
+  return cur;
+}
+
+// Status reset, geometry, view bases and binning of one pipeline pass;
+// returns the arguments the per-tile stages read (tile lists in b.vals[0]).
+PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
   if (a.ev[0]) (void)hipEventRecord(a.ev[0], s);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(a.status), 0u,
                      (uint32_t)(sizeof(Status) / 4));
@@ -895,26 +978,8 @@ void enqueue_frame(const PipeArgs& a, hipStream_t s) {
                        wflag, a.TS, a.binoff, a.tile_start, a.tile_cur, a.cap_pairs, a.keys[0], a.vals[0]);
     // stable radix sort by tile key (one 11-bit pass when the key fits, else
     // 8-bit passes), then the tile ranges
-    int cur = 0;
-    const bool one_pass = a.key_bits <= 11;
-    const int step = one_pass ? 11 : 8;
-    for (int shift = 0; shift < a.key_bits; shift += step) {
-      const uint32_t nh = (one_pass ? 2048u : 256u) * kRadixW;
-      if (one_pass)
-        hipLaunchKernelGGL(k_radix_hist<11>, dim3(kRadixG), dim3(256), 0, s, a.keys[cur], &a.status->n_pairs,
-                           a.cap_pairs, shift, a.hist);
-      else
-        hipLaunchKernelGGL(k_radix_hist<8>, dim3(kRadixG), dim3(256), 0, s, a.keys[cur], &a.status->n_pairs,
-                           a.cap_pairs, shift, a.hist);
-      scan_u32(a.hist, a.hist, nullptr, nh, nh, a.partials, nullptr, 0, nullptr, 0, s);
-      if (one_pass)
-        hipLaunchKernelGGL(k_radix_scatter<11>, dim3(kRadixG), dim3(256), 0, s, a.keys[cur], a.vals[cur],
-                           a.keys[cur ^ 1], a.vals[cur ^ 1], &a.status->n_pairs, a.cap_pairs, shift, a.hist);
-      else
-        hipLaunchKernelGGL(k_radix_scatter<8>, dim3(kRadixG), dim3(256), 0, s, a.keys[cur], a.vals[cur],
-                           a.keys[cur ^ 1], a.vals[cur ^ 1], &a.status->n_pairs, a.cap_pairs, shift, a.hist);
-      cur ^= 1;
-    }
+    const int cur = enqueue_radix_sort(a.keys, a.vals, &a.status->n_pairs, a.cap_pairs, a.key_bits, a.hist,
+                                       a.partials, s);
     b.vals[0] = a.vals[cur];
     hipLaunchKernelGGL(k_fill_u32, dim3(std::min<uint32_t>(cdiv(2ull * a.n_tiles, 256), 2048u)), dim3(256), 0, s,
                        reinterpret_cast<uint32_t*>(a.ranges), 0u, 2u * a.n_tiles);
@@ -937,7 +1002,34 @@ void enqueue_frame(const PipeArgs& a, hipStream_t s) {
                        a.tile_start, a.tile_cnt, a.n_tiles, a.cap_pairs, a.ranges);
   }
   if (a.ev[2]) (void)hipEventRecord(a.ev[2], s);
-  // 9 raster, 10 shade (dh_raster.hip)
+  return b;
+}
+
+// Before the transparent pass: object-group bases of the opaque ids (from
+// the opaque pass's counts, still in cnt/blk), then each transparent mesh
+// sorted back to front in place -- the mesh keeps the new order, as the
+// reference's sort_by on the mesh's triangle vector does.
+void enqueue_transparent_prep(const PipeArgs& opaque, const TransArgs& t, hipStream_t s) {
+  if (t.n_groups)
+    hipLaunchKernelGGL(k_group_base, dim3(cdiv(t.n_groups, 256)), dim3(256), 0, s, t.grp_first, t.n_groups,
+                       opaque.n_tri_in, opaque.cnt, opaque.blk, opaque.status, t.grp_base);
+  for (uint32_t m = 0; m < t.n_meshes; m++) {
+    const TransMesh& tm = t.meshes[m];
+    if (tm.n_tri < 2) continue;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, t.sort_n, tm.n_tri, 1u);
+    hipLaunchKernelGGL(k_tsort_keys, dim3(cdiv(tm.n_tri, 256)), dim3(256), 0, s, tm.tri, tm.n_tri, tm.pos,
+                       t.cam[0], t.cam[1], t.cam[2], opaque.keys[0], opaque.vals[0]);
+    const int cur = enqueue_radix_sort(opaque.keys, opaque.vals, t.sort_n, opaque.cap_pairs, 32, opaque.hist,
+                                       opaque.partials, s);
+    hipLaunchKernelGGL(k_permute_tris, dim3(cdiv(tm.n_tri, 256)), dim3(256), 0, s, tm.tri, opaque.vals[cur],
+                       tm.n_tri, t.perm_tmp);
+    (void)hipMemcpyAsync(tm.tri, t.perm_tmp, 36ull * tm.n_tri, hipMemcpyDeviceToDevice, s);
+  }
+}
+
+void enqueue_frame(const PipeArgs& a, hipStream_t s) {
+  const PipeArgs b = enqueue_front(a, s);
+  // raster, shade (dh_raster.hip)
   enqueue_raster(b, s);
   if (a.ev[3]) (void)hipEventRecord(a.ev[3], s);
   enqueue_shade(a, s);

@@ -44,7 +44,32 @@ struct PipeArgs {
   hipEvent_t ev[5];    // optional stage events (all null when not profiling)
 };
 
+// One transparent mesh (device pointers): its triangles, sorted in place.
+struct TransMesh {
+  uint32_t* tri;
+  uint32_t n_tri;
+  const float* pos;
+};
+
+// The transparent pass of a one-view frame (scene/mod.rs:1088-1247).
+struct TransArgs {
+  const TransMesh* meshes;     // host array
+  uint32_t n_meshes;
+  float cam[3];
+  uint32_t* sort_n;            // [1] device scratch: item count of a sort
+  uint32_t* perm_tmp;          // [9 * max transparent mesh triangles]
+  const uint32_t* grp_first;   // [n_groups] first opaque input triangle of each object group
+  uint32_t* grp_base;          // [n_groups] -> first opaque clipped-triangle id of each group
+  uint32_t n_groups;
+  PipeArgs pass;               // geometry + binning of the transparent draws (own Status)
+};
+
+PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s);
 void enqueue_frame(const PipeArgs& a, hipStream_t s);
+int enqueue_radix_sort(uint32_t* const keys[2], uint32_t* const vals[2], const uint32_t* n_ptr, uint32_t cap,
+                       int key_bits, uint32_t* hist, uint32_t* partials, hipStream_t s);
+void enqueue_transparent_prep(const PipeArgs& opaque, const TransArgs& t, hipStream_t s);
+void enqueue_blend(const PipeArgs& b, const TransArgs& t, hipStream_t s);
 bool raster_config_supported(int TS, int waves);
 // Per-pixel stages (dh_raster.hip): a.vals[0] must hold the sorted bin values.
 void enqueue_raster(const PipeArgs& a, hipStream_t s);

@@ -388,6 +388,123 @@ __global__ __launch_bounds__(256) void k_shade(const DevView* __restrict__ views
 }
 
 
+// ========================================================= transparent blend
+// Transparent fragments (Canvas::draw_pixel_coord_with_depth with opacity < 1,
+// canvas.rs:906-930; Pixel Mul/Add, canvas.rs:136-169): per channel
+// bg*(1 - a) and c*a each truncated to u8, summed with u8 wrap-around, pad 0.
+__device__ inline uint32_t blend_px(uint32_t bg, uint32_t c, float a) {
+  const float ia = 1.0f - a;
+  uint32_t out = 0;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const uint32_t p = f2u8((float)((bg >> (8 * i)) & 255u) * ia);
+    const uint32_t q = f2u8((float)((c >> (8 * i)) & 255u) * a);
+    out |= ((p + q) & 255u) << (8 * i);
+  }
+  return out;  // pad byte 0
+}
+
+// One workgroup per tile of a one-view frame.  The opaque pass has left each
+// pixel's final colour, depth D and winner.  Every later opaque fragment of
+// the reference's sequence loses to that winner, so from the winner on the
+// depth buffer holds D: a transparent fragment matters iff its object group
+// is not before the winner's (transparent meshes are drawn after their own
+// object's opaque meshes, scene/mod.rs:1088) and its depth is < D.  Lanes own
+// pixels and apply the tile's transparent triangles in submission order.
+template <int TS>
+__global__ __launch_bounds__(256) void k_blend(const DevView* __restrict__ views, const uint2* __restrict__ ranges,
+                                               const uint32_t* __restrict__ vals, const SetupRec* __restrict__ setup,
+                                               const ShadeRec* __restrict__ shade,
+                                               const DevMaterial* __restrict__ mats,
+                                               const uint32_t* __restrict__ grp_base, uint32_t n_groups,
+                                               Status* __restrict__ status) {
+  constexpr int PPT = (TS * TS + 255) / 256;
+  constexpr int CH = 64;
+  __shared__ SetupRec s_rec[CH];
+  __shared__ uint32_t s_id[CH];
+  const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+  const DevView& vw = views[0];
+  const int W = vw.fr.W, H = vw.fr.H;
+  const int tx0 = (int)(tile % vw.ntx) * TS, ty0 = (int)(tile / vw.ntx) * TS;
+  const int tid = threadIdx.x;
+  uint32_t col[PPT];
+  float D[PPT];
+  int gw[PPT];
+  bool ok[PPT];
+#pragma unroll
+  for (int q = 0; q < PPT; q++) {
+    const int p = tid + 256 * q;
+    const int px = tx0 + p % TS, py = ty0 + p / TS;
+    ok[q] = p < TS * TS && px < W && py < H;
+    col[q] = 0;
+    D[q] = 0.0f;
+    gw[q] = -1;
+    if (ok[q]) {
+      col[q] = vw.out_bgra[(uint32_t)(H - 1 - py) * (uint32_t)W + (uint32_t)px];
+      D[q] = vw.out_depth[(uint32_t)py * (uint32_t)W + (uint32_t)px];
+      const uint32_t w = vw.out_winner[(uint32_t)py * (uint32_t)W + (uint32_t)px];
+      if (w != kNone) {  // last group whose first id is <= w
+        uint32_t lo = 0, hi = n_groups;
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (grp_base[mid] <= w) lo = mid;
+          else hi = mid;
+        }
+        gw[q] = (int)lo;
+      }
+    }
+  }
+  const uint2 rg = ranges[tile];
+  for (uint32_t base = rg.x; base < rg.y; base += CH) {
+    const int n = (int)min((uint32_t)CH, rg.y - base);
+    __syncthreads();
+    for (int i = tid; i < n * 6; i += 256) {
+      const int k = i / 6, w = i % 6;
+      const uint32_t id = vals[base + k];
+      if (w == 0) s_id[k] = id;
+      reinterpret_cast<uint4*>(s_rec + k)[w] = reinterpret_cast<const uint4*>(setup + id)[w];
+    }
+    __syncthreads();
+    for (int k = 0; k < n; k++) {
+      const SetupRec& r = s_rec[k];
+      if (r.flags & kDrop) continue;
+      const int rx0 = (int)(r.rx & 0xFFFFu), rx1 = (int)(r.rx >> 16);
+      const int ry0 = (int)(r.ry & 0xFFFFu), ry1 = (int)(r.ry >> 16);
+#pragma unroll
+      for (int q = 0; q < PPT; q++) {
+        const int p = tid + 256 * q;
+        const int px = tx0 + p % TS, py = ty0 + p / TS;
+        if (!ok[q] || px < rx0 || px > rx1 || py < ry0 || py > ry1) continue;
+        const float xf = (float)px, yf = (float)py;
+        bool cov;
+        FragOut o;
+        if (r.flags & kFast) {
+          cov = covered_fast(edge_eval(r.ea, xf, yf), r.f[0], r.flags & kTieA) &&
+                covered_fast(edge_eval(r.eb, xf, yf), r.f[1], r.flags & kTieB) &&
+                covered_fast(edge_eval(r.eg, xf, yf), r.f[2], r.flags & kTieG);
+          if (cov) o = bary(r, xf, yf);
+        } else {
+          o = bary(r, xf, yf);
+          cov = o.al >= 0.0f && o.be >= 0.0f && o.ga >= 0.0f && (o.al > 0.0f || (r.flags & kTieA)) &&
+                (o.be > 0.0f || (r.flags & kTieB)) && (o.ga > 0.0f || (r.flags & kTieG));
+        }
+        if (!cov) continue;
+        // both maps are sampled before the depth test (canvas.rs:689-695)
+        const ShadeRec& sr = shade[s_id[k]];
+        const DevMaterial& m = mats[sr.material];
+        const uint32_t c = shade_pixel(sr, m, o, status);
+        if (o.d < D[q] && (int)sr.group >= gw[q]) col[q] = blend_px(col[q], c, m.alpha);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < PPT; q++) {
+    const int p = tid + 256 * q;
+    const int px = tx0 + p % TS, py = ty0 + p / TS;
+    if (ok[q]) vw.out_bgra[(uint32_t)(H - 1 - py) * (uint32_t)W + (uint32_t)px] = col[q];
+  }
+}
+
 // ============================================================ launch helpers
 static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
@@ -420,6 +537,19 @@ void enqueue_raster(const PipeArgs& a, hipStream_t s) {
     else if (qpw == 8) launch_raster<32, 8>(a, s);
     else launch_raster<32, 16>(a, s);
   }
+}
+
+void enqueue_blend(const PipeArgs& b, const TransArgs& t, hipStream_t s) {
+  const dim3 g(b.n_tiles), blk(256);
+  if (b.TS == 8)
+    hipLaunchKernelGGL(k_blend<8>, g, blk, 0, s, b.views, b.ranges, b.vals[0], b.setup, b.shade, b.mats,
+                       t.grp_base, t.n_groups, b.status);
+  else if (b.TS == 16)
+    hipLaunchKernelGGL(k_blend<16>, g, blk, 0, s, b.views, b.ranges, b.vals[0], b.setup, b.shade, b.mats,
+                       t.grp_base, t.n_groups, b.status);
+  else
+    hipLaunchKernelGGL(k_blend<32>, g, blk, 0, s, b.views, b.ranges, b.vals[0], b.setup, b.shade, b.mats,
+                       t.grp_base, t.n_groups, b.status);
 }
 
 void enqueue_shade(const PipeArgs& a, hipStream_t s) {

@@ -40,6 +40,7 @@ struct Job {
   uint32_t flags = 0;
   uint64_t n_tri = 0;
   uint64_t canvas_bytes = 0;
+  uint64_t need = 0;  // scratch a launch of this job alone asked for (0 = not known)
   int32_t status = DH_JOB_QUEUED;
   uint32_t batch_views = 0;
   dh_handle canvas = 0;
@@ -119,7 +120,8 @@ void worker_loop(dh_server s, Worker* w) {
       const uint64_t est = dh::scratch_estimate(tris + j->n_tri, tiles + tiles_of(j->fr.W, j->fr.H, TS),
                                                 px + (uint64_t)j->fr.W * j->fr.H, draws + j->meshes.size(),
                                                 batch.size() + 1);
-      const uint64_t want_scratch = std::max<uint64_t>(dh::ctx_scratch_bytes(w->ctx), est);
+      const uint64_t want_scratch =
+          std::max<uint64_t>(std::max<uint64_t>(dh::ctx_scratch_bytes(w->ctx), est), j->need);
       const uint64_t scratch_extra = want_scratch > w->scratch_reserved ? want_scratch - w->scratch_reserved : 0;
       auto fc = s->free_canvases.find(std::make_pair(j->fr.W, j->fr.H));
       const uint64_t canvas_extra = (fc == s->free_canvases.end()) ? j->canvas_bytes : 0;
@@ -163,6 +165,22 @@ void worker_loop(dh_server s, Worker* w) {
         w->scratch_reserved = 0;
         s->epoch++;
         s->cv_work.notify_all();
+      } else if (s->running == 0 && s->cfg.budget_bytes && !s->stop) {
+        // Nothing runs and this worker holds no scratch.  If no unreleased
+        // result holds a canvas and no other worker holds scratch, nothing
+        // will ever free memory: the front job cannot run.
+        bool held = s->scratch_bytes != 0;
+        for (auto& kv : s->jobs) held |= kv.second->canvas != 0;
+        if (!held) {
+          Job* j = s->queue.front();
+          s->queue.pop_front();
+          j->status = DH_E_BUDGET;
+          j->t_done = s->now_ms();
+          s->st.failed++;
+          s->epoch++;
+          s->cv_done.notify_all();
+          s->cv_work.notify_all();
+        }
       } else if (s->stop && s->running == 0) {
         // Shutting down and nothing running will free memory: the rest
         // cannot be admitted.
@@ -188,11 +206,50 @@ void worker_loop(dh_server s, Worker* w) {
       views[k].n_meshes = (uint32_t)batch[k]->meshes.size();
     }
     s->running++;
-    lk.unlock();
-    const int rc = dh_render_batch(w->ctx, views.data(), (uint32_t)views.size(), 0);
-    const uint64_t scratch_now = dh::ctx_scratch_bytes(w->ctx);
-    lk.lock();
+    // Launch with the scratch capped at this worker's reservation.  A launch
+    // whose bin lists outgrow the estimate asks for more: grow the
+    // reservation if the budget allows, else hand back all but the first job
+    // and relaunch; a single job that cannot grow goes back to the queue
+    // front and waits for memory.
+    int rc;
+    for (;;) {
+      dh::ctx_set_scratch_cap(w->ctx, s->cfg.budget_bytes ? w->scratch_reserved : 0);
+      lk.unlock();
+      rc = dh_render_batch(w->ctx, views.data(), (uint32_t)views.size(), 0);
+      lk.lock();
+      if (rc != DH_E_BUDGET) break;
+      const uint64_t need = dh::ctx_scratch_need(w->ctx);
+      const uint64_t extra = need > w->scratch_reserved ? need - w->scratch_reserved : 0;
+      if (!s->fits(extra)) evict_for(s, extra, 0, 0);
+      if (s->fits(extra)) {
+        s->scratch_bytes += extra;
+        w->scratch_reserved += extra;
+        s->note_peak();
+        continue;
+      }
+      const size_t keep = batch.size() > 1 ? 1 : 0;
+      for (size_t k = batch.size(); k-- > keep;) {  // back to the queue front, in order
+        Job* j = batch[k];
+        s->free_canvases.emplace(std::make_pair(j->fr.W, j->fr.H), j->canvas);
+        j->canvas = 0;
+        j->status = DH_JOB_QUEUED;
+        s->queue.push_front(j);
+      }
+      if (keep == 0) batch[0]->need = need;  // admit it again only with room for that
+      batch.resize(keep);
+      views.resize(keep);
+      if (keep == 0) break;
+    }
     s->running--;
+    const uint64_t scratch_now = dh::ctx_scratch_bytes(w->ctx);
+    if (batch.empty()) {
+      // the front job did not fit: wait for completions or releases
+      s->scratch_bytes = s->scratch_bytes - w->scratch_reserved + scratch_now;
+      w->scratch_reserved = scratch_now;
+      s->epoch++;
+      s->cv_work.notify_all();
+      continue;
+    }
     // the reservation follows the buffers actually held
     s->scratch_bytes = s->scratch_bytes - w->scratch_reserved + scratch_now;
     w->scratch_reserved = scratch_now;

@@ -130,6 +130,22 @@ class Renderer:
             self._canvas[key] = h.value
         return self._canvas[key]
 
+    def mesh_triangles(self, handle: int, n_tri: int) -> np.ndarray:
+        """A mesh's current triangle triplets, (n_tri, 9) uint32 (dh_mesh_read)."""
+        tri = np.empty((n_tri, 9), np.uint32)
+        self._check(self.L.dh_mesh_read(self.ctx, int(handle), tri.ctypes.data), "dh_mesh_read")
+        return tri
+
+    def sync_transparent_order(self, scene: Scene):
+        """Copy the device order of every transparent mesh back into the host
+        meshes: the reference sorts a transparent mesh's triangle vector in
+        place each frame (scene/mod.rs:1100), and so does the device."""
+        for o in scene.objects:
+            handles = self.upload(o)
+            for m, h in zip(o.meshes, handles):
+                if o.material(m).alpha < 1.0:
+                    m.triangles = self.mesh_triangles(h, m.triangles.shape[0])
+
     def mesh_handles(self, scene: Scene) -> List[int]:
         hs: List[int] = []
         for o in scene.objects:

@@ -367,6 +367,27 @@ def batched_scenes(seed: int = 4, n_meshes: int = 8, cams_per_mesh: int = 32, tr
     return views
 
 
+def window_object(n: int, half: float, z: float, texture: Texture, x0: float = 0.0, y0: float = 0.0) -> Object:
+    """An n x n grid of quads (2n^2 triangles) in the plane z = ``z``, centred
+    on (x0, y0), facing +z, UVs spanning [0, 1]^2.  Symmetric about x = x0, so
+    mirrored triangles have equal centroid distances from a camera on that
+    plane of symmetry (ties for the transparent sort)."""
+    g = np.linspace(-half, half, n + 1, dtype=np.float64)
+    xs, ys = np.meshgrid(g + x0, g + y0, indexing="xy")
+    verts = np.stack([xs.ravel(), ys.ravel(), np.full(xs.size, z)], 1).astype(F32)
+    uv = np.stack([(xs.ravel() - x0 + half) / (2 * half), (ys.ravel() - y0 + half) / (2 * half)], 1)
+    uv = np.clip(uv, 0.0, 1.0).astype(F32)
+    tris = []
+    for j in range(n):
+        for i in range(n):
+            a, b = j * (n + 1) + i, j * (n + 1) + i + 1
+            c, d = (j + 1) * (n + 1) + i + 1, (j + 1) * (n + 1) + i
+            for t in ((a, b, c), (a, c, d)):   # counter-clockwise seen from +z
+                tris.append([t[0], t[1], t[2], t[0], t[1], t[2], 0, 0, 0])
+    normals = np.array([[0.0, 0.0, 1.0]], F32)
+    return Object(verts, normals, uv, [IndexedMesh(np.array(tris, np.uint32), 0)], [texture])
+
+
 SERVING_RESOLUTIONS = [((854, 480), 0.4), ((1280, 720), 0.3), ((1920, 1080), 0.25), ((3840, 2160), 0.05)]
 
 

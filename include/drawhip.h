@@ -144,6 +144,11 @@ int dh_object_create(dh_ctx ctx, const float* pos, uint32_t n_pos, const float* 
  * material 0 = Texture::default (scene/mod.rs:1072-1075). */
 int dh_mesh_create(dh_ctx ctx, dh_handle object, const uint32_t* tri, uint32_t n_tri, dh_handle material,
                    dh_handle* out);
+/* Copy a mesh's triangle triplets back (9 * n_tri u32).  The transparent
+ * pass sorts a transparent mesh's triangles in place every frame, as the
+ * reference's sort_by on the mesh's triangle vector does (scene/mod.rs:1100),
+ * so a caller that mirrors its meshes on the host reads the new order here. */
+int dh_mesh_read(dh_ctx ctx, dh_handle mesh, uint32_t* tri);
 /* Device-resident Canvas (canvas.rs:353-433): BGRA8 frame + f32 depth +
  * u32 winner id per pixel. */
 int dh_canvas_create(dh_ctx ctx, int32_t W, int32_t H, dh_handle* out);
@@ -151,8 +156,12 @@ int dh_release(dh_ctx ctx, dh_handle h);
 
 /* ---- rendering -------------------------------------------------------- */
 /* Scene::render (scene/mod.rs:901-1249) of the given meshes, in submission
- * order, into a device canvas.  Opaque meshes (material alpha >= 1) only;
- * returns when the frame is complete on the device. */
+ * order, into a device canvas; returns when the frame is complete on the
+ * device.  Consecutive meshes of one object form an object group: as the
+ * reference's object loop does, a group's opaque meshes are drawn first, in
+ * list order, then its transparent meshes (material alpha < 1), each sorted
+ * back to front in place (scene/mod.rs:1088-1247), without back-face culling
+ * and without depth writes. */
 int dh_render(dh_ctx ctx, const dh_frame_params* fr, const dh_handle* meshes, uint32_t n_meshes,
               dh_handle canvas);
 /* One view of a batch: uniforms, target canvas (same size as fr), meshes in
@@ -173,7 +182,9 @@ enum { DH_BATCH_GRAPH = 1u };
  * and pixels.  DH_BATCH_GRAPH captures that launch sequence in a hipGraph and
  * replays it while the batch keeps the same structure (views, sizes, canvases,
  * meshes); uniforms may change between replays.  No counterpart in the
- * reference, which renders one frame per call (app/mod.rs:96-235). */
+ * reference, which renders one frame per call (app/mod.rs:96-235).
+ * Transparent meshes are not accepted here (DH_E_UNSUPPORTED): their in-place
+ * sort is a per-frame sequence. */
 int dh_render_batch(dh_ctx ctx, const dh_view* views, uint32_t n_views, uint32_t flags);
 
 /* Copy a canvas to host memory: bgra W*H*4 bytes, rows top-down

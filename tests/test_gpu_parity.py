@@ -65,6 +65,18 @@ def test_soup_1m_1080p_sampled_bit_exact(renderer):
         assert_same(f, d, w, ref, mask=grid_mask(1080, 1920, grid), label=f"soup1M grid {grid}")
 
 
+def test_shells_20m_4k_sampled_bit_exact(renderer):
+    """The 3840x2160 stress workload: ~20M micro-triangles on 8 nested shells
+    (5% flipped, 5% outside the frustum, 8192x8192 texture) against the oracle
+    on two pixel grids, and the emitted-triangle count exactly."""
+    s = workloads.shells_scene(3)
+    f, d, w = _gpu(renderer, s)
+    for grid in [(7, 5, 1, 2), (16, 16, 3, 5)]:
+        ref = oracle(s, grid=grid)
+        assert_same(f, d, w, ref, mask=grid_mask(2160, 3840, grid), label=f"shells20M grid {grid}")
+    assert renderer.stats()["tris_out"] == ref.stats["tris_emitted"]
+
+
 def test_soup_1m_tiling_invariance(renderer):
     s = workloads.soup_scene(2)
     base = _gpu(renderer, s, 16, 4)

@@ -82,3 +82,44 @@ def test_poisson_stream_completes(renderer):
         assert st["queued"] == 0 and st["completed"] == out["requests"]
     finally:
         srv.close()
+
+
+def _stacked_quads(n=40, width=1280, height=720):
+    """n full-screen quads at slightly different depths: every triangle lands
+    in every tile, far beyond the 4 bin entries per triangle a launch is
+    first sized for."""
+    from draw_amd.scene import Camera, IndexedMesh, Object, Scene, Texture
+    quads = [workloads.window_object(1, 3.0, -0.01 * k, Texture()) for k in range(n)]
+    verts = np.concatenate([q.vertices for q in quads])
+    uv = np.concatenate([q.texture_vertices for q in quads])
+    tri = np.concatenate([q.meshes[0].triangles + np.array([4 * k] * 6 + [0] * 3, np.uint32)
+                          for k, q in enumerate(quads)])
+    obj = Object(verts, quads[0].normals_vertices, uv, [IndexedMesh(tri, 0)], [Texture()])
+    cam = Camera(position=(0.0, 0.0, 3.0), direction=None, fov_x=60.0, near=-0.1, far=-100.0)
+    return Scene(width, height, cam, (0.0, 6.0, 6.0), [obj])
+
+
+def test_launch_outgrowing_its_estimate_stays_within_budget(renderer):
+    s = _stacked_quads()
+    fr, hs = frame_uniforms(s), renderer.mesh_handles(s)
+    budget = 256 << 20
+    srv = Server(renderer, budget_bytes=budget, workers=1)
+    try:
+        t = srv.submit(fr, hs)
+        res = srv.poll(t, wait=True)
+        assert res.status == 0
+        got = srv.read(res, s.width, s.height)
+        srv.release(t)
+        assert srv.stats()["bytes_peak"] <= budget
+    finally:
+        srv.close()
+    c, w = renderer.render(s, frame=fr)
+    assert np.array_equal(got[0], c.frame) and np.array_equal(got[2], w)
+    assert_same(c.frame, c.depth, w, oracle(s, grid=(9, 7, 2, 3)),
+                mask=_grid(s, (9, 7, 2, 3)), label="stacked quads")
+
+
+def _grid(s, g):
+    m = np.zeros((s.height, s.width), bool)
+    m[g[3]::g[1], g[2]::g[0]] = True
+    return m
