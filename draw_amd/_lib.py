@@ -64,6 +64,24 @@ class DhServerStats(C.Structure):
                 ("queued", C.c_uint64)]
 
 
+class DhObjSummary(C.Structure):
+    _fields_ = [("n_pos", C.c_uint32), ("n_nrm", C.c_uint32), ("n_uv", C.c_uint32), ("n_meshes", C.c_uint32),
+                ("n_textures", C.c_uint32), ("pad", C.c_uint32), ("pos", C.POINTER(C.c_float)),
+                ("nrm", C.POINTER(C.c_float)), ("uv", C.POINTER(C.c_float))]
+
+
+class DhObjMeshDesc(C.Structure):
+    _fields_ = [("tri", C.POINTER(C.c_uint32)), ("n_tri", C.c_uint32), ("texture_idx", C.c_uint32),
+                ("name", C.c_char_p)]
+
+
+class DhObjTextureDesc(C.Structure):
+    _fields_ = [("ka", C.c_float * 3), ("kd", C.c_float * 3), ("ks", C.c_float * 3), ("alpha", C.c_float),
+                ("name", C.c_char_p), ("map_kd", C.POINTER(C.c_uint8)), ("kd_w", C.c_int32), ("kd_h", C.c_int32),
+                ("kd_comp", C.c_int32), ("map_ka", C.POINTER(C.c_uint8)), ("ka_w", C.c_int32), ("ka_h", C.c_int32),
+                ("ka_comp", C.c_int32)]
+
+
 DH_JOB_AUTORELEASE = 1
 DH_JOB_DONE, DH_JOB_QUEUED, DH_JOB_RUNNING = 0, 1, 2
 
@@ -102,6 +120,12 @@ SIGNATURES = {
     "dh_canvas_export_png": [_P, _H, C.c_char_p],
     "dh_write_png": [C.c_char_p, _P, C.c_int32, C.c_int32],
     "dh_get_stats": [_P, C.POINTER(DhStats)],
+    "dh_obj_load": [C.c_char_p, C.POINTER(_P), C.c_char_p, C.c_uint32],
+    "dh_obj_info": [_P, C.POINTER(DhObjSummary)],
+    "dh_obj_mesh": [_P, C.c_uint32, C.POINTER(DhObjMeshDesc)],
+    "dh_obj_texture": [_P, C.c_uint32, C.POINTER(DhObjTextureDesc)],
+    "dh_obj_upload": [_P, _P, C.POINTER(_H), C.POINTER(_H), C.c_uint32],
+    "dh_obj_free": [_P],
     "dh_srv_create": [_P, C.POINTER(DhServerConfig), C.POINTER(_P)],
     "dh_srv_submit": [_P, C.POINTER(DhJob), C.POINTER(C.c_uint64)],
     "dh_srv_poll": [_P, C.c_uint64, C.c_int32, C.POINTER(DhJobInfo)],

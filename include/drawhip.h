@@ -209,6 +209,53 @@ int dh_get_stats(dh_ctx ctx, dh_stats* out);
 /* Enable (1) or disable (0) per-stage HIP event timing of dh_render. */
 int dh_set_profiling(dh_ctx ctx, int32_t enable);
 
+/* ---- OBJ / MTL ingest ------------------------------------------------- */
+/* Object::load_from_file (object.rs:106-419) on the host: OBJ + MTL parsing
+ * (the reference uses the `obj` crate), PNG textures (the reference uses
+ * stb_image; JPEG is not decoded here: DH_E_UNSUPPORTED), normals normalised,
+ * positions rescaled to a largest norm of 100, quads split (a,b,c)+(c,d,a),
+ * a dummy texture coordinate per group lacking them, generated normals per
+ * group lacking them.  Textures: index 0 is Texture::default, then every
+ * material of every library in file order; a mesh's texture is the first
+ * with its group's material name. */
+typedef struct dh_obj_s* dh_obj;
+
+typedef struct {
+    uint32_t n_pos, n_nrm, n_uv, n_meshes, n_textures;
+    uint32_t pad;
+    const float* pos;   /* xyz per vertex, after rescaling    */
+    const float* nrm;   /* xyz per normal                     */
+    const float* uv;    /* uv per texture coordinate          */
+} dh_obj_summary;
+
+typedef struct {
+    const uint32_t* tri;  /* 9 per triangle: pos a,b,c; uv a,b,c; nrm a,b,c */
+    uint32_t n_tri;
+    uint32_t texture_idx;
+    const char* name;     /* the OBJ group name                                */
+} dh_obj_mesh_desc;
+
+typedef struct {
+    float ka[3], kd[3], ks[3];
+    float alpha;               /* MTL d (default 1)                           */
+    const char* name;
+    const uint8_t* map_kd;     /* NULL = TextureMap::default                  */
+    int32_t kd_w, kd_h, kd_comp;
+    const uint8_t* map_ka;
+    int32_t ka_w, ka_h, ka_comp;
+} dh_obj_texture_desc;
+
+/* Parse path (and the material libraries and textures it names, relative to
+ * its directory).  On failure err (if given) receives a message. */
+int dh_obj_load(const char* path, dh_obj* out, char* err, uint32_t err_len);
+int dh_obj_info(dh_obj obj, dh_obj_summary* out);
+int dh_obj_mesh(dh_obj obj, uint32_t i, dh_obj_mesh_desc* out);
+int dh_obj_texture(dh_obj obj, uint32_t i, dh_obj_texture_desc* out);
+/* Upload into ctx's resident pool: one object, its textures and materials,
+ * and one mesh per OBJ group (meshes[cap], in order). */
+int dh_obj_upload(dh_ctx ctx, dh_obj obj, dh_handle* object, dh_handle* meshes, uint32_t cap);
+int dh_obj_free(dh_obj obj);
+
 /* ---- serving ---------------------------------------------------------- */
 /* A render server with continuous batching.  No counterpart in the reference,
  * whose app loop renders one frame per iteration (app/mod.rs:96-235); each job
