@@ -82,6 +82,15 @@ class DhObjTextureDesc(C.Structure):
                 ("ka_comp", C.c_int32)]
 
 
+class DhGuiVertex(C.Structure):
+    _fields_ = [("pos", C.c_float * 2), ("uv", C.c_float * 2), ("col", C.c_uint8 * 4)]
+
+
+class DhGuiCmd(C.Structure):
+    _fields_ = [("clip_rect", C.c_float * 4), ("idx_offset", C.c_uint32), ("count", C.c_uint32),
+                ("vtx_offset", C.c_uint32), ("pad", C.c_uint32)]
+
+
 DH_JOB_AUTORELEASE = 1
 DH_JOB_DONE, DH_JOB_QUEUED, DH_JOB_RUNNING = 0, 1, 2
 
@@ -120,6 +129,7 @@ SIGNATURES = {
     "dh_canvas_export_png": [_P, _H, C.c_char_p],
     "dh_write_png": [C.c_char_p, _P, C.c_int32, C.c_int32],
     "dh_get_stats": [_P, C.POINTER(DhStats)],
+    "dh_draw_gui": [_P, _H, _P, C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, _H],
     "dh_obj_load": [C.c_char_p, C.POINTER(_P), C.c_char_p, C.c_uint32],
     "dh_obj_info": [_P, C.POINTER(DhObjSummary)],
     "dh_obj_mesh": [_P, C.c_uint32, C.POINTER(DhObjMeshDesc)],

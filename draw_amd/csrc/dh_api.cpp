@@ -120,6 +120,10 @@ struct dh_ctx_s {
   DevBuf<dh::DevDraw> draws;
   DevBuf<dh::DevView> views;
   DevBuf<uint32_t> export_buf;  // RGBA staging of dh_canvas_read_rgba
+  DevBuf<dh_gui_vertex> gui_vtx;  // GUI overlay draw list
+  DevBuf<uint4> gui_tri, gui_clip;
+  DevBuf<uint4> gui_rec;          // dh::kGuiRecBytes per triangle
+  DevBuf<uint32_t> gui_err;
   uint64_t scratch_cap = 0;     // 0 = none; else frame scratch may not grow past it (DH_E_BUDGET)
   uint64_t scratch_need = 0;    // bytes the last frame refused by the cap asked for
   uint32_t cap_tri = 0, cap_pairs = 0;
@@ -324,6 +328,7 @@ extern "C" int dh_ctx_destroy(dh_ctx c) {
   c->tile_cnt.release(); c->tile_start.release();
   c->status.release(); c->draws.release(); c->views.release(); c->export_buf.release();
   c->tdraws.release(); c->grp.release(); c->tperm.release(); c->sort_n.release();
+  c->gui_vtx.release(); c->gui_tri.release(); c->gui_clip.release(); c->gui_rec.release(); c->gui_err.release();
   if (c->h_status) hipHostFree(c->h_status);
   for (int i = 0; i < 5; i++)
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
@@ -918,6 +923,61 @@ extern "C" int dh_canvas_export_png(dh_ctx c, dh_handle canvas, const char* path
   return rc ? fail(c, rc, std::string("cannot write ") + path) : DH_OK;
 }
 
+extern "C" int dh_draw_gui(dh_ctx c, dh_handle canvas, const dh_gui_vertex* vtx, uint32_t n_vtx, const uint32_t* idx,
+                           uint32_t n_idx, const dh_gui_cmd* cmds, uint32_t n_cmds, dh_handle font) {
+  if (!c || (n_vtx && !vtx) || (n_idx && !idx) || (n_cmds && !cmds)) return DH_E_ARG;
+  CanvasRes cv;
+  int rc = find_canvas(c, canvas, &cv);
+  if (rc) return rc;
+  TexRes ft;
+  {
+    std::lock_guard<std::mutex> lk(c->res->mu);
+    auto it = handle_kind(font) == kTex ? c->res->tex.find(font & ((1ull << 56) - 1)) : c->res->tex.end();
+    if (it == c->res->tex.end()) return fail(c, DH_E_HANDLE, "font is not a texture handle");
+    ft = it->second;
+  }
+  // Gui::render (gui.rs:382-478): commands in order; an inverted clip rect
+  // skips its command; the clip rect's y is flipped (H - y) and cast to usize.
+  std::vector<uint4> tris, clips;
+  const float Hf = (float)cv.H;
+  for (uint32_t ci = 0; ci < n_cmds; ci++) {
+    const dh_gui_cmd& cm = cmds[ci];
+    const float* cr = cm.clip_rect;
+    if (cr[0] > cr[2] || cr[1] > cr[3]) continue;
+    auto cl = [](float v) { return (uint32_t)std::min<uint64_t>(dh::f2usize(v), 0x7FFFFFFFull); };  // order-preserving
+    clips.push_back(make_uint4(cl(cr[0]), cl(Hf - cr[1]), cl(cr[2]), cl(Hf - cr[3])));
+    if ((uint64_t)cm.idx_offset + cm.count > n_idx) return fail(c, DH_E_INDEX, "command indices past the index buffer");
+    for (uint32_t t = 0; t + 3 <= cm.count; t += 3) {
+      uint32_t v[3];
+      for (int k = 0; k < 3; k++) {
+        const uint64_t vi = (uint64_t)idx[cm.idx_offset + t + k] + cm.vtx_offset;
+        if (vi >= n_vtx) return fail(c, DH_E_INDEX, "vertex index past the vertex buffer");
+        v[k] = (uint32_t)vi;
+      }
+      tris.push_back(make_uint4(v[0], v[1], v[2], (uint32_t)clips.size() - 1u));
+    }
+  }
+  if (tris.empty()) return DH_OK;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, c->gui_vtx.ensure(n_vtx));
+  HIPCHK(c, c->gui_tri.ensure(tris.size()));
+  HIPCHK(c, c->gui_clip.ensure(clips.size()));
+  HIPCHK(c, c->gui_rec.ensure(tris.size() * (dh::kGuiRecBytes / sizeof(uint4))));
+  HIPCHK(c, c->gui_err.ensure(1));
+  HIPCHK(c, hipMemcpyAsync(c->gui_vtx.p, vtx, n_vtx * sizeof(dh_gui_vertex), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->gui_tri.p, tris.data(), tris.size() * sizeof(uint4), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->gui_clip.p, clips.data(), clips.size() * sizeof(uint4), hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipMemsetAsync(c->gui_err.p, 0, 4, c->stream));
+  HIPCHK(c, dh::enqueue_gui(c->gui_vtx.p, c->gui_tri.p, (uint32_t)tris.size(), c->gui_clip.p, c->gui_rec.p, cv.W,
+                            cv.H, ft.d, ft.w, ft.h, ft.comp, cv.bgra, cv.depth, c->gui_err.p, c->stream));
+  uint32_t err = 0;
+  HIPCHK(c, hipMemcpyAsync(&err, c->gui_err.p, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (err) return fail(c, DH_E_TEXCOORD, "font sample outside the texture (get_rgba_slice would panic)");
+  return DH_OK;
+}
+
 extern "C" int dh_canvas_device_ptrs(dh_ctx c, dh_handle canvas, void** bgra, void** depth, void** winner) {
   if (!c) return DH_E_ARG;
   CanvasRes cv;
@@ -986,6 +1046,7 @@ void ctx_release_scratch(dh_ctx c) {
   c->tile_cnt.release(); c->tile_start.release();
   c->status.release(); c->draws.release(); c->views.release(); c->export_buf.release();
   c->tdraws.release(); c->grp.release(); c->tperm.release(); c->sort_n.release();
+  c->gui_vtx.release(); c->gui_tri.release(); c->gui_clip.release(); c->gui_rec.release(); c->gui_err.release();
   c->cap_tri = c->cap_pairs = 0;
 }
 }  // namespace dh

@@ -16,5 +16,9 @@ void ctx_set_scratch_cap(dh_ctx c, uint64_t cap);
 uint64_t ctx_scratch_need(dh_ctx c);
 int mesh_triangles(dh_ctx c, dh_handle mesh, uint32_t* n_tri);
 void ctx_release_scratch(dh_ctx c);
+hipError_t enqueue_gui(const dh_gui_vertex* vtx, const uint4* tris, uint32_t n_tri, const uint4* clips, void* recs,
+                       int W, int H, const uint8_t* font, int32_t fw, int32_t fh, int32_t fcomp, uint32_t* bgra,
+                       const float* depth, uint32_t* error, hipStream_t s);
+constexpr size_t kGuiRecBytes = 144;  // sizeof(GuiRec)
 hipError_t enqueue_bgra_to_rgba(const uint32_t* src, uint32_t* dst, uint32_t n, hipStream_t s);
 }  // namespace dh

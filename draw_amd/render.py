@@ -204,6 +204,16 @@ class Renderer:
                                           winner.ctypes.data), "dh_canvas_read")
         return bgra, depth, winner
 
+    def draw_gui(self, canvas_handle: int, vtx: np.ndarray, idx: np.ndarray, cmds: np.ndarray, font) -> None:
+        """Gui::render onto a device canvas (dh_draw_gui): vtx/cmds in the
+        draw_amd.gui record layouts, idx u32, font a TextureMap (RGBA)."""
+        from .gui import GUI_CMD, GUI_VERTEX
+        vtx = np.ascontiguousarray(vtx, GUI_VERTEX)
+        idx = np.ascontiguousarray(idx, np.uint32)
+        cmds = np.ascontiguousarray(cmds, GUI_CMD)
+        self._check(self.L.dh_draw_gui(self.ctx, int(canvas_handle), vtx.ctypes.data, len(vtx), idx.ctypes.data,
+                                       len(idx), cmds.ctypes.data, len(cmds), self.texture(font)), "dh_draw_gui")
+
     def read_rgba(self, canvas_handle: int, W: int, H: int) -> np.ndarray:
         """The frame as App::export_frame_as prepares it: RGBA8, rows top-down."""
         rgba = np.empty((H, W, 4), np.uint8)

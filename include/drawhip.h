@@ -209,6 +209,21 @@ int dh_get_stats(dh_ctx ctx, dh_stats* out);
 /* Enable (1) or disable (0) per-stage HIP event timing of dh_render. */
 int dh_set_profiling(dh_ctx ctx, int32_t enable);
 
+/* ---- GUI overlay ------------------------------------------------------ */
+/* Gui::render (app/gui.rs:382-478) over Canvas::draw_triangle
+ * (canvas.rs:435-575): an imgui draw list drawn onto a canvas after the
+ * scene.  Vertices as ImDrawVert (screen position with y down, uv, RGBA
+ * colour), u32 indices, commands (clip rect x0,y0,x1,y1 with y down, index
+ * offset and count, vertex offset) applied in order; font: an RGBA texture
+ * handle sampled with get_rgba_slice.  Fragments blend over the canvas, are
+ * depth-tested against 0 and write no depth.  DH_E_INDEX for indices past
+ * the buffers, DH_E_TEXCOORD when a sample leaves the font image (the
+ * reference panics on both). */
+typedef struct { float pos[2]; float uv[2]; uint8_t col[4]; } dh_gui_vertex;
+typedef struct { float clip_rect[4]; uint32_t idx_offset, count, vtx_offset, pad; } dh_gui_cmd;
+int dh_draw_gui(dh_ctx ctx, dh_handle canvas, const dh_gui_vertex* vtx, uint32_t n_vtx, const uint32_t* idx,
+                uint32_t n_idx, const dh_gui_cmd* cmds, uint32_t n_cmds, dh_handle font);
+
 /* ---- OBJ / MTL ingest ------------------------------------------------- */
 /* Object::load_from_file (object.rs:106-419) on the host: OBJ + MTL parsing
  * (the reference uses the `obj` crate), PNG textures (the reference uses

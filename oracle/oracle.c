@@ -703,3 +703,134 @@ int or_render_range(const or_frame* fr, const or_object* objs, uint32_t n_obj, c
                     uint32_t* winner, or_stats* st) {
     return render_impl(fr, objs, n_obj, mats, n_mat, tri_begin, tri_end, bgra, depth, winner, st);
 }
+
+/* ------------------------------------------------------------ GUI overlay */
+typedef struct { uint64_t x, y, w, h; } rect_t; /* Rectangle, canvas.rs:293-313 */
+static rect_t rect_from_coords(uint64_t x0, uint64_t y0, uint64_t x1, uint64_t y1) { /* canvas.rs:315-330 */
+    rect_t r;
+    uint64_t xa = x0 < x1 ? x0 : x1, xb = x0 < x1 ? x1 : x0;
+    uint64_t ya = y0 < y1 ? y0 : y1, yb = y0 < y1 ? y1 : y0;
+    r.x = xa; r.y = ya; r.w = xb - xa; r.h = yb - ya;
+    return r;
+}
+static rect_t rect_clip(rect_t a, rect_t b) { /* canvas.rs:332-350 */
+    uint64_t x0 = a.x > b.x ? a.x : b.x, y0 = a.y > b.y ? a.y : b.y;
+    uint64_t ax = a.x + a.w, bx = b.x + b.w, ay = a.y + a.h, by = b.y + b.h;
+    uint64_t x1 = ax < bx ? ax : bx, y1 = ay < by ? ay : by;
+    if (x0 > x1) { x0 = 0; x1 = 0; }
+    if (y0 > y1) { y0 = 0; y1 = 0; }
+    return rect_from_coords(x0, y0, x1, y1);
+}
+/* TextureMap::get_rgba_slice scene/mod.rs:137-152, release semantics: the
+ * slice [offset, offset + 4) must lie in the image, else the frame panics. */
+static int rgba_slice(const uint8_t* img, int32_t w, int32_t h, int32_t comp, float u, float v, uint8_t out[4]) {
+    uint64_t ui = f2usize(floorf(u * (float)w));
+    uint64_t vi = ((uint64_t)h - 1u) - f2usize(floorf(v * (float)h));
+    uint64_t off = (vi * (uint64_t)w + ui) * (uint64_t)comp;
+    uint64_t end = off + 4u, len = (uint64_t)w * (uint64_t)h * (uint64_t)comp;
+    if (off > end || end > len) return OR_E_TEXCOORD;
+    memcpy(out, img + off, 4);
+    return OR_OK;
+}
+/* Pixel * f32 (canvas.rs:155-169): each channel truncated to u8. */
+static void px_mul(const uint8_t p[3], float s, uint8_t o[3]) {
+    for (int i = 0; i < 3; i++) o[i] = f2u8((float)p[i] * s);
+}
+static void px_add(const uint8_t a[3], const uint8_t b[3], uint8_t o[3]) { /* canvas.rs:136-153, wrapping */
+    for (int i = 0; i < 3; i++) o[i] = (uint8_t)(a[i] + b[i]);
+}
+
+typedef struct { v2 s, uv; uint8_t rgb[3]; float alpha; } gvert_t; /* VertexSimpleAttributes canvas.rs:186-191 */
+
+static int gui_triangle(int32_t W, int32_t H, const gvert_t* A, const gvert_t* B, const gvert_t* Cv, rect_t clip,
+                        const uint8_t* font, int32_t fw, int32_t fh, int32_t fcomp, uint8_t* bgra,
+                        const float* depth) {
+    /* Canvas::draw_triangle canvas.rs:435-575 (no canvas offset) */
+    v2 a, b, c;
+    a.x = floorf(A->s.x + 0.5f); a.y = floorf(A->s.y + 0.5f); /* pos_map_center canvas.rs:896-904 */
+    b.x = floorf(B->s.x + 0.5f); b.y = floorf(B->s.y + 0.5f);
+    c.x = floorf(Cv->s.x + 0.5f); c.y = floorf(Cv->s.y + 0.5f);
+    rect_t drawable = rect_from_coords(f2usize(min3f(a.x, b.x, c.x)), f2usize(min3f(a.y, b.y, c.y)),
+                                       f2usize(max3f(a.x, b.x, c.x)), f2usize(max3f(a.y, b.y, c.y)));
+    rect_t screen = rect_from_coords(0, 0, (uint64_t)W - 1u, (uint64_t)H - 1u);
+    drawable = rect_clip(drawable, screen);
+    rect_t valid = rect_clip(clip, drawable);
+    float fa = edge_fn(b, c, a.x, a.y), fb = edge_fn(c, a, b.x, b.y), fg = edge_fn(a, b, c.x, c.y);
+    float oa = edge_fn(b, c, -1.0f, -1.0f), ob = edge_fn(c, a, -1.0f, -1.0f), og = edge_fn(a, b, -1.0f, -1.0f);
+    for (uint64_t y = valid.y; y <= valid.y + valid.h; y++) {
+        float yf = (float)y;
+        for (uint64_t x = valid.x; x <= valid.x + valid.w; x++) {
+            float xf = (float)x;
+            float al = edge_fn(b, c, xf, yf) / fa;
+            float be = edge_fn(c, a, xf, yf) / fb;
+            float ga = edge_fn(a, b, xf, yf) / fg;
+            if (!(al >= 0.0f && be >= 0.0f && ga >= 0.0f)) continue;
+            if (!((al > 0.0f || fa * oa > 0.0f) && (be > 0.0f || fb * ob > 0.0f) && (ga > 0.0f || fg * og > 0.0f)))
+                continue;
+            uint8_t t0[3], t1[3], t2[3], col[3];
+            px_mul(A->rgb, al, t0);
+            px_mul(B->rgb, be, t1);
+            px_mul(Cv->rgb, ga, t2);
+            px_add(t0, t1, col);
+            px_add(col, t2, col);
+            float calpha = (al * A->alpha + be * B->alpha) + ga * Cv->alpha;
+            float u = (A->uv.x * al + B->uv.x * be) + Cv->uv.x * ga;
+            float v = (A->uv.y * al + B->uv.y * be) + Cv->uv.y * ga;
+            uint8_t tex[4];
+            if (rgba_slice(font, fw, fh, fcomp, u, v, tex)) return OR_E_TEXCOORD;
+            float ta = (float)tex[3] / 255.0f;
+            uint8_t ct[3] = {tex[0], tex[1], tex[2]}; /* Color::Custom -> Pixel::new(r, g, b) */
+            px_mul(col, ta, t0);
+            px_mul(ct, 1.0f - ta, t1);
+            px_add(t0, t1, col);
+            float fin = calpha * ta;
+            uint8_t out[3];
+            uint64_t W64 = (uint64_t)W, H64 = (uint64_t)H;
+            uint8_t* px = bgra + 4u * (W64 * (H64 - y - 1u) + x);
+            if (fin < 1.0f) { /* draw_pixel_coord_with_depth canvas.rs:906-930 */
+                uint8_t bg[3] = {px[2], px[1], px[0]};
+                px_mul(bg, 1.0f - fin, t0);
+                px_mul(col, fin, t1);
+                px_add(t0, t1, out);
+            } else {
+                memcpy(out, col, 3);
+            }
+            if (0.0f < depth[W64 * y + x]) {
+                px[0] = out[2]; px[1] = out[1]; px[2] = out[0]; px[3] = 0; /* Pixel sums carry pad 0 */
+            }
+        }
+    }
+    return OR_OK;
+}
+
+int or_gui_draw(int32_t W, int32_t H, const or_gui_vertex* vtx, uint32_t n_vtx, const uint32_t* idx,
+                uint32_t n_idx, const or_gui_cmd* cmds, uint32_t n_cmds, const uint8_t* font, int32_t fw,
+                int32_t fh, int32_t fcomp, uint8_t* bgra, const float* depth) {
+    if (W <= 0 || H <= 0 || !bgra || !depth || !font || fw <= 0 || fh <= 0) return OR_E_ARG;
+    for (uint32_t ci = 0; ci < n_cmds; ci++) { /* Gui::render gui.rs:382-478 */
+        const or_gui_cmd* cm = &cmds[ci];
+        float cr[4] = {cm->clip_rect[0], cm->clip_rect[1], cm->clip_rect[2], cm->clip_rect[3]};
+        if (cr[0] > cr[2] || cr[1] > cr[3]) continue;
+        cr[1] = (float)H - cr[1];
+        cr[3] = (float)H - cr[3];
+        rect_t clip = rect_from_coords(f2usize(cr[0]), f2usize(cr[1]), f2usize(cr[2]), f2usize(cr[3]));
+        if ((uint64_t)cm->idx_offset + cm->count > n_idx) return OR_E_INDEX;
+        for (uint32_t t = 0; t + 3 <= cm->count; t += 3) {
+            gvert_t gv[3];
+            for (int k = 0; k < 3; k++) {
+                uint64_t vi = (uint64_t)idx[cm->idx_offset + t + k] + cm->vtx_offset;
+                if (vi >= n_vtx) return OR_E_INDEX;
+                const or_gui_vertex* q = &vtx[vi];
+                gv[k].s.x = q->pos[0];
+                gv[k].s.y = ((float)H - 1.0f) - q->pos[1]; /* invert_y */
+                gv[k].uv.x = q->uv[0];
+                gv[k].uv.y = 1.0f - q->uv[1];
+                memcpy(gv[k].rgb, q->col, 3);
+                gv[k].alpha = (float)q->col[3] / 255.0f;
+            }
+            int e = gui_triangle(W, H, &gv[0], &gv[1], &gv[2], clip, font, fw, fh, fcomp, bgra, depth);
+            if (e) return e;
+        }
+    }
+    return OR_OK;
+}

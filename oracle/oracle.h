@@ -106,6 +106,18 @@ int or_draw_triangle(const or_frame* fr, const or_attr* a, const or_attr* b,
                      uint32_t id, uint8_t* bgra, float* depth, uint32_t* winner,
                      or_stats* st);
 
+/* GUI overlay (f4): one imgui-style draw list through Gui::render
+ * (app/gui.rs:382-478) and Canvas::draw_triangle (canvas.rs:435-575):
+ * vertices (pos, uv, RGBA col), u32 indices, commands (clip rect x0,y0,x1,y1
+ * in imgui's top-down convention, idx_offset, count, vtx_offset), an RGBA
+ * font texture (get_rgba_slice).  Depth is tested (`0 < depth`) but never
+ * written.  Returns OR_OK, OR_E_INDEX or OR_E_TEXCOORD (a panic). */
+typedef struct { float pos[2], uv[2]; uint8_t col[4]; } or_gui_vertex;
+typedef struct { float clip_rect[4]; uint32_t idx_offset, count, vtx_offset, pad; } or_gui_cmd;
+int or_gui_draw(int32_t W, int32_t H, const or_gui_vertex* vtx, uint32_t n_vtx, const uint32_t* idx,
+                uint32_t n_idx, const or_gui_cmd* cmds, uint32_t n_cmds, const uint8_t* font, int32_t fw,
+                int32_t fh, int32_t fcomp, uint8_t* bgra, const float* depth);
+
 /* Canvas::clear (canvas.rs:425-433). */
 void or_clear(const or_frame* fr, uint8_t* bgra, float* depth, uint32_t* winner);
 

@@ -215,3 +215,29 @@ def clip(frame: OrFrame, pos9, attrs):
     out_attr = (OrAttr * 36)()
     n = lib().or_clip(C.byref(frame), pos, at, out_pos, out_attr)
     return n, np.frombuffer(out_pos, np.float32).reshape(12, 3, 3)[:n].copy(), [out_attr[i] for i in range(3 * n)]
+
+
+# ------------------------------------------------------------ GUI overlay
+GUI_VERTEX = np.dtype([("pos", "<f4", (2,)), ("uv", "<f4", (2,)), ("col", "u1", (4,))])          # 20 B
+GUI_CMD = np.dtype([("clip_rect", "<f4", (4,)), ("idx_offset", "<u4"), ("count", "<u4"), ("vtx_offset", "<u4"),
+                    ("pad", "<u4")])                                                               # 32 B
+
+
+def gui_draw(bgra, depth, vtx, idx, cmds, font) -> int:
+    """or_gui_draw on (H, W, 4) BGRA (rows top-down, modified in place) and
+    (H, W) depth (rows bottom-up); font: a TextureMap-like with .img."""
+    L = lib()
+    if not hasattr(L.or_gui_draw, "_set"):
+        L.or_gui_draw.argtypes = [C.c_int32, C.c_int32, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
+                                  C.c_uint32, C.c_void_p, C.c_int32, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p]
+        L.or_gui_draw._set = True
+    H, W = depth.shape
+    vtx = np.ascontiguousarray(vtx, GUI_VERTEX)
+    idx = np.ascontiguousarray(idx, np.uint32)
+    cmds = np.ascontiguousarray(cmds, GUI_CMD)
+    img = np.ascontiguousarray(font.img, np.uint8)
+    depth = np.ascontiguousarray(depth, np.float32)
+    assert bgra.flags.c_contiguous and bgra.dtype == np.uint8
+    return L.or_gui_draw(W, H, vtx.ctypes.data, len(vtx), idx.ctypes.data, len(idx), cmds.ctypes.data, len(cmds),
+                         img.ctypes.data, img.shape[1], img.shape[0], img.shape[2], bgra.ctypes.data,
+                         depth.ctypes.data)

@@ -394,3 +394,105 @@ def clip(planes, pts, at):
                 nxt.append(([Cc, nA, nB], [aC, naA, naB]))
         pool = nxt
     return pool
+
+
+# ---------------------------------------------------------------- GUI overlay
+def _rect_from(x0, y0, x1, y1):  # Rectangle::from_coords canvas.rs:315-330 -> (x, y, w, h)
+    return (min(x0, x1), min(y0, y1), max(x0, x1) - min(x0, x1), max(y0, y1) - min(y0, y1))
+
+
+def _rect_clip(a, b):  # Rectangle::clip canvas.rs:332-350
+    x0, y0 = max(a[0], b[0]), max(a[1], b[1])
+    x1, y1 = min(a[0] + a[2], b[0] + b[2]), min(a[1] + a[3], b[1] + b[3])
+    if x0 > x1:
+        x0 = x1 = 0
+    if y0 > y1:
+        y0 = y1 = 0
+    return _rect_from(x0, y0, x1, y1)
+
+
+def _pmul(p, s):  # Pixel * f32: per channel (c as f32 * s) as u8
+    return [int(_u8(F(c) * F(s))) for c in p]
+
+
+def _padd(a, b):  # Pixel + Pixel: u8 wrapping
+    return [(x + y) & 255 for x, y in zip(a, b)]
+
+
+def gui_draw(bgra, depth, vtx, idx, cmds, font):
+    """Gui::render (gui.rs:382-478) + Canvas::draw_triangle (canvas.rs:435-575),
+    scalar float32; returns 0, -2 (index) or -3 (texture slice)."""
+    H, W = depth.shape
+    img = font.img.reshape(-1)
+    fh, fw, fc = font.img.shape
+
+    def ef(p, q, x, y):
+        return ((F(p[1] - q[1]) * F(x) + F(q[0] - p[0]) * F(y)) + F(p[0] * q[1])) - F(q[0] * p[1])
+
+    for cm in cmds:
+        cr = [F(c) for c in cm["clip_rect"]]
+        if cr[0] > cr[2] or cr[1] > cr[3]:
+            continue
+        cr[1] = F(H) - cr[1]
+        cr[3] = F(H) - cr[3]
+        clip = _rect_from(_usize(cr[0]), _usize(cr[1]), _usize(cr[2]), _usize(cr[3]))
+        io, cnt, vo = int(cm["idx_offset"]), int(cm["count"]), int(cm["vtx_offset"])
+        if io + cnt > len(idx):
+            return -2
+        for t in range(0, cnt - cnt % 3, 3):
+            vs = []
+            for k in range(3):
+                vi = int(idx[io + t + k]) + vo
+                if vi >= len(vtx):
+                    return -2
+                q = vtx[vi]
+                s = (F(q["pos"][0]), (F(H) - F(1.0)) - F(q["pos"][1]))
+                vs.append(dict(c=(np.floor(s[0] + F(0.5)), np.floor(s[1] + F(0.5))),
+                               uv=(F(q["uv"][0]), F(1.0) - F(q["uv"][1])),
+                               rgb=[int(x) for x in q["col"][:3]], a=F(q["col"][3]) / F(255.0)))
+            a, b, c = vs[0]["c"], vs[1]["c"], vs[2]["c"]
+            xs, ys = [a[0], b[0], c[0]], [a[1], b[1], c[1]]
+            def mn(l):  # canvas.rs:491-500 (NaN never replaces the running value)
+                r = F(np.inf)
+                for z in l:
+                    if z < r:
+                        r = z
+                return r
+
+            def mx(l):
+                r = F(-np.inf)
+                for z in l:
+                    if z > r:
+                        r = z
+                return r
+            drawable = _rect_from(_usize(mn(xs)), _usize(mn(ys)), _usize(mx(xs)), _usize(mx(ys)))
+            drawable = _rect_clip(drawable, _rect_from(0, 0, W - 1, H - 1))
+            valid = _rect_clip(clip, drawable)
+            fa, fb, fg = ef(b, c, a[0], a[1]), ef(c, a, b[0], b[1]), ef(a, b, c[0], c[1])
+            oa, ob, og = ef(b, c, -1, -1), ef(c, a, -1, -1), ef(a, b, -1, -1)
+            for y in range(valid[1], valid[1] + valid[3] + 1):
+                for x in range(valid[0], valid[0] + valid[2] + 1):
+                    al, be, ga = ef(b, c, x, y) / fa, ef(c, a, x, y) / fb, ef(a, b, x, y) / fg
+                    if not (al >= 0 and be >= 0 and ga >= 0):
+                        continue
+                    if not ((al > 0 or fa * oa > 0) and (be > 0 or fb * ob > 0) and (ga > 0 or fg * og > 0)):
+                        continue
+                    col = _padd(_padd(_pmul(vs[0]["rgb"], al), _pmul(vs[1]["rgb"], be)), _pmul(vs[2]["rgb"], ga))
+                    calpha = (al * vs[0]["a"] + be * vs[1]["a"]) + ga * vs[2]["a"]
+                    u = (vs[0]["uv"][0] * al + vs[1]["uv"][0] * be) + vs[2]["uv"][0] * ga
+                    vv = (vs[0]["uv"][1] * al + vs[1]["uv"][1] * be) + vs[2]["uv"][1] * ga
+                    ui = _usize(np.floor(u * F(fw)))
+                    vi = ((fh - 1) - _usize(np.floor(vv * F(fh)))) & M64
+                    off = ((vi * fw + ui) * fc) & M64
+                    if off + 4 > fw * fh * fc:
+                        return -3
+                    tex = [int(z) for z in img[off:off + 4]]
+                    ta = F(tex[3]) / F(255.0)
+                    col = _padd(_pmul(col, ta), _pmul(tex[:3], F(1.0) - ta))
+                    fin = calpha * ta
+                    px = bgra[H - 1 - y, x]
+                    out = _padd(_pmul([int(px[2]), int(px[1]), int(px[0])], F(1.0) - fin), _pmul(col, fin)) \
+                        if fin < F(1.0) else col
+                    if F(0.0) < depth[y, x]:
+                        px[0], px[1], px[2], px[3] = out[2], out[1], out[0], 0
+    return 0

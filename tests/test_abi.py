@@ -44,7 +44,8 @@ def test_struct_layouts_match_c_compiler(tmp_path):
                "dh_material_desc": _lib.DhMaterialDesc, "dh_stats": _lib.DhStats, "dh_view": _lib.DhView,
                "dh_server_config": _lib.DhServerConfig, "dh_job": _lib.DhJob, "dh_job_info": _lib.DhJobInfo,
                "dh_server_stats": _lib.DhServerStats, "dh_obj_summary": _lib.DhObjSummary,
-               "dh_obj_mesh_desc": _lib.DhObjMeshDesc, "dh_obj_texture_desc": _lib.DhObjTextureDesc}
+               "dh_obj_mesh_desc": _lib.DhObjMeshDesc, "dh_obj_texture_desc": _lib.DhObjTextureDesc,
+               "dh_gui_vertex": _lib.DhGuiVertex, "dh_gui_cmd": _lib.DhGuiCmd}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "drawhip.h"', "int main(void) {"]
     for cname, ct in structs.items():
         lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
