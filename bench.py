@@ -79,6 +79,12 @@ def measured_traffic(kernel: str):
     return None
 
 
+def raster_kernel_name(kind: int, tile: int, waves: int) -> str:
+    """The instantiated raster kernel: k_raster<tile, quadrants per wave> or
+    k_raster_tp<tile, waves>."""
+    return f"k_raster_tp<{tile}, {waves}>" if kind == 1 else f"k_raster<{tile}, {(tile // 8) ** 2 // waves}>"
+
+
 def roofline(kernel: str, nbytes: dict, ms: float, traffic=None) -> dict:
     gbs = nbytes["algorithmic"] / (ms * 1e-3) / 1e9 if ms > 0 else None
     return {"kernel": kernel, "bound": "hbm", "achieved": round(gbs, 1) if gbs else None, "peak": HBM_PEAK_GBS,
@@ -124,6 +130,7 @@ def bench_batched(reps: int) -> dict:
     meshes = [r.mesh_handles(s) for s in views]
     log("batched: autotune ...")
     best, table = autotune(r, lambda: r.render_batch(items, prepared=prepared), frames=2)
+    kind, tile, waves = best
     # plain path: one frame per launch
     for (s, fr), hs in zip(items[:8], meshes[:8]):
         r.render_device(s, fr, hs)
@@ -155,7 +162,7 @@ def bench_batched(reps: int) -> dict:
         t_vis += tv
     tris = sum(s.triangle_count for s in views)
     px = sum(s.width * s.height for s in views)
-    rast_kernel = f"k_raster<{best[0]}, {(best[0] // 8) ** 2 // best[1]}>"
+    rast_kernel = raster_kernel_name(kind, tile, waves)
     r.close()
     plain_mpix = covered / plain_s / 1e6
     batch_mpix = covered / batch_s / 1e6
@@ -163,7 +170,7 @@ def bench_batched(reps: int) -> dict:
         "workload": "256 views: 8 meshes (tori, UV spheres, soups; 1k-500k triangles log-uniform) x 32 random "
                     "cameras, resolutions uniform over 256x256/640x480/1280x720/1920x1080, 1024x1024 textures",
         "views": len(views), "triangles": tris, "pixels": px, "covered_px": covered,
-        "tile": best[0], "waves_per_tile": best[1], "autotune_ms": table,
+        "raster_kind": kind, "tile": tile, "waves_per_tile": waves, "autotune_ms": table,
         "plain": {"shaded_mpix_per_s": round(plain_mpix, 1), "tris_per_s": round(tris / plain_s, 1),
                   "frame_latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 4),
                                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 4)},
@@ -278,6 +285,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=30)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--raster-kind", type=int, default=0, help="0: quadrant-owning waves, 1: triangle-parallel")
     ap.add_argument("--tile", type=int, default=16)
     ap.add_argument("--waves", type=int, default=4)
     ap.add_argument("--n-tri", type=int, default=1_000_000)
@@ -306,32 +314,41 @@ def main():
     scene = workloads.soup_scene(2, n_tri=args.n_tri)
     fr = frame_uniforms(scene)
     W, H = fr.W, fr.H
-    r = Renderer(0, tile=args.tile, waves=args.waves, profiling=True)
+    r = Renderer(0, profiling=True)
+    r.set_raster(args.raster_kind, args.tile, args.waves)
     meshes = r.mesh_handles(scene)
     tune_table = None
     if args.autotune:
         from draw_amd.autotune import autotune
-        log("autotuning tile size / waves per tile ...")
-        (args.tile, args.waves), tune_table = autotune(r, lambda: r.render_device(scene, fr, meshes))
-        log(f"autotune: {tune_table} -> {args.tile}x{args.waves}")
+        log("autotuning raster kernel / tile size / waves per tile ...")
+        (args.raster_kind, args.tile, args.waves), tune_table = autotune(r, lambda: r.render_device(scene, fr, meshes))
+        log(f"autotune: {tune_table} -> kind {args.raster_kind} {args.tile}x{args.waves}")
     log("uploaded; warmup ...")
     for _ in range(args.warmup):
         r.render_device(scene, fr, meshes)
-    lat, st_acc = [], []
+    lat, graph_ms = [], []
     log(f"timing {args.steps} frames ...")
     t0 = time.perf_counter()
     for _ in range(args.steps):
         a = time.perf_counter()
         cv = r.render_device(scene, fr, meshes)
         lat.append(time.perf_counter() - a)
-        st_acc.append(r.stats())
+        graph_ms.append(r.stats()["ms_frame"])
     total = time.perf_counter() - t0
     _, _, winner = r.read(cv, W, H)
     p_cov, t_vis = visible_triangles(winner)
+    # per-stage HIP event times need eager launches (a replayed graph records
+    # only the whole frame): the same frame again, untimed, without the graph
+    r.set_graph(False)
+    st_acc = []
+    for _ in range(max(3, min(args.steps, 10))):
+        r.render_device(scene, fr, meshes)
+        st_acc.append(r.stats())
+    r.set_graph(True)
     st = {k: float(np.mean([s[k] for s in st_acc])) for k in st_acc[0]}
     frame_ms = total / args.steps * 1e3
     value = p_cov * args.steps / total / 1e6
-    rast_kernel = f"k_raster<{args.tile}, {(args.tile // 8) ** 2 // args.waves}>"
+    rast_kernel = raster_kernel_name(args.raster_kind, args.tile, args.waves)
     roof_raster = roofline(rast_kernel, raster_bytes(W * H, int(st["bin_entries"])), st["ms_raster"],
                            measured_traffic(rast_kernel))
     roof_shade = roofline("k_shade", shade_bytes(W * H, p_cov, t_vis), st["ms_shade"], measured_traffic("k_shade"))
@@ -340,13 +357,17 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(frame_ms, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "plain GPU path, 1920x1080, seeded 1M-triangle soup (20% near-plane straddlers, "
-                               "log-normal area median 40 px), 2048x2048 RGBA8 texture, one frame per dh_render",
-                   "width": W, "height": H, "triangles": scene.triangle_count, "tile": args.tile,
+                               "log-normal area median 40 px), 2048x2048 RGBA8 texture, one frame per dh_render "
+                               "(launch sequence replayed from a hipGraph)",
+                   "width": W, "height": H, "triangles": scene.triangle_count, "raster_kind": args.raster_kind,
+                   "tile": args.tile,
                    "waves_per_tile": args.waves, "autotune_ms": tune_table},
         "tris_per_s": round(scene.triangle_count * args.steps / total, 1),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 4),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 4)},
-        "stage_ms": {k: round(st[k], 4) for k in ("ms_geometry", "ms_binning", "ms_raster", "ms_shade", "ms_frame")},
+        "device_ms_per_frame": round(float(np.mean(graph_ms)), 4),
+        "stage_ms_eager": {k: round(st[k], 4) for k in ("ms_geometry", "ms_binning", "ms_raster", "ms_shade",
+                                                        "ms_frame")},
         "frame": {"covered_px": p_cov, "visible_tris": t_vis, "clipped_tris": int(st["tris_out"]),
                   "bin_entries": int(st["bin_entries"]), "fragments": int(st["frags"]),
                   "frags_per_px": round(st["frags"] / (W * H), 2)},

@@ -102,6 +102,8 @@ struct dh_ctx_s {
   std::shared_ptr<ResPool> res;
   // raster configuration
   int TS = 16, waves = 4;
+  int raster_kind = 0;
+  bool graph_frames = true;  // dh_render replays a captured launch sequence while the frame structure repeats
   bool profiling = false;
   hipEvent_t ev[5] = {};
   // scratch
@@ -310,6 +312,7 @@ extern "C" int dh_ctx_create_shared(dh_ctx parent, dh_ctx* out) {
   if (rc == DH_OK) {
     (*out)->TS = parent->TS;
     (*out)->waves = parent->waves;
+    (*out)->raster_kind = parent->raster_kind;
     (*out)->ordered_bins = parent->ordered_bins;
   }
   return rc;
@@ -342,9 +345,30 @@ extern "C" const char* dh_last_error(dh_ctx c) { return c ? c->err.c_str() : "nu
 
 extern "C" int dh_set_tiling(dh_ctx c, int32_t tile, int32_t waves) {
   if (!c) return DH_E_ARG;
-  if (!dh::raster_config_supported(tile, waves)) return fail(c, DH_E_ARG, "unsupported tile/waves");
+  if (!dh::raster_config_supported(tile, waves, 0)) return fail(c, DH_E_ARG, "unsupported tile/waves");
+  c->raster_kind = 0;
   c->TS = tile;
   c->waves = waves;
+  return DH_OK;
+}
+
+extern "C" int dh_set_raster_kernel(dh_ctx c, int32_t kind, int32_t tile, int32_t waves) {
+  if (!c) return DH_E_ARG;
+  if (!dh::raster_config_supported(tile, waves, kind)) return fail(c, DH_E_ARG, "unsupported raster kernel config");
+  c->raster_kind = kind;
+  c->TS = tile;
+  c->waves = waves;
+  return DH_OK;
+}
+
+namespace {
+void drop_graph(dh_ctx c);
+}
+
+extern "C" int dh_set_graph(dh_ctx c, int32_t enable) {
+  if (!c) return DH_E_ARG;
+  c->graph_frames = enable != 0;
+  if (!c->graph_frames) drop_graph(c);
   return DH_OK;
 }
 
@@ -564,7 +588,8 @@ int build_tables(dh_ctx c, const dh_view* views, uint32_t n_views, bool allow_tr
   dh_handle last_obj = 0;
   key->clear();
   key->push_back(n_views);
-  key->push_back((uint64_t)c->TS << 32 | (uint64_t)c->waves << 1 | (c->ordered_bins ? 1u : 0u));
+  key->push_back((uint64_t)c->TS << 32 | (uint64_t)c->raster_kind << 24 | (uint64_t)c->waves << 1 |
+                 (c->ordered_bins ? 1u : 0u));
   uint64_t n_in = 0, n_tiles = 0, n_pixels = 0;
   for (uint32_t v = 0; v < n_views; v++) {
     const dh_view& vq = views[v];
@@ -672,6 +697,7 @@ dh::PipeArgs make_args(dh_ctx c, const dh::DevMaterial* mats, uint32_t n_in, uin
   a.mats = mats;
   a.TS = c->TS;
   a.waves = c->waves;
+  a.raster_kind = c->raster_kind;
   a.cnt = c->cnt.p;
   a.blk = c->blk.p;
   a.setup = c->setup.p;
@@ -862,7 +888,7 @@ extern "C" int dh_render(dh_ctx c, const dh_frame_params* fr, const dh_handle* m
   v.canvas = canvas;
   v.meshes = meshes;
   v.n_meshes = n_meshes;
-  return render_views(c, &v, 1, false, true);
+  return render_views(c, &v, 1, c->graph_frames, true);
 }
 
 extern "C" int dh_render_batch(dh_ctx c, const dh_view* views, uint32_t n_views, uint32_t flags) {

@@ -443,50 +443,41 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict_
   if (threadIdx.x == 0) partials[blockIdx.x] = total;
 }
 
-__global__ __launch_bounds__(1024) void k_scan_spine(uint32_t* __restrict__ partials, uint32_t* total_out,
-                                                     uint32_t cap_check, uint32_t* overflow_word,
-                                                     uint32_t overflow_bit) {
-  // kScanG == 1024: one value per thread.
-  __shared__ uint32_t s_w[16];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t v = partials[threadIdx.x];
-  uint32_t x = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    uint32_t y = __shfl_up(x, o, 64);
-    if (lane >= o) x += y;
-  }
-  if (lane == 63) s_w[wave] = x;
-  __syncthreads();
-  uint32_t before = 0, all = 0;
-  for (int w = 0; w < 16; w++) {
-    if (w < wave) before += s_w[w];
-    all += s_w[w];
-  }
-  partials[threadIdx.x] = before + x - v;
-  if (threadIdx.x == 0) {
-    if (total_out) *total_out = all;
-    if (overflow_word && all > cap_check) atomicOr(overflow_word, overflow_bit);
-  }
-}
-
+// Second pass: each block takes the exclusive prefix of the partial sums
+// before it (at most kScanG values, read straight from the partials) and
+// scans its chunk; block 0 also publishes the total and the overflow flag.
 __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
                                                     const uint32_t* n_ptr, uint32_t n_const, uint32_t cap,
-                                                    const uint32_t* __restrict__ partials) {
+                                                    const uint32_t* __restrict__ partials, uint32_t* total_out,
+                                                    uint32_t cap_check, uint32_t* overflow_word,
+                                                    uint32_t overflow_bit) {
   __shared__ uint32_t s_tmp[4];
   const uint32_t n = scan_count(n_ptr, n_const, cap);
   const uint32_t C = scan_chunk(n);
   const uint32_t b0 = blockIdx.x * C;
   const uint32_t b1 = min(b0 + C, n);
-  uint32_t running = partials[blockIdx.x];
+  const uint32_t upto = blockIdx.x == 0 ? (uint32_t)kScanG : blockIdx.x;
+  uint32_t p = 0;
+  for (uint32_t i = threadIdx.x; i < upto; i += 256) p += partials[i];
+  uint32_t sum;
+  block_exclusive_scan256(p, s_tmp, &sum);
+  uint32_t running = 0;
+  if (blockIdx.x == 0) {
+    if (threadIdx.x == 0) {
+      if (total_out) *total_out = sum;
+      if (overflow_word && sum > cap_check) atomicOr(overflow_word, overflow_bit);
+    }
+  } else {
+    running = sum;
+  }
   for (uint32_t tb = b0; tb < b1; tb += 1024) {
     const uint32_t i0 = tb + threadIdx.x * 4u;
     uint32_t v[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) v[k] = (i0 + k < b1) ? in[i0 + k] : 0u;
-    const uint32_t s = v[0] + v[1] + v[2] + v[3];
+    const uint32_t s4 = v[0] + v[1] + v[2] + v[3];
     uint32_t total;
-    uint32_t ex = block_exclusive_scan256(s, s_tmp, &total) + running;
+    uint32_t ex = block_exclusive_scan256(s4, s_tmp, &total) + running;
 #pragma unroll
     for (int k = 0; k < 4; k++) {
       if (i0 + k < b1) out[i0 + k] = ex;
@@ -496,13 +487,53 @@ __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__
   }
 }
 
+// Scans of at most 4 * 1024 elements whose count the host knows: one block.
+__global__ __launch_bounds__(1024) void k_scan_small(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                     uint32_t n, uint32_t* total_out, uint32_t cap_check,
+                                                     uint32_t* overflow_word, uint32_t overflow_bit) {
+  __shared__ uint32_t s_w[16];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t i0 = threadIdx.x * 4u;
+  uint32_t v[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) v[k] = (i0 + k < n) ? in[i0 + k] : 0u;
+  const uint32_t s4 = v[0] + v[1] + v[2] + v[3];
+  uint32_t x = s4;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_w[wave] = x;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+  for (int w = 0; w < 16; w++) {
+    if (w < wave) before += s_w[w];
+    all += s_w[w];
+  }
+  uint32_t ex = before + x - s4;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    if (i0 + k < n) out[i0 + k] = ex;
+    ex += v[k];
+  }
+  if (threadIdx.x == 0) {
+    if (total_out) *total_out = all;
+    if (overflow_word && all > cap_check) atomicOr(overflow_word, overflow_bit);
+  }
+}
+
 static void scan_u32(const uint32_t* in, uint32_t* out, const uint32_t* n_ptr, uint32_t n_const, uint32_t cap,
                      uint32_t* partials, uint32_t* total_out, uint32_t cap_check, uint32_t* overflow_word,
                      uint32_t overflow_bit, hipStream_t s) {
+  if (!n_ptr && n_const <= 4096u) {
+    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, s, in, out, n_const, total_out, cap_check,
+                       overflow_word, overflow_bit);
+    return;
+  }
   hipLaunchKernelGGL(k_scan_reduce, dim3(kScanG), dim3(256), 0, s, in, n_ptr, n_const, cap, partials);
-  hipLaunchKernelGGL(k_scan_spine, dim3(1), dim3(1024), 0, s, partials, total_out, cap_check, overflow_word,
-                     overflow_bit);
-  hipLaunchKernelGGL(k_scan_apply, dim3(kScanG), dim3(256), 0, s, in, out, n_ptr, n_const, cap, partials);
+  hipLaunchKernelGGL(k_scan_apply, dim3(kScanG), dim3(256), 0, s, in, out, n_ptr, n_const, cap, partials, total_out,
+                     cap_check, overflow_word, overflow_bit);
 }
 
 // =================================================================== binning

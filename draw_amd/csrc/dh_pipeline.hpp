@@ -20,6 +20,7 @@ struct PipeArgs {
   const DevMaterial* mats;
   int TS;              // tile size in pixels
   int waves;           // waves per tile
+  int raster_kind;     // 0: waves own 8x8 quadrants; 1: triangle-parallel rectangle sweeps
   // scratch
   uint8_t* cnt;        // [n_tri_in] clipped-triangle count per input triangle
   uint32_t* blk;       // [ceil(n_tri_in/256)] block sums -> offsets
@@ -70,7 +71,7 @@ int enqueue_radix_sort(uint32_t* const keys[2], uint32_t* const vals[2], const u
                        int key_bits, uint32_t* hist, uint32_t* partials, hipStream_t s);
 void enqueue_transparent_prep(const PipeArgs& opaque, const TransArgs& t, hipStream_t s);
 void enqueue_blend(const PipeArgs& b, const TransArgs& t, hipStream_t s);
-bool raster_config_supported(int TS, int waves);
+bool raster_config_supported(int TS, int waves, int kind);
 // Per-pixel stages (dh_raster.hip): a.vals[0] must hold the sorted bin values.
 void enqueue_raster(const PipeArgs& a, hipStream_t s);
 void enqueue_shade(const PipeArgs& a, hipStream_t s);

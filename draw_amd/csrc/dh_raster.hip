@@ -311,6 +311,172 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
   if (lane == 0) atomicAdd(&status->frags, nfrag);  // nfrag is wave-uniform
 }
 
+// Triangle-parallel variant of the per-tile visibility pass.  The waves of a
+// tile take the staged triangles one at a time from an LDS counter; a
+// triangle's scan rectangle clipped to the tile is covered row-major, 64
+// pixels per visit (a 9x9 box takes 2 visits where 8x8-aligned quadrants take
+// 4 or more).  Coverage is the same exact test as k_raster: two fma steps per
+// edge when every edge value over the rectangle is an integer below 2^24,
+// covered_fast otherwise, the divided predicate for other triangles.
+// Early-Z reads the pixel's recorded key from LDS; covered fragments go to a
+// per-wave queue drained through the exact divisions into the LDS atomicMin.
+template <int TS, int NW>
+__global__ __launch_bounds__(64 * NW) void k_raster_tp(
+    const DevView* __restrict__ views, uint32_t n_views, const uint2* __restrict__ ranges,
+    const uint32_t* __restrict__ vals, const SetupRec* __restrict__ setup, const ShadeRec* __restrict__ shade,
+    const DevMaterial* __restrict__ mats, uint32_t* __restrict__ vis_out, Status* __restrict__ status) {
+  constexpr int NT = 64 * NW;
+  constexpr int CH = 128;
+  constexpr int QCAP = 256;
+  __shared__ SetupRec s_rec[CH];
+  __shared__ uint4 s_hdr[CH];
+  __shared__ float4 s_pre[CH][3];
+  __shared__ uint32_t s_id[CH];
+  __shared__ unsigned long long s_vis[TS * TS];
+  __shared__ uint32_t s_q[NW][QCAP];
+  __shared__ uint32_t s_next;
+
+  const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
+  const DevView& vw = views[find_view_tile(views, n_views, tile)];
+  const uint32_t lt = tile - vw.tile_base;
+  const int tx0 = (int)(lt % vw.ntx) * TS, ty0 = (int)(lt / vw.ntx) * TS;
+  const int W = vw.fr.W, H = vw.fr.H;
+  const float clear_depth = vw.fr.clear_depth;
+  const int tx1 = min(tx0 + TS, W) - 1, ty1 = min(ty0 + TS, H) - 1;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const uint2 rg = ranges[tile];
+  const uint64_t lanemask_lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int p = tid; p < TS * TS; p += NT) s_vis[p] = ~0ull;
+  uint32_t qn = 0, nfrag = 0;
+
+  auto submit = [&](const SetupRec& r, uint32_t id, int px, int py) {
+    const FragOut o = bary(r, (float)px, (float)py);
+    if (!(r.flags & kUvSafe) && !uv_ok(shade, mats, id, o)) atomicOr(&status->error, 1u);
+    if (o.d < clear_depth) {
+      const unsigned long long key = ((unsigned long long)depth_key(o.d) << 32) | id;
+      atomicMin(&s_vis[(py - ty0) * TS + (px - tx0)], key);
+    }
+  };
+  auto flush = [&]() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    for (uint32_t j = (uint32_t)lane; j < qn; j += 64) {
+      const uint32_t q = s_q[wave][j];
+      const uint32_t k = q >> 16, l = q & 0xFFFFu;
+      if (s_hdr[k].w > (uint32_t)(s_vis[l] >> 32)) continue;  // early-Z recheck
+      submit(s_rec[k], s_id[k], tx0 + (int)(l % TS), ty0 + (int)(l / TS));
+    }
+    nfrag += qn;
+    qn = 0;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  };
+  const volatile uint32_t* vis_hi = reinterpret_cast<const volatile uint32_t*>(s_vis);
+
+  for (uint32_t base = rg.x; base < rg.y; base += CH) {
+    const int n = (int)min((uint32_t)CH, rg.y - base);
+    __syncthreads();
+    for (int i = tid; i < n; i += NT) {
+      const uint32_t id = vals[base + i];
+      s_id[i] = id;
+      const uint4* src = reinterpret_cast<const uint4*>(setup + id);
+      uint4* dst = reinterpret_cast<uint4*>(s_rec + i);
+      uint4 w[6];
+#pragma unroll
+      for (int q = 0; q < 6; q++) w[q] = src[q];
+#pragma unroll
+      for (int q = 0; q < 6; q++) dst[q] = w[q];
+      s_hdr[i] = make_uint4(w[5].x, w[5].y, w[5].z, depth_key(__uint_as_float(w[4].w)));
+      const float fs[3] = {__uint_as_float(w[3].x), __uint_as_float(w[3].y), __uint_as_float(w[3].z)};
+#pragma unroll
+      for (int e = 0; e < 3; e++) {
+        const float sg = fs[e] > 0.0f ? 1.0f : -1.0f;
+        const float A = __uint_as_float(w[e].x), B = __uint_as_float(w[e].y);
+        const float C = __uint_as_float(w[e].z), D = __uint_as_float(w[e].w);
+        s_pre[i][e] = make_float4(sg * A, sg * B, sg * (C - D), fabsf(C) + fabsf(D));
+      }
+    }
+    if (tid == 0) s_next = 0;
+    __syncthreads();
+    for (;;) {
+      uint32_t kk = 0;
+      if (lane == 0) kk = atomicAdd(&s_next, 1u);
+      const int k = (int)__builtin_amdgcn_readfirstlane(kk);
+      if (k >= n) break;
+      const uint4 h = s_hdr[k];
+      const int cx0 = max((int)(__builtin_amdgcn_readfirstlane(h.x) & 0xFFFFu), tx0);
+      const int cx1 = min((int)(__builtin_amdgcn_readfirstlane(h.x) >> 16), tx1);
+      const int cy0 = max((int)(__builtin_amdgcn_readfirstlane(h.y) & 0xFFFFu), ty0);
+      const int cy1 = min((int)(__builtin_amdgcn_readfirstlane(h.y) >> 16), ty1);
+      if (cx0 > cx1 || cy0 > cy1) continue;
+      const uint32_t flags = __builtin_amdgcn_readfirstlane(h.z);
+      const uint32_t zkey = __builtin_amdgcn_readfirstlane(h.w);
+      const int bw = cx1 - cx0 + 1, area = bw * (cy1 - cy0 + 1);
+      const float inv_w = 1.0f / (float)bw;
+      const SetupRec& r = s_rec[k];
+      // Exact edge values over the rectangle (see k_raster): E at its corner.
+      bool exact = (flags & kFast) != 0;
+      float A[3], B[3], E[3];
+#pragma unroll
+      for (int e = 0; e < 3; e++) {
+        const float4 q = s_pre[k][e];
+        const float S = (fabsf(q.x) * (float)cx1 + fabsf(q.y) * (float)cy1) + q.w;
+        if (!(S < 16777000.0f)) exact = false;
+        A[e] = q.x;
+        B[e] = q.y;
+        E[e] = ((q.x * (float)cx0 + q.y * (float)cy0) + q.z) - ((flags & (kTieA << e)) ? 0.0f : 1.0f);
+      }
+      for (int v0 = 0; v0 < area; v0 += 64) {
+        const int idx = v0 + lane;
+        const int dy = (int)(((float)idx + 0.5f) * inv_w);
+        const int dx = idx - dy * bw;
+        const int px = cx0 + dx, py = cy0 + dy;
+        const int l = (py - ty0) * TS + (px - tx0);
+        bool cov = false;
+        if (idx < area) {
+          if (exact) {
+            const float fx = (float)dx, fy = (float)dy;
+            const float w0 = fmaf(A[0], fx, fmaf(B[0], fy, E[0]));
+            const float w1 = fmaf(A[1], fx, fmaf(B[1], fy, E[1]));
+            const float w2 = fmaf(A[2], fx, fmaf(B[2], fy, E[2]));
+            cov = fminf(fminf(w0, w1), w2) >= 0.0f;
+          } else if (flags & kFast) {
+            const float xf = (float)px, yf = (float)py;
+            cov = covered_fast(edge_eval(r.ea, xf, yf), r.f[0], flags & kTieA) &&
+                  covered_fast(edge_eval(r.eb, xf, yf), r.f[1], flags & kTieB) &&
+                  covered_fast(edge_eval(r.eg, xf, yf), r.f[2], flags & kTieG);
+          } else {
+            const FragOut o = bary(r, (float)px, (float)py);
+            cov = o.al >= 0.0f && o.be >= 0.0f && o.ga >= 0.0f && (o.al > 0.0f || (flags & kTieA)) &&
+                  (o.be > 0.0f || (flags & kTieB)) && (o.ga > 0.0f || (flags & kTieG));
+          }
+          if (cov) cov = !(zkey > vis_hi[2 * l + 1]);  // early-Z on the recorded depth key
+        }
+        if (!(flags & kFast)) {  // any other triangle: submitted directly
+          if (cov) submit(r, s_id[k], px, py);
+          nfrag += (uint32_t)__popcll(__ballot(cov));
+          continue;
+        }
+        const uint64_t m = __ballot(cov);
+        if (m) {
+          if (cov) s_q[wave][qn + (uint32_t)__popcll(m & lanemask_lt)] = ((uint32_t)k << 16) | (uint32_t)l;
+          qn += (uint32_t)__popcll(m);
+          if (qn > QCAP - 64) flush();
+        }
+      }
+    }
+    if (qn) flush();
+  }
+  __syncthreads();
+  for (int p = tid; p < TS * TS; p += NT) {
+    const int px = tx0 + p % TS, py = ty0 + p / TS;
+    if (px < W && py < H) {
+      const unsigned long long v = s_vis[p];
+      vis_out[vw.px_base + (uint32_t)py * (uint32_t)W + (uint32_t)px] = (v == ~0ull) ? kNone : (uint32_t)v;
+    }
+  }
+  if (lane == 0) atomicAdd(&status->frags, nfrag);
+}
+
 // ===================================================================== shade
 __device__ inline void fetch_rgb(const uint8_t* img, int32_t comp, uint64_t off, uint32_t c[3]) {
   if (comp == 4) {
@@ -515,7 +681,15 @@ static void launch_raster(const PipeArgs& a, hipStream_t s) {
                      a.setup, a.shade, a.mats, a.vis, a.status);
 }
 
-bool raster_config_supported(int TS, int waves) {
+template <int TS, int NW>
+static void launch_raster_tp(const PipeArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL((k_raster_tp<TS, NW>), dim3(a.n_tiles), dim3(64 * NW), 0, s, a.views, a.n_views, a.ranges,
+                     a.vals[0], a.setup, a.shade, a.mats, a.vis, a.status);
+}
+
+bool raster_config_supported(int TS, int waves, int kind) {
+  if (kind == 1) return (TS == 16 || TS == 32) && (waves == 2 || waves == 4 || waves == 8);
+  if (kind != 0) return false;
   if (TS != 8 && TS != 16 && TS != 32) return false;
   const int nq = (TS / 8) * (TS / 8);
   if (waves < 1 || nq % waves) return false;
@@ -524,6 +698,18 @@ bool raster_config_supported(int TS, int waves) {
 }
 
 void enqueue_raster(const PipeArgs& a, hipStream_t s) {
+  if (a.raster_kind == 1) {
+    if (a.TS == 16) {
+      if (a.waves == 2) launch_raster_tp<16, 2>(a, s);
+      else if (a.waves == 4) launch_raster_tp<16, 4>(a, s);
+      else launch_raster_tp<16, 8>(a, s);
+    } else {
+      if (a.waves == 2) launch_raster_tp<32, 2>(a, s);
+      else if (a.waves == 4) launch_raster_tp<32, 4>(a, s);
+      else launch_raster_tp<32, 8>(a, s);
+    }
+    return;
+  }
   const int qpw = (a.TS / 8) * (a.TS / 8) / a.waves;
   if (a.TS == 8) launch_raster<8, 1>(a, s);
   else if (a.TS == 16) {

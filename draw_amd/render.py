@@ -70,8 +70,17 @@ class Renderer:
     def set_tiling(self, tile: int, waves: int):
         self._check(self.L.dh_set_tiling(self.ctx, int(tile), int(waves)), "dh_set_tiling")
 
+    def set_raster(self, kind: int, tile: int, waves: int):
+        """Raster kernel kind (0: quadrant-owning waves, 1: triangle-parallel)
+        with its tile size and waves per tile."""
+        self._check(self.L.dh_set_raster_kernel(self.ctx, int(kind), int(tile), int(waves)), "dh_set_raster_kernel")
+
     def set_bin_order(self, ordered: bool):
         self._check(self.L.dh_set_bin_order(self.ctx, int(bool(ordered))), "dh_set_bin_order")
+
+    def set_graph(self, on: bool):
+        """dh_set_graph: replay repeated frame structures from a hipGraph."""
+        self._check(self.L.dh_set_graph(self.ctx, int(bool(on))), "dh_set_graph")
 
     def set_profiling(self, on: bool):
         self._check(self.L.dh_set_profiling(self.ctx, int(bool(on))), "dh_set_profiling")

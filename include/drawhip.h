@@ -122,8 +122,18 @@ int dh_ctx_destroy(dh_ctx ctx);
 /* Human-readable text of the last error on this context (never NULL). */
 const char* dh_last_error(dh_ctx ctx);
 /* Raster tuning: tile size 8/16/32 px and waves per tile (1..16, divides
- * (tile/8)^2).  Defaults 16 and 4. */
+ * (tile/8)^2) for raster kernel kind 0, which this selects.  Defaults 16 and 4. */
 int dh_set_tiling(dh_ctx ctx, int32_t tile, int32_t waves_per_tile);
+/* Raster kernel and its tiling together: kind 0 = waves own 8x8 quadrants of
+ * the tile (the dh_set_tiling configurations); kind 1 = triangle-parallel:
+ * the waves of a tile take triangles from a shared counter and sweep each
+ * one's rectangle (tile 16 or 32, waves 2, 4 or 8).  Output is identical. */
+int dh_set_raster_kernel(dh_ctx ctx, int32_t kind, int32_t tile, int32_t waves_per_tile);
+/* dh_render captures its launch sequence in a hipGraph after a frame and
+ * replays it while the next frames keep the same structure (meshes, canvas,
+ * sizes, tiling); uniforms may change.  1 (default) enables this, 0 launches
+ * every kernel eagerly (per-stage profiling times need eager frames). */
+int dh_set_graph(dh_ctx ctx, int32_t enable);
 /* Bin order: 1 (default) sorts each tile's triangle list stably so that it
  * keeps the submission order; 0 builds the lists with atomics, in no
  * particular order.  The frame is the same either way: each pixel keeps the

@@ -13,8 +13,8 @@ pytestmark = pytest.mark.gpu
 TILINGS = [(8, 1), (16, 1), (16, 2), (16, 4), (32, 4), (32, 16)]
 
 
-def _gpu(renderer, scene, tile=16, waves=4):
-    renderer.set_tiling(tile, waves)
+def _gpu(renderer, scene, tile=16, waves=4, kind=0):
+    renderer.set_raster(kind, tile, waves)
     try:
         canvas, winner = renderer.render(scene)
     finally:
@@ -38,13 +38,14 @@ def test_torus_power32_bit_exact(renderer):
     assert_same(f, d, w, ref, label="torus p32")
 
 
-@pytest.mark.parametrize("tile,waves", TILINGS)
-def test_small_soups_all_tilings(renderer, tile, waves):
+@pytest.mark.parametrize("kind,tile,waves", [(0, t, w) for t, w in TILINGS] + [(1, 16, 4), (1, 16, 8), (1, 32, 4),
+                                                                             (1, 32, 8), (1, 16, 2)])
+def test_small_soups_all_tilings(renderer, kind, tile, waves):
     for seed in range(3):
         s = workloads.small_random_scene(100 + seed, n_tri=400, width=97, height=61, cross_frac=0.3)
         ref = oracle(s)
-        f, d, w = _gpu(renderer, s, tile, waves)
-        assert_same(f, d, w, ref, label=f"soup seed {seed} tile {tile}x{waves}")
+        f, d, w = _gpu(renderer, s, tile, waves, kind)
+        assert_same(f, d, w, ref, label=f"soup seed {seed} kind {kind} tile {tile}x{waves}")
 
 
 def test_soup_20k_320x240_bit_exact(renderer):
@@ -80,10 +81,10 @@ def test_shells_20m_4k_sampled_bit_exact(renderer):
 def test_soup_1m_tiling_invariance(renderer):
     s = workloads.soup_scene(2)
     base = _gpu(renderer, s, 16, 4)
-    for tile, waves in [(8, 1), (32, 4)]:
-        other = _gpu(renderer, s, tile, waves)
+    for kind, tile, waves in [(0, 8, 1), (0, 32, 4), (1, 32, 8), (1, 16, 4)]:
+        other = _gpu(renderer, s, tile, waves, kind)
         for a, b in zip(base, other):
-            assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), f"tile {tile}x{waves} differs"
+            assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), f"kind {kind} tile {tile}x{waves} differs"
 
 
 def test_bin_order_modes_identical(renderer):
@@ -142,3 +143,35 @@ def test_repeat_frames_identical(renderer):
 def test_frame_uniforms_match_oracle():
     for s in [workloads.torus_scene(1), workloads.soup_scene(2, n_tri=10)]:
         assert bytes(frame_uniforms(s)) == bytes(O.frame_uniforms(s))
+
+
+def test_graph_replay_matches_eager_and_oracle(renderer):
+    """dh_render replays its captured launch sequence while the frame
+    structure repeats; new uniforms (camera, light) each frame must give the
+    eager frame's bytes, and the oracle's."""
+    import copy
+    from draw_amd.scene import Camera
+    s = workloads.soup_scene(4, n_tri=30_000, width=640, height=480, tex_size=256)
+    meshes = renderer.mesh_handles(s)
+    scenes = []
+    for k, (pos, light) in enumerate([((0.0, 0.0, 150.0), (0.0, 300.0, 300.0)),
+                                      ((20.0, -10.0, 140.0), (100.0, 200.0, 300.0)),
+                                      ((-30.0, 25.0, 120.0), (-50.0, 300.0, 100.0))]):
+        v = copy.copy(s)
+        v.camera = Camera(position=pos)
+        v.light_source = light
+        scenes.append(v)
+    renderer.set_graph(False)
+    eager = []
+    for v in scenes:
+        cv = renderer.render_device(v, frame_uniforms(v), meshes)
+        eager.append(renderer.read(cv, 640, 480))
+    renderer.set_graph(True)
+    for rep in range(2):
+        for k in [0, 1, 2, 1, 0]:
+            cv = renderer.render_device(scenes[k], frame_uniforms(scenes[k]), meshes)
+            got = renderer.read(cv, 640, 480)
+            for a, b in zip(got, eager[k]):
+                assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), f"replay {rep} view {k}"
+    ref = oracle(scenes[1])
+    assert_same(*eager[1], ref, label="moved camera")
