@@ -108,6 +108,7 @@ struct dh_ctx_s {
   hipEvent_t ev[5] = {};
   // scratch
   DevBuf<uint8_t> cnt;
+  DevBuf<uint32_t> surv;  // (input triangle, first output id) of each input triangle that emits
   DevBuf<uint32_t> blk;
   DevBuf<dh::SetupRec> setup;
   DevBuf<dh::ShadeRec> shade;
@@ -204,7 +205,7 @@ dh::DevMaterial default_material(const uint8_t* white) {
 uint64_t scratch_bytes_after(dh_ctx c, uint64_t n_in, uint64_t cap_tri, uint64_t cap_pairs, uint64_t n_tiles,
                              uint64_t n_pixels, uint64_t n_draws, uint64_t n_views) {
   auto mx = [](uint64_t have, uint64_t want) { return std::max<uint64_t>(have, std::max<uint64_t>(want, 1)); };
-  uint64_t b = mx(c->cnt.n, n_in) + 4 * mx(c->blk.n, (n_in + 255) / 256 + 1);
+  uint64_t b = mx(c->cnt.n, n_in) + 4 * mx(c->blk.n, 2 * ((n_in + 255) / 256 + 1)) + 4 * mx(c->surv.n, 2 * n_in);
   b += (sizeof(dh::SetupRec) * mx(c->setup.n, cap_tri)) + sizeof(dh::ShadeRec) * mx(c->shade.n, cap_tri);
   b += 4 * (mx(c->bincnt.n, cap_tri) + mx(c->binoff.n, cap_tri) + mx(c->big.n, cap_tri));
   b += 4 * (mx(c->keys0.n, cap_pairs) + mx(c->keys1.n, cap_pairs) + mx(c->vals0.n, cap_pairs) + mx(c->vals1.n, cap_pairs));
@@ -219,7 +220,8 @@ uint64_t scratch_bytes_after(dh_ctx c, uint64_t n_in, uint64_t cap_tri, uint64_t
 int ensure_scratch(dh_ctx c, uint32_t n_in, uint32_t cap_tri, uint32_t cap_pairs, uint32_t n_tiles,
                    uint32_t n_pixels, uint32_t n_draws, uint32_t n_views) {
   HIPCHK(c, c->cnt.ensure(n_in));
-  HIPCHK(c, c->blk.ensure((n_in + 255) / 256 + 1));
+  HIPCHK(c, c->blk.ensure(2ull * ((n_in + 255) / 256 + 1)));  // output counts, then emitting-triangle counts
+  HIPCHK(c, c->surv.ensure(2ull * n_in));
   HIPCHK(c, c->setup.ensure(cap_tri));
   HIPCHK(c, c->shade.ensure(cap_tri));
   HIPCHK(c, c->bincnt.ensure(cap_tri));
@@ -324,7 +326,7 @@ extern "C" int dh_ctx_destroy(dh_ctx c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->graph_exec) hipGraphExecDestroy(c->graph_exec);
   if (c->graph) hipGraphDestroy(c->graph);
-  c->cnt.release(); c->blk.release(); c->setup.release(); c->shade.release();
+  c->cnt.release(); c->blk.release(); c->surv.release(); c->setup.release(); c->shade.release();
   c->bincnt.release(); c->binoff.release(); c->big.release();
   c->keys0.release(); c->keys1.release(); c->vals0.release(); c->vals1.release();
   c->hist.release(); c->partials.release(); c->ranges.release(); c->vis.release();
@@ -700,6 +702,7 @@ dh::PipeArgs make_args(dh_ctx c, const dh::DevMaterial* mats, uint32_t n_in, uin
   a.raster_kind = c->raster_kind;
   a.cnt = c->cnt.p;
   a.blk = c->blk.p;
+  a.surv = reinterpret_cast<uint2*>(c->surv.p);
   a.setup = c->setup.p;
   a.shade = c->shade.p;
   a.cap_tri = c->cap_tri;
@@ -786,7 +789,7 @@ int render_views(dh_ctx c, const dh_view* views, uint32_t n_views, bool use_grap
     }
     // everything a captured launch sequence bakes in
     std::vector<uint64_t> gkey = key;
-    const void* ptrs[] = {c->cnt.p, c->blk.p, c->setup.p, c->shade.p, c->bincnt.p, c->binoff.p, c->big.p,
+    const void* ptrs[] = {c->cnt.p, c->blk.p, c->surv.p, c->setup.p, c->shade.p, c->bincnt.p, c->binoff.p, c->big.p,
                           c->keys0.p, c->keys1.p, c->vals0.p, c->vals1.p, c->hist.p, c->partials.p, c->ranges.p,
                           c->tile_cnt.p, c->tile_start.p, c->vis.p, c->status.p, c->draws.p, c->views.p};
     for (const void* p : ptrs) gkey.push_back((uint64_t)(uintptr_t)p);
@@ -1026,7 +1029,7 @@ namespace dh {
 
 uint64_t ctx_scratch_bytes(dh_ctx c) {
   uint64_t b = 0;
-  b += c->cnt.n + 4 * c->blk.n + sizeof(SetupRec) * c->setup.n + sizeof(ShadeRec) * c->shade.n;
+  b += c->cnt.n + 4 * (c->blk.n + c->surv.n) + sizeof(SetupRec) * c->setup.n + sizeof(ShadeRec) * c->shade.n;
   b += 4 * (c->bincnt.n + c->binoff.n + c->big.n + c->keys0.n + c->keys1.n + c->vals0.n + c->vals1.n);
   b += 4 * (c->hist.n + c->partials.n + c->tile_cnt.n + c->tile_start.n + c->vis.n) + 8 * c->ranges.n;
   b += sizeof(Status) * c->status.n + sizeof(DevDraw) * c->draws.n + sizeof(DevView) * c->views.n;
@@ -1038,7 +1041,8 @@ uint64_t ctx_scratch_bytes(dh_ctx c) {
 uint64_t scratch_estimate(uint64_t n_in, uint64_t n_tiles, uint64_t n_pixels, uint64_t n_draws,
                           uint64_t n_views) {
   const uint64_t tri = n_in + n_in / 4 + 1024, pairs = 4 * tri + n_tiles;
-  return std::max<uint64_t>(n_in, 1) + 4 * ((n_in + 255) / 256 + 1) + (sizeof(SetupRec) + sizeof(ShadeRec) + 12) * tri + 16 * pairs +
+  return std::max<uint64_t>(n_in, 1) + 8 * ((n_in + 255) / 256 + 1) + 8 * n_in +
+         (sizeof(SetupRec) + sizeof(ShadeRec) + 12) * tri + 16 * pairs +
          4ull * kRadixHistEntries + 4096 + 20 * n_tiles + 4 * n_pixels + sizeof(DevDraw) * n_draws +
          sizeof(DevView) * n_views + 2 * sizeof(Status);
 }
@@ -1065,7 +1069,7 @@ void ctx_release_scratch(dh_ctx c) {
   hipSetDevice(c->device);
   hipStreamSynchronize(c->stream);
   drop_graph(c);
-  c->cnt.release(); c->blk.release(); c->setup.release(); c->shade.release();
+  c->cnt.release(); c->blk.release(); c->surv.release(); c->setup.release(); c->shade.release();
   c->bincnt.release(); c->binoff.release(); c->big.release();
   c->keys0.release(); c->keys1.release(); c->vals0.release(); c->vals1.release();
   c->hist.release(); c->partials.release(); c->ranges.release(); c->vis.release();

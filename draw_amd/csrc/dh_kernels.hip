@@ -7,8 +7,10 @@
 //   3 k_geom_emit    per input triangle: per-vertex lighting (917-926), clip
 //                    with attributes, projection (1047-1063), raster setup
 //                    (canvas.rs:585-666) -> SetupRec + ShadeRec at its
-//                    compacted index
-//   4 k_bin_count    per clipped triangle: conservative tile overlap count
+//                    compacted index, and (ordered bins) the triangle's
+//                    conservative tile overlap count
+//   4 k_bin_count_wide  the counts of triangles spanning many tiles, a wave
+//                    each (k_bin_count counts per tile for unordered bins)
 //   5 scan           -> bin entry offsets
 //   6 k_bin_emit     (tile, triangle) pairs in triangle order
 //   7 radix sort     stable LSD sort of the pairs by tile -> per-tile lists
@@ -212,7 +214,7 @@ __device__ inline void rect_axis(uint64_t a0, uint64_t a1, uint64_t smax, uint32
 // SetupRec and ShadeRec of one clipped triangle with 16-byte stores.
 __device__ inline void make_setup(const FrameParamsPod& fr, const FV& A, const FV& B, const FV& C, uint32_t material,
                                   const DevMaterial& mat, uint32_t view, uint32_t group, SetupRec* __restrict__ rout,
-                                  ShadeRec* __restrict__ sout) {
+                                  ShadeRec* __restrict__ sout, SetupRec& rec) {
   const float* M = fr.M;
   float X[3], Y[3];
   const V3 P[3] = {A.p, B.p, C.p};
@@ -306,14 +308,22 @@ __device__ inline void make_setup(const FrameParamsPod& fr, const FV& A, const F
       zlb = dmin * (1.0f - 9.5367431640625e-07f);
     if ((flags & kUvSafe) && zlb >= fr.clear_depth) flags |= kDrop;  // no pixel can pass `<` the clear depth
   }
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    rec.ea[k] = ea[k];
+    rec.eb[k] = eb[k];
+    rec.eg[k] = eg[k];
+  }
+  rec.f[0] = fa; rec.f[1] = fb; rec.f[2] = fg; rec.f[3] = 0.0f;
+  rec.d[0] = A.d; rec.d[1] = B.d; rec.d[2] = C.d; rec.d[3] = zlb;
+  rec.rx = x0 | (x1 << 16);
+  rec.ry = y0 | (y1 << 16);
+  rec.flags = flags;
+  rec.view = view;
   float4* ro = reinterpret_cast<float4*>(rout);
-  ro[0] = make_float4(ea[0], ea[1], ea[2], ea[3]);
-  ro[1] = make_float4(eb[0], eb[1], eb[2], eb[3]);
-  ro[2] = make_float4(eg[0], eg[1], eg[2], eg[3]);
-  ro[3] = make_float4(fa, fb, fg, 0.0f);
-  ro[4] = make_float4(A.d, B.d, C.d, zlb);
-  ro[5] = make_float4(__uint_as_float(x0 | (x1 << 16)), __uint_as_float(y0 | (y1 << 16)), __uint_as_float(flags),
-                      __uint_as_float(view));
+  const float4* ri = reinterpret_cast<const float4*>(&rec);
+#pragma unroll
+  for (int k = 0; k < 6; k++) ro[k] = ri[k];
   // shading attributes: uv[6], material, pad, n[9], l[9], h[9], pad
   float4* so = reinterpret_cast<float4*>(sout);
   so[0] = make_float4(A.u, A.v, B.u, B.v);
@@ -327,10 +337,173 @@ __device__ inline void make_setup(const FrameParamsPod& fr, const FV& A, const F
   so[8] = make_float4(C.h.x, C.h.y, C.h.z, 0.0f);
 }
 
+// =================================================================== binning
+// Conservative tile sets of a clipped triangle.  For a FAST triangle every
+// edge value at an integer pixel is e = fl(fl(fl(fl(Ax)+fl(By))+C)-D) with
+// |e - (Ax+By+C-D)| <= 4.01 u S, S = |A|x + |B|y + |C| + |D|; a pixel can only
+// be covered when sign(f) * e >= 0 on all three edges.  Each edge's
+// exact-arithmetic function, widened by that bound (plus 1), therefore gives
+// a superset of the tiles holding covered pixels: per tile for triangles of a
+// few tiles (tile_may_cover), per tile row for wide ones (spans_row).  Other
+// triangles, and those within one tile, are binned to every tile of their
+// scan rectangle.
+struct Spans {
+  int cx0, cx1, cy0, cy1;
+  bool exact;
+  double a[3], b[3], k[3], m[3];
+};
+
+__device__ inline void spans_init(const SetupRec& r, int TS, Spans& g) {
+  const uint32_t x0 = r.rx & 0xFFFFu, x1 = r.rx >> 16, y0 = r.ry & 0xFFFFu, y1 = r.ry >> 16;
+  g.cx0 = (int)(x0 / TS); g.cx1 = (int)(x1 / TS);
+  g.cy0 = (int)(y0 / TS); g.cy1 = (int)(y1 / TS);
+  g.exact = (r.flags & kFast) != 0;
+  if (g.cx0 == g.cx1 && g.cy0 == g.cy1) {  // one tile: binned there without edge tests
+    g.exact = false;
+    return;
+  }
+  const float* es[3] = {r.ea, r.eb, r.eg};
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const double A = es[i][0], B = es[i][1], C = es[i][2], D = es[i][3];
+    const double s = r.f[i] > 0.0f ? 1.0 : -1.0;
+    const double S = fabs(A) * (double)x1 + fabs(B) * (double)y1 + fabs(C) + fabs(D);
+    g.a[i] = s * A;
+    g.b[i] = s * B;
+    g.k[i] = s * (C - D);
+    g.m[i] = 4.02 * 5.9604644775390625e-08 * S + 1.0 + 1e-9 * S;
+    if (!isfinite(S)) g.exact = false;
+  }
+}
+
+// Columns [lo, hi] of tile row ty that may hold covered pixels (lo > hi: none).
+__device__ inline void spans_row(const Spans& g, int TS, int ty, int* lo, int* hi) {
+  int l = g.cx0, h = g.cx1;
+  if (g.exact) {
+    const double ya = (double)(ty * TS), yb = (double)(ty * TS + TS - 1);
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      const double K = g.k[i] + fmax(g.b[i] * ya, g.b[i] * yb);
+      const double a = g.a[i];
+      if (a > 0.0) {
+        // need a*(c*TS + TS-1) + K >= -m
+        double xl = (-g.m[i] - K) / a - 1.0;
+        double c = (xl - (double)(TS - 1)) / (double)TS;
+        c = fmin(fmax(c, -1.0), 1.0e6);
+        const int ci = (int)ceil(c);
+        l = max(l, ci);
+      } else if (a < 0.0) {
+        // need a*(c*TS) + K >= -m
+        double xr = (K + g.m[i]) / (-a) + 1.0;
+        double c = xr / (double)TS;
+        c = fmin(fmax(c, -2.0), 1.0e6);
+        const int ci = (int)floor(c);
+        h = min(h, ci);
+      } else if (K < -g.m[i]) {
+        h = l - 1;
+      }
+    }
+  }
+  *lo = l;
+  *hi = h;
+}
+
+// Work split: a triangle whose tile range is more than kWideSpan tiles tall
+// or wide is binned by a whole wave (lanes over rows, then over entries); the
+// rest by one thread each (at most kWideSpan^2 entries).  The split only
+// balances work.
+constexpr int kWideSpan = 4;
+
+__device__ inline bool bin_is_wide(const Spans& g) {
+  return g.cy1 - g.cy0 + 1 > kWideSpan || g.cx1 - g.cx0 + 1 > kWideSpan;
+}
+
+__device__ inline uint64_t lanemask_lt_u64(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
+
+// Tile (tx, ty) may hold a covered pixel: every edge's exact function,
+// widened by m, reaches 0 at the tile corner maximising it.  Products of a
+// float and a pixel coordinate below 2^16 are exact in double.
+__device__ inline bool tile_may_cover(const Spans& g, int TS, int tx, int ty) {
+  if (!g.exact) return true;
+  const double xa = (double)(tx * TS), xb = (double)(tx * TS + TS - 1);
+  const double ya = (double)(ty * TS), yb = (double)(ty * TS + TS - 1);
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const double v = (g.a[i] * (g.a[i] > 0.0 ? xb : xa) + g.b[i] * (g.b[i] > 0.0 ? yb : ya)) + g.k[i];
+    if (v < -g.m[i]) return false;
+  }
+  return true;
+}
+
+// Calls f(e, ty, tx) for every tile of a triangle that is not wide (at most
+// kWideSpan x kWideSpan tiles in its rectangle), e numbering the tiles in
+// row-major order; returns the tile count.
+template <class F>
+__device__ inline uint32_t thread_tiles(const Spans& g, int TS, F f) {
+  uint32_t e = 0;
+  for (int ty = g.cy0; ty <= g.cy1; ty++)
+    for (int tx = g.cx0; tx <= g.cx1; tx++)
+      if (tile_may_cover(g, TS, tx, ty)) f(e++, ty, tx);
+  return e;
+}
+
+// Wave version: lanes take tile rows, a wave scan places each row's span and
+// the entries are spread over the lanes.  s_lo/s_off: 64-entry LDS scratch of
+// this wave.  Returns the tile count (wave-uniform).
+template <class F>
+__device__ inline uint32_t wave_tiles(const Spans& g, int TS, int lane, int* s_lo, uint32_t* s_off, F f) {
+  uint32_t o = 0;
+  for (int row0 = g.cy0; row0 <= g.cy1; row0 += 64) {
+    const int ty = row0 + lane;
+    int lo = 0, hi = -1;
+    if (ty <= g.cy1) spans_row(g, TS, ty, &lo, &hi);
+    const uint32_t c = hi >= lo ? (uint32_t)(hi - lo + 1) : 0u;
+    uint32_t x = c;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = (uint32_t)__shfl_up((int)x, d, 64);
+      if (lane >= d) x += y;
+    }
+    const uint32_t total = (uint32_t)__shfl((int)x, 63, 64);
+    s_lo[lane] = lo;
+    s_off[lane] = x - c;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    for (uint32_t e = (uint32_t)lane; e < total; e += 64) {
+      // the row of entry e: the last row whose offset is <= e (rows without
+      // entries share their successor's offset, so the last one has entries)
+      int k = 0;
+#pragma unroll
+      for (int step = 32; step >= 1; step >>= 1)
+        if (s_off[k + step] <= e) k += step;
+      f(o + e, row0 + k, s_lo[k] + (int)(e - s_off[k]));
+    }
+    o += total;
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  }
+  return o;
+}
+
+// Ordered bins, one clipped triangle: its wide flag and, when it is not wide,
+// its tile count (the wide ones are counted a wave at a time later).
+__device__ inline void bin_count_one(const SetupRec& r, int TS, uint32_t i, uint32_t* __restrict__ bincnt,
+                                     uint8_t* __restrict__ wflag) {
+  uint32_t c = 0;
+  bool is_wide = false;
+  if (!(r.flags & kDrop)) {
+    Spans g;
+    spans_init(r, TS, g);
+    is_wide = bin_is_wide(g);
+    if (!is_wide) c = thread_tiles(g, TS, [](uint32_t, int, int) {});
+  }
+  wflag[i] = is_wide ? 1u : 0u;
+  if (!is_wide) bincnt[i] = c;
+}
+
 // ================================================================== geometry
 __global__ __launch_bounds__(256) void k_geom_count(const DevDraw* __restrict__ draws, uint32_t n_draws,
                                                     const DevView* __restrict__ views, uint32_t n_in,
-                                                    uint8_t* __restrict__ cnt, uint32_t* __restrict__ blk) {
+                                                    uint8_t* __restrict__ cnt, uint32_t* __restrict__ blk,
+                                                    uint32_t* __restrict__ blk_emit) {
   __shared__ uint32_t s_tmp[4];
   const uint32_t t = blockIdx.x * 256u + threadIdx.x;
   uint32_t c = 0;
@@ -345,25 +518,45 @@ __global__ __launch_bounds__(256) void k_geom_count(const DevDraw* __restrict__ 
     c = (uint32_t)cull_and_clip(fr, a, b, cc, !(dr.flags & kDrawNoCull), [](int, const PV&, const PV&, const PV&) {});
     cnt[t] = (uint8_t)c;
   }
+  // packed: clipped triangles << 16 | emitting triangles (a block has at
+  // most 256 * 4 of the former -- near and far clipping emit at most 4 per
+  // input triangle -- and 256 of the latter)
   uint32_t total;
-  block_exclusive_scan256(c, s_tmp, &total);
-  if (threadIdx.x == 0) blk[blockIdx.x] = total;
+  block_exclusive_scan256((c << 16) | (c ? 1u : 0u), s_tmp, &total);
+  if (threadIdx.x == 0) {
+    blk[blockIdx.x] = total >> 16;
+    blk_emit[blockIdx.x] = total & 0xFFFFu;
+  }
 }
 
-__global__ __launch_bounds__(256) void k_geom_emit(const DevDraw* __restrict__ draws, uint32_t n_draws,
-                                                   const DevView* __restrict__ views, uint32_t n_in,
-                                                   const uint8_t* __restrict__ cnt,
-                                                   const uint32_t* __restrict__ blk_off,
-                                                   const DevMaterial* __restrict__ mats, SetupRec* __restrict__ setup,
-                                                   ShadeRec* __restrict__ shade, uint32_t cap_tri,
-                                                   Status* __restrict__ status) {
+// Ordered list of the input triangles that emit: (triangle, first output id).
+__global__ __launch_bounds__(256) void k_geom_compact(uint32_t n_in, const uint8_t* __restrict__ cnt,
+                                                      const uint32_t* __restrict__ blk_off,
+                                                      const uint32_t* __restrict__ emit_off,
+                                                      uint2* __restrict__ surv) {
   __shared__ uint32_t s_tmp[4];
   const uint32_t t = blockIdx.x * 256u + threadIdx.x;
   const uint32_t c = t < n_in ? (uint32_t)cnt[t] : 0u;
   uint32_t total;
-  const uint32_t local = block_exclusive_scan256(c, s_tmp, &total);
-  if (c == 0) return;
-  const uint32_t base = blk_off[blockIdx.x] + local;
+  const uint32_t local = block_exclusive_scan256((c << 16) | (c ? 1u : 0u), s_tmp, &total);
+  if (c) surv[emit_off[blockIdx.x] + (local & 0xFFFFu)] = make_uint2(t, blk_off[blockIdx.x] + (local >> 16));
+}
+
+
+
+// One thread per emitting input triangle (dense: culled triangles take no lane).
+template <bool BIN>
+__global__ __launch_bounds__(256) void k_geom_emit(const DevDraw* __restrict__ draws, uint32_t n_draws,
+                                                   const DevView* __restrict__ views,
+                                                   const uint2* __restrict__ surv,
+                                                   const DevMaterial* __restrict__ mats, SetupRec* __restrict__ setup,
+                                                   ShadeRec* __restrict__ shade, uint32_t cap_tri,
+                                                   Status* __restrict__ status, int TS, uint32_t* __restrict__ bincnt,
+                                                   uint8_t* __restrict__ wflag) {
+  const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+  if (j >= status->n_emit) return;
+  const uint2 sv = surv[j];
+  const uint32_t t = sv.x, base = sv.y;
   const DevDraw& dr = draws[find_draw(draws, n_draws, t)];
   const FrameParamsPod& fr = views[dr.view].fr;
   const uint32_t* ti = dr.tri + 9ull * (t - dr.tri_begin);
@@ -393,7 +586,9 @@ __global__ __launch_bounds__(256) void k_geom_emit(const DevDraw* __restrict__ d
       atomicOr(&status->overflow, 1u);
       return;
     }
-    make_setup(fr, A, B, C, material, mats[material], view, group, setup + idx, shade + idx);
+    SetupRec rec;
+    make_setup(fr, A, B, C, material, mats[material], view, group, setup + idx, shade + idx, rec);
+    if (BIN) bin_count_one(rec, TS, idx, bincnt, wflag);
   });
 }
 
@@ -536,162 +731,28 @@ static void scan_u32(const uint32_t* in, uint32_t* out, const uint32_t* n_ptr, u
                      cap_check, overflow_word, overflow_bit);
 }
 
-// =================================================================== binning
-// Conservative tile-row spans of a clipped triangle.  For a FAST triangle
-// every edge value at an integer pixel is e = fl(fl(fl(fl(Ax)+fl(By))+C)-D)
-// with |e - (Ax+By+C-D)| <= 4.01 u S, S = |A|x + |B|y + |C| + |D|; a pixel can
-// only be covered when sign(f) * e >= 0 on all three edges.  Each edge's
-// exact-arithmetic function, widened by that bound (plus 1), therefore gives
-// a superset of the tiles holding covered pixels.  Other triangles are binned
-// to every tile of their scan rectangle.
-struct Spans {
-  int cx0, cx1, cy0, cy1;
-  bool exact;
-  double a[3], b[3], k[3], m[3];
-};
-
-__device__ inline void spans_init(const SetupRec& r, int TS, Spans& g) {
-  const uint32_t x0 = r.rx & 0xFFFFu, x1 = r.rx >> 16, y0 = r.ry & 0xFFFFu, y1 = r.ry >> 16;
-  g.cx0 = (int)(x0 / TS); g.cx1 = (int)(x1 / TS);
-  g.cy0 = (int)(y0 / TS); g.cy1 = (int)(y1 / TS);
-  g.exact = (r.flags & kFast) != 0;
-  const float* es[3] = {r.ea, r.eb, r.eg};
-#pragma unroll
-  for (int i = 0; i < 3; i++) {
-    const double A = es[i][0], B = es[i][1], C = es[i][2], D = es[i][3];
-    const double s = r.f[i] > 0.0f ? 1.0 : -1.0;
-    const double S = fabs(A) * (double)x1 + fabs(B) * (double)y1 + fabs(C) + fabs(D);
-    g.a[i] = s * A;
-    g.b[i] = s * B;
-    g.k[i] = s * (C - D);
-    g.m[i] = 4.02 * 5.9604644775390625e-08 * S + 1.0 + 1e-9 * S;
-    if (!isfinite(S)) g.exact = false;
-  }
-}
-
-// Columns [lo, hi] of tile row ty that may hold covered pixels (lo > hi: none).
-__device__ inline void spans_row(const Spans& g, int TS, int ty, int* lo, int* hi) {
-  int l = g.cx0, h = g.cx1;
-  if (g.exact) {
-    const double ya = (double)(ty * TS), yb = (double)(ty * TS + TS - 1);
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-      const double K = g.k[i] + fmax(g.b[i] * ya, g.b[i] * yb);
-      const double a = g.a[i];
-      if (a > 0.0) {
-        // need a*(c*TS + TS-1) + K >= -m
-        double xl = (-g.m[i] - K) / a - 1.0;
-        double c = (xl - (double)(TS - 1)) / (double)TS;
-        c = fmin(fmax(c, -1.0), 1.0e6);
-        const int ci = (int)ceil(c);
-        l = max(l, ci);
-      } else if (a < 0.0) {
-        // need a*(c*TS) + K >= -m
-        double xr = (K + g.m[i]) / (-a) + 1.0;
-        double c = xr / (double)TS;
-        c = fmin(fmax(c, -2.0), 1.0e6);
-        const int ci = (int)floor(c);
-        h = min(h, ci);
-      } else if (K < -g.m[i]) {
-        h = l - 1;
-      }
-    }
-  }
-  *lo = l;
-  *hi = h;
-}
-
-// Work split: a triangle whose tile range is more than kWideSpan tiles tall
-// or wide is binned by a whole wave (lanes over rows, then over entries); the
-// rest by one thread each (at most kWideSpan^2 entries).  The split only
-// balances work.
-constexpr int kWideSpan = 4;
-
-__device__ inline bool bin_is_wide(const Spans& g) {
-  return g.cy1 - g.cy0 + 1 > kWideSpan || g.cx1 - g.cx0 + 1 > kWideSpan;
-}
-
-__device__ inline uint64_t lanemask_lt_u64(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
-
-// Calls f(e, ty, tx) for every tile of the triangle, e numbering the tiles in
-// row-major order; returns the tile count.
-template <class F>
-__device__ inline uint32_t thread_tiles(const Spans& g, int TS, F f) {
-  uint32_t e = 0;
-  for (int ty = g.cy0; ty <= g.cy1; ty++) {
-    int lo, hi;
-    spans_row(g, TS, ty, &lo, &hi);
-    for (int tx = lo; tx <= hi; tx++, e++) f(e, ty, tx);
-  }
-  return e;
-}
-
-// Wave version: lanes take tile rows, a wave scan places each row's span and
-// the entries are spread over the lanes.  s_lo/s_off: 64-entry LDS scratch of
-// this wave.  Returns the tile count (wave-uniform).
-template <class F>
-__device__ inline uint32_t wave_tiles(const Spans& g, int TS, int lane, int* s_lo, uint32_t* s_off, F f) {
-  uint32_t o = 0;
-  for (int row0 = g.cy0; row0 <= g.cy1; row0 += 64) {
-    const int ty = row0 + lane;
-    int lo = 0, hi = -1;
-    if (ty <= g.cy1) spans_row(g, TS, ty, &lo, &hi);
-    const uint32_t c = hi >= lo ? (uint32_t)(hi - lo + 1) : 0u;
-    uint32_t x = c;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t y = (uint32_t)__shfl_up((int)x, d, 64);
-      if (lane >= d) x += y;
-    }
-    const uint32_t total = (uint32_t)__shfl((int)x, 63, 64);
-    s_lo[lane] = lo;
-    s_off[lane] = x - c;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    for (uint32_t e = (uint32_t)lane; e < total; e += 64) {
-      // the row of entry e: the last row whose offset is <= e (rows without
-      // entries share their successor's offset, so the last one has entries)
-      int k = 0;
-#pragma unroll
-      for (int step = 32; step >= 1; step >>= 1)
-        if (s_off[k + step] <= e) k += step;
-      f(o + e, row0 + k, s_lo[k] + (int)(e - s_off[k]));
-    }
-    o += total;
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  }
-  return o;
-}
-
-// Pass 1, one thread per clipped triangle: flag wide triangles (wflag, one
-// byte each) and count the tiles of the others -- per tile with atomics
-// (unordered bins) or per triangle (ordered bins).
-template <bool ORDERED>
+// Pass 1 of unordered bins, one thread per clipped triangle: flag wide
+// triangles (wflag, one byte each) and count the tiles of the others per tile
+// with atomics.  (Ordered bins take the per-triangle counts from k_geom_emit.)
 __global__ __launch_bounds__(256) void k_bin_count(const SetupRec* __restrict__ setup, const DevView* __restrict__ views,
                                                    const Status* status, uint32_t cap_tri, int TS,
-                                                   uint32_t* __restrict__ bincnt, uint32_t* __restrict__ tile_cnt,
-                                                   uint8_t* __restrict__ wflag) {
+                                                   uint32_t* __restrict__ tile_cnt, uint8_t* __restrict__ wflag) {
   const uint32_t n = min(status->n_out, cap_tri);
   for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
     const SetupRec r = setup[i];
-    uint32_t c = 0;
     bool is_wide = false;
     if (!(r.flags & kDrop)) {
       Spans g;
       spans_init(r, TS, g);
       is_wide = bin_is_wide(g);
       if (!is_wide) {
-        if (ORDERED) {
-          c = thread_tiles(g, TS, [](uint32_t, int, int) {});
-        } else {
-          const DevView& vw = views[r.view];
-          thread_tiles(g, TS, [&](uint32_t, int ty, int tx) {
-            atomicAdd(&tile_cnt[vw.tile_base + (uint32_t)ty * vw.ntx + (uint32_t)tx], 1u);
-          });
-        }
+        const DevView& vw = views[r.view];
+        thread_tiles(g, TS, [&](uint32_t, int ty, int tx) {
+          atomicAdd(&tile_cnt[vw.tile_base + (uint32_t)ty * vw.ntx + (uint32_t)tx], 1u);
+        });
       }
     }
     wflag[i] = is_wide ? 1u : 0u;
-    if (ORDERED && !is_wide) bincnt[i] = c;
   }
 }
 
@@ -808,9 +869,13 @@ __global__ __launch_bounds__(256) void k_ranges_from_counts(const uint32_t* __re
 // the scan over (digit, wave) turns the counts into output bases, and
 // k_radix_scatter ranks the wave's items in input order (ballot match) and
 // advances its own LDS bases -- no block barrier inside the scatter.  Equal
-// keys keep the triangle order.
+// keys keep the triangle order.  The chunks of one XCD are contiguous, so a
+// tile's output run is written by few XCDs: with ~1 item per (wave, tile),
+// interleaved chunks would make every XCD's L2 write back partial lines of
+// every run.
 constexpr int kRadixG = 256;               // blocks of 4 waves
 constexpr int kRadixW = kRadixG * 4;       // waves
+constexpr int kRadixU = 8;                 // 64-item groups whose loads a wave issues together
 static_assert(2048u * kRadixW == kRadixHistEntries, "histogram size");
 
 __device__ inline uint32_t radix_chunk(uint32_t n) { return (((n + kRadixW - 1u) / kRadixW) + 63u) & ~63u; }
@@ -823,17 +888,29 @@ __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* __restrict__
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t n = min(*n_ptr, cap);
   const uint32_t C = radix_chunk(n);
-  const uint32_t gw = blockIdx.x * 4u + (uint32_t)wave;
+  const uint32_t lb = xcd_remap(blockIdx.x, gridDim.x);
+  const uint32_t gw = lb * 4u + (uint32_t)wave;
   const uint32_t b0 = min(gw * C, n), b1 = min(b0 + C, n);
   for (uint32_t d = (uint32_t)tid; d < ND; d += 256)
 #pragma unroll
     for (int w = 0; w < 4; w++) h[w][d] = 0;
   __syncthreads();
-  for (uint32_t i = b0 + (uint32_t)lane; i < b1; i += 64) atomicAdd(&h[wave][(keys[i] >> shift) & (ND - 1u)], 1u);
+  // kRadixU loads in flight per lane before their LDS increments
+  for (uint32_t tb = b0; tb < b1; tb += 64u * kRadixU) {
+    uint32_t k[kRadixU];
+#pragma unroll
+    for (int u = 0; u < kRadixU; u++) {
+      const uint32_t i = tb + 64u * (uint32_t)u + (uint32_t)lane;
+      k[u] = i < b1 ? keys[i] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < kRadixU; u++)
+      if (tb + 64u * (uint32_t)u + (uint32_t)lane < b1) atomicAdd(&h[wave][(k[u] >> shift) & (ND - 1u)], 1u);
+  }
   __syncthreads();
   for (uint32_t d = (uint32_t)tid; d < ND; d += 256)
 #pragma unroll
-    for (int w = 0; w < 4; w++) hist[d * kRadixW + blockIdx.x * 4u + (uint32_t)w] = h[w][d];
+    for (int w = 0; w < 4; w++) hist[d * kRadixW + lb * 4u + (uint32_t)w] = h[w][d];
 }
 
 template <int BITS>
@@ -846,34 +923,43 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t n = min(*n_ptr, cap);
   const uint32_t C = radix_chunk(n);
-  const uint32_t gw = blockIdx.x * 4u + (uint32_t)wave;
+  const uint32_t gw = xcd_remap(blockIdx.x, gridDim.x) * 4u + (uint32_t)wave;
   const uint32_t b0 = min(gw * C, n), b1 = min(b0 + C, n);
   for (uint32_t d = (uint32_t)lane; d < ND; d += 64) run[wave][d] = hist_off[d * kRadixW + gw];
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  for (uint32_t tb = b0; tb < b1; tb += 64) {
-    const uint32_t i = tb + (uint32_t)lane;
-    const bool valid = i < b1;
-    const uint32_t key = valid ? kin[i] : 0u;
-    const uint32_t val = valid ? vin[i] : 0u;
-    const uint32_t dgt = (key >> shift) & (ND - 1u);
-    uint64_t peers = __ballot(valid);
+  for (uint32_t tb = b0; tb < b1; tb += 64u * kRadixU) {
+    // kRadixU groups of 64 items loaded up front, then ranked in order
+    uint32_t kk[kRadixU], vv[kRadixU];
 #pragma unroll
-    for (int bit = 0; bit < BITS; bit++) {
-      const bool on = (dgt >> bit) & 1u;
-      const uint64_t m = __ballot(on);
-      peers &= on ? m : ~m;
+    for (int u = 0; u < kRadixU; u++) {
+      const uint32_t i = tb + 64u * (uint32_t)u + (uint32_t)lane;
+      kk[u] = i < b1 ? kin[i] : 0u;
+      vv[u] = i < b1 ? vin[i] : 0u;
     }
-    const uint32_t rank = (uint32_t)__popcll(peers & lt);
-    uint32_t base = 0;
-    if (valid) base = run[wave][dgt];
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    if (valid) {
-      kout[base + rank] = key;
-      vout[base + rank] = val;
-      if (rank == 0) run[wave][dgt] = base + (uint32_t)__popcll(peers);
+#pragma unroll
+    for (int u = 0; u < kRadixU; u++) {
+      const bool valid = tb + 64u * (uint32_t)u + (uint32_t)lane < b1;
+      const uint32_t key = kk[u];
+      const uint32_t dgt = (key >> shift) & (ND - 1u);
+      uint64_t peers = __ballot(valid);
+#pragma unroll
+      for (int bit = 0; bit < BITS; bit++) {
+        const bool on = (dgt >> bit) & 1u;
+        const uint64_t m = __ballot(on);
+        peers &= on ? m : ~m;
+      }
+      const uint32_t rank = (uint32_t)__popcll(peers & lt);
+      uint32_t base = 0;
+      if (valid) base = run[wave][dgt];
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      if (valid) {
+        kout[base + rank] = key;
+        vout[base + rank] = vv[u];
+        if (rank == 0) run[wave][dgt] = base + (uint32_t)__popcll(peers);
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   }
 }
 
@@ -981,12 +1067,21 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
   const uint32_t n_blk = cdiv(a.n_tri_in, 256);
   // 1-3 geometry with ordered compaction
   if (n_blk) {
+    uint32_t* blk_emit = a.blk + n_blk + 1;
     hipLaunchKernelGGL(k_geom_count, dim3(n_blk), dim3(256), 0, s, a.draws, a.n_draws, a.views, a.n_tri_in, a.cnt,
-                       a.blk);
+                       a.blk, blk_emit);
     scan_u32(a.blk, a.blk, nullptr, n_blk, n_blk, a.partials, &a.status->n_out, a.cap_tri, &a.status->overflow, 1u,
              s);
-    hipLaunchKernelGGL(k_geom_emit, dim3(n_blk), dim3(256), 0, s, a.draws, a.n_draws, a.views, a.n_tri_in, a.cnt,
-                       a.blk, a.mats, a.setup, a.shade, a.cap_tri, a.status);
+    scan_u32(blk_emit, blk_emit, nullptr, n_blk, n_blk, a.partials, &a.status->n_emit, 0u, nullptr, 0u, s);
+    hipLaunchKernelGGL(k_geom_compact, dim3(n_blk), dim3(256), 0, s, a.n_tri_in, a.cnt, a.blk, blk_emit, a.surv);
+    // ordered bins: the per-triangle tile counts and wide flags come with the records
+    if (a.ordered_bins)
+      hipLaunchKernelGGL(k_geom_emit<true>, dim3(n_blk), dim3(256), 0, s, a.draws, a.n_draws, a.views, a.surv,
+                         a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, a.bincnt,
+                         reinterpret_cast<uint8_t*>(a.big));
+    else
+      hipLaunchKernelGGL(k_geom_emit<false>, dim3(n_blk), dim3(256), 0, s, a.draws, a.n_draws, a.views, a.surv,
+                         a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, nullptr, nullptr);
   }
   if (a.n_views > 1)
     hipLaunchKernelGGL(k_view_id_base, dim3(cdiv(a.n_views, 256)), dim3(256), 0, s, const_cast<DevView*>(a.views),
@@ -997,8 +1092,6 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
   uint8_t* wflag = reinterpret_cast<uint8_t*>(a.big);
   PipeArgs b = a;
   if (a.ordered_bins) {
-    hipLaunchKernelGGL(k_bin_count<true>, dim3(gb), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri, a.TS,
-                       a.bincnt, a.tile_cnt, wflag);
     hipLaunchKernelGGL(k_bin_count_wide<true>, dim3(1024), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri,
                        a.TS, wflag, a.bincnt, a.tile_cnt);
     scan_u32(a.bincnt, a.binoff, &a.status->n_out, 0, a.cap_tri, a.partials, &a.status->n_pairs, a.cap_pairs,
@@ -1019,8 +1112,8 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
   } else {
     hipLaunchKernelGGL(k_fill_u32, dim3(std::min<uint32_t>(cdiv(2ull * a.n_tiles, 256), 2048u)), dim3(256), 0, s,
                        a.tile_cnt, 0u, 2u * a.n_tiles);  // tile_cnt and tile_cur are adjacent
-    hipLaunchKernelGGL(k_bin_count<false>, dim3(gb), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri, a.TS,
-                       a.bincnt, a.tile_cnt, wflag);
+    hipLaunchKernelGGL(k_bin_count, dim3(gb), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri, a.TS,
+                       a.tile_cnt, wflag);
     hipLaunchKernelGGL(k_bin_count_wide<false>, dim3(1024), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri,
                        a.TS, wflag, a.bincnt, a.tile_cnt);
     scan_u32(a.tile_cnt, a.tile_start, nullptr, a.n_tiles, a.n_tiles, a.partials, &a.status->n_pairs, a.cap_pairs,

@@ -23,7 +23,9 @@ struct PipeArgs {
   int raster_kind;     // 0: waves own 8x8 quadrants; 1: triangle-parallel rectangle sweeps
   // scratch
   uint8_t* cnt;        // [n_tri_in] clipped-triangle count per input triangle
-  uint32_t* blk;       // [ceil(n_tri_in/256)] block sums -> offsets
+  uint32_t* blk;       // [2 * (ceil(n_tri_in/256) + 1)] per-block output counts -> offsets, then (at
+                       // ceil(n_tri_in/256) + 1) per-block emitting-triangle counts -> offsets
+  uint2* surv;         // [n_tri_in] (input triangle, first output id) of the emitting triangles, in order
   SetupRec* setup;     // [cap_tri]
   ShadeRec* shade;     // [cap_tri]
   uint32_t cap_tri;
