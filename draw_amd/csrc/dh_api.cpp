@@ -242,6 +242,8 @@ int ensure_scratch(dh_ctx c, uint32_t n_in, uint32_t cap_tri, uint32_t cap_pairs
   HIPCHK(c, c->views.ensure(n_views));
   c->cap_tri = (uint32_t)std::min<size_t>(c->setup.n, 0xFFFFFFF0u);
   c->cap_pairs = (uint32_t)std::min<size_t>(c->keys0.n, 0xFFFFFFF0u);
+  // binning keeps the wide-triangle list (up to cap_tri entries) in keys[1]
+  if (c->cap_pairs < c->cap_tri) return fail(c, DH_E_CAPACITY, "bin buffers smaller than the triangle records");
   return DH_OK;
 }
 

@@ -34,7 +34,8 @@ struct Status {
   uint32_t error;     // bit0: texture slice out of range (scene/mod.rs:167 panic)
   uint32_t frags;     // covered fragments submitted to the visibility test
   uint32_t n_emit;    // input triangles that emit at least one clipped triangle
-  uint32_t pad[2];
+  uint32_t n_wide;    // triangles on the wide-triangle bin list
+  uint32_t pad;
 };
 
 // Material = the reference's `Texture` (scene/mod.rs:206-216) with device maps.

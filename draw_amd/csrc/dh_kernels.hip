@@ -756,32 +756,60 @@ __global__ __launch_bounds__(256) void k_bin_count(const SetupRec* __restrict__ 
   }
 }
 
-// Visits the flagged (wide) triangles one wave at a time: each wave reads the
-// flags of 64 consecutive triangles and processes the set ones in order.
-template <class F>
-__device__ inline void for_each_wide(const uint8_t* __restrict__ wflag, uint32_t n, int lane, F f) {
-  const uint32_t gw = (blockIdx.x * 256u + threadIdx.x) >> 6, nwaves = gridDim.x * 4u;
-  for (uint32_t base = gw * 64u; base < n; base += nwaves * 64u) {
-    const uint32_t i = base + (uint32_t)lane;
-    uint64_t m = __ballot(i < n && wflag[i] != 0);
-    while (m) {
-      const uint32_t k = base + (uint32_t)__builtin_ctzll(m);
-      m &= m - 1;
-      f(k);
+// The wide-triangle list: the flagged triangles of each 4096-triangle chunk,
+// 16 flags per thread, placed by a block scan; one atomic per chunk reserves
+// the chunk's run of the list.  The list order does not matter: a triangle's
+// bin entries go to its own scanned offset (or through per-tile atomics for
+// unordered bins).
+__global__ __launch_bounds__(256) void k_wide_list(const uint8_t* __restrict__ wflag, Status* status,
+                                                   uint32_t cap_tri, uint32_t* __restrict__ list) {
+  __shared__ uint32_t s_tmp[4];
+  __shared__ uint32_t s_base;
+  const uint32_t n = min(status->n_out, cap_tri);
+  for (uint32_t c0 = blockIdx.x * 4096u; c0 < n; c0 += gridDim.x * 4096u) {
+    const uint32_t t0 = c0 + threadIdx.x * 16u;
+    uint32_t mask = 0;
+    if (t0 + 16u <= n) {
+      const uint4 f = *reinterpret_cast<const uint4*>(wflag + t0);  // t0 is a multiple of 16
+      const uint32_t w[4] = {f.x, f.y, f.z, f.w};
+#pragma unroll
+      for (int k = 0; k < 16; k++)
+        if ((w[k >> 2] >> (8 * (k & 3))) & 0xFFu) mask |= 1u << k;
+    } else {
+      for (uint32_t k = 0; t0 + k < n; k++)
+        if (wflag[t0 + k]) mask |= 1u << k;
     }
+    uint32_t total;
+    const uint32_t local = block_exclusive_scan256((uint32_t)__popc(mask), s_tmp, &total);
+    if (threadIdx.x == 0 && total) s_base = atomicAdd(&status->n_wide, total);
+    __syncthreads();
+    uint32_t o = s_base + local;
+    while (mask) {
+      const uint32_t k = (uint32_t)__builtin_ctz(mask);
+      mask &= mask - 1u;
+      list[o++] = t0 + k;
+    }
+    __syncthreads();
   }
+}
+
+// Visits the listed (wide) triangles, one wave per triangle.
+template <class F>
+__device__ inline void for_each_wide(const uint32_t* __restrict__ list, const Status* status, F f) {
+  const uint32_t gw = (blockIdx.x * 256u + threadIdx.x) >> 6, nwaves = gridDim.x * 4u;
+  const uint32_t n = status->n_wide;
+  for (uint32_t w = gw; w < n; w += nwaves) f((uint32_t)__builtin_amdgcn_readfirstlane(list[w]));
 }
 
 template <bool ORDERED>
 __global__ __launch_bounds__(256) void k_bin_count_wide(const SetupRec* __restrict__ setup,
                                                         const DevView* __restrict__ views, const Status* status,
-                                                        uint32_t cap_tri, int TS, const uint8_t* __restrict__ wflag,
+                                                        int TS, const uint32_t* __restrict__ wide_list,
                                                         uint32_t* __restrict__ bincnt, uint32_t* __restrict__ tile_cnt) {
   __shared__ int s_lo[4][64];
   __shared__ uint32_t s_off[4][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t n = min(status->n_out, cap_tri);
-  for_each_wide(wflag, n, lane, [&](uint32_t i) {
+  for_each_wide(wide_list, status, [&](uint32_t i) {
     const SetupRec r = setup[i];
     const DevView& vw = views[r.view];
     Spans g;
@@ -827,7 +855,7 @@ __global__ __launch_bounds__(256) void k_bin_emit(const SetupRec* __restrict__ s
 template <bool ORDERED>
 __global__ __launch_bounds__(256) void k_bin_emit_wide(const SetupRec* __restrict__ setup,
                                                        const DevView* __restrict__ views, const Status* status,
-                                                       uint32_t cap_tri, const uint8_t* __restrict__ wflag, int TS,
+                                                       const uint32_t* __restrict__ wide_list, int TS,
                                                        const uint32_t* __restrict__ binoff,
                                                        const uint32_t* __restrict__ tile_start,
                                                        uint32_t* __restrict__ tile_cur, uint32_t cap_pairs,
@@ -835,8 +863,7 @@ __global__ __launch_bounds__(256) void k_bin_emit_wide(const SetupRec* __restric
   __shared__ int s_lo[4][64];
   __shared__ uint32_t s_off[4][64];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t n = min(status->n_out, cap_tri);
-  for_each_wide(wflag, n, lane, [&](uint32_t i) {
+  for_each_wide(wide_list, status, [&](uint32_t i) {
     const SetupRec r = setup[i];
     const DevView& vw = views[r.view];
     Spans g;
@@ -1090,16 +1117,22 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
   // 4-8 binning
   const uint32_t gb = std::min<uint32_t>(cdiv(a.cap_tri, 256), 4096u);
   uint8_t* wflag = reinterpret_cast<uint8_t*>(a.big);
+  // the wide-triangle list lives in keys[1] until the radix sort needs it
+  // (cap_pairs >= cap_tri); a wave per listed triangle, up to 16384 waves
+  uint32_t* wide_list = a.keys[1];
+  const uint32_t gl = std::min<uint32_t>(cdiv(a.cap_tri, 4096), 1024u);
+  const uint32_t gw = std::min<uint32_t>(cdiv(a.cap_tri, 64), 4096u);
   PipeArgs b = a;
   if (a.ordered_bins) {
-    hipLaunchKernelGGL(k_bin_count_wide<true>, dim3(1024), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri,
-                       a.TS, wflag, a.bincnt, a.tile_cnt);
+    hipLaunchKernelGGL(k_wide_list, dim3(gl), dim3(256), 0, s, wflag, a.status, a.cap_tri, wide_list);
+    hipLaunchKernelGGL(k_bin_count_wide<true>, dim3(gw), dim3(256), 0, s, a.setup, a.views, a.status, a.TS,
+                       wide_list, a.bincnt, a.tile_cnt);
     scan_u32(a.bincnt, a.binoff, &a.status->n_out, 0, a.cap_tri, a.partials, &a.status->n_pairs, a.cap_pairs,
              &a.status->overflow, 2u, s);
     hipLaunchKernelGGL(k_bin_emit<true>, dim3(gb), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri, a.TS,
                        wflag, a.binoff, a.tile_start, a.tile_cur, a.cap_pairs, a.keys[0], a.vals[0]);
-    hipLaunchKernelGGL(k_bin_emit_wide<true>, dim3(1024), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri,
-                       wflag, a.TS, a.binoff, a.tile_start, a.tile_cur, a.cap_pairs, a.keys[0], a.vals[0]);
+    hipLaunchKernelGGL(k_bin_emit_wide<true>, dim3(gw), dim3(256), 0, s, a.setup, a.views, a.status, wide_list,
+                       a.TS, a.binoff, a.tile_start, a.tile_cur, a.cap_pairs, a.keys[0], a.vals[0]);
     // stable radix sort by tile key (one 11-bit pass when the key fits, else
     // 8-bit passes), then the tile ranges
     const int cur = enqueue_radix_sort(a.keys, a.vals, &a.status->n_pairs, a.cap_pairs, a.key_bits, a.hist,
@@ -1114,14 +1147,15 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
                        a.tile_cnt, 0u, 2u * a.n_tiles);  // tile_cnt and tile_cur are adjacent
     hipLaunchKernelGGL(k_bin_count, dim3(gb), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri, a.TS,
                        a.tile_cnt, wflag);
-    hipLaunchKernelGGL(k_bin_count_wide<false>, dim3(1024), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri,
-                       a.TS, wflag, a.bincnt, a.tile_cnt);
+    hipLaunchKernelGGL(k_wide_list, dim3(gl), dim3(256), 0, s, wflag, a.status, a.cap_tri, wide_list);
+    hipLaunchKernelGGL(k_bin_count_wide<false>, dim3(gw), dim3(256), 0, s, a.setup, a.views, a.status, a.TS,
+                       wide_list, a.bincnt, a.tile_cnt);
     scan_u32(a.tile_cnt, a.tile_start, nullptr, a.n_tiles, a.n_tiles, a.partials, &a.status->n_pairs, a.cap_pairs,
              &a.status->overflow, 2u, s);
     hipLaunchKernelGGL(k_bin_emit<false>, dim3(gb), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri, a.TS,
                        wflag, a.binoff, a.tile_start, a.tile_cur, a.cap_pairs, a.keys[0], a.vals[0]);
-    hipLaunchKernelGGL(k_bin_emit_wide<false>, dim3(1024), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri,
-                       wflag, a.TS, a.binoff, a.tile_start, a.tile_cur, a.cap_pairs, a.keys[0], a.vals[0]);
+    hipLaunchKernelGGL(k_bin_emit_wide<false>, dim3(gw), dim3(256), 0, s, a.setup, a.views, a.status, wide_list,
+                       a.TS, a.binoff, a.tile_start, a.tile_cur, a.cap_pairs, a.keys[0], a.vals[0]);
     hipLaunchKernelGGL(k_ranges_from_counts, dim3(std::min<uint32_t>(cdiv(a.n_tiles, 256), 2048u)), dim3(256), 0, s,
                        a.tile_start, a.tile_cnt, a.n_tiles, a.cap_pairs, a.ranges);
   }
