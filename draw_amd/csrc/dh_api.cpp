@@ -205,7 +205,7 @@ dh::DevMaterial default_material(const uint8_t* white) {
 uint64_t scratch_bytes_after(dh_ctx c, uint64_t n_in, uint64_t cap_tri, uint64_t cap_pairs, uint64_t n_tiles,
                              uint64_t n_pixels, uint64_t n_draws, uint64_t n_views) {
   auto mx = [](uint64_t have, uint64_t want) { return std::max<uint64_t>(have, std::max<uint64_t>(want, 1)); };
-  uint64_t b = mx(c->cnt.n, n_in) + 4 * mx(c->blk.n, 2 * ((n_in + 255) / 256 + 1)) + 4 * mx(c->surv.n, 2 * n_in);
+  uint64_t b = mx(c->cnt.n, n_in) + 4 * mx(c->blk.n, 3 * ((n_in + 255) / 256 + 1)) + 4 * mx(c->surv.n, 2 * n_in);
   b += (sizeof(dh::SetupRec) * mx(c->setup.n, cap_tri)) + sizeof(dh::ShadeRec) * mx(c->shade.n, cap_tri);
   b += 4 * (mx(c->bincnt.n, cap_tri) + mx(c->binoff.n, cap_tri) + mx(c->big.n, cap_tri));
   b += 4 * (mx(c->keys0.n, cap_pairs) + mx(c->keys1.n, cap_pairs) + mx(c->vals0.n, cap_pairs) + mx(c->vals1.n, cap_pairs));
@@ -220,7 +220,7 @@ uint64_t scratch_bytes_after(dh_ctx c, uint64_t n_in, uint64_t cap_tri, uint64_t
 int ensure_scratch(dh_ctx c, uint32_t n_in, uint32_t cap_tri, uint32_t cap_pairs, uint32_t n_tiles,
                    uint32_t n_pixels, uint32_t n_draws, uint32_t n_views) {
   HIPCHK(c, c->cnt.ensure(n_in));
-  HIPCHK(c, c->blk.ensure(2ull * ((n_in + 255) / 256 + 1)));  // output counts, then emitting-triangle counts
+  HIPCHK(c, c->blk.ensure(3ull * ((n_in + 255) / 256 + 1)));  // output, unclipped and clipped emitter counts
   HIPCHK(c, c->surv.ensure(2ull * n_in));
   HIPCHK(c, c->setup.ensure(cap_tri));
   HIPCHK(c, c->shade.ensure(cap_tri));
@@ -1043,7 +1043,7 @@ uint64_t ctx_scratch_bytes(dh_ctx c) {
 uint64_t scratch_estimate(uint64_t n_in, uint64_t n_tiles, uint64_t n_pixels, uint64_t n_draws,
                           uint64_t n_views) {
   const uint64_t tri = n_in + n_in / 4 + 1024, pairs = 4 * tri + n_tiles;
-  return std::max<uint64_t>(n_in, 1) + 8 * ((n_in + 255) / 256 + 1) + 8 * n_in +
+  return std::max<uint64_t>(n_in, 1) + 12 * ((n_in + 255) / 256 + 1) + 8 * n_in +
          (sizeof(SetupRec) + sizeof(ShadeRec) + 12) * tri + 16 * pairs +
          4ull * kRadixHistEntries + 4096 + 20 * n_tiles + 4 * n_pixels + sizeof(DevDraw) * n_draws +
          sizeof(DevView) * n_views + 2 * sizeof(Status);

@@ -33,9 +33,9 @@ struct Status {
   uint32_t overflow;  // bit0: triangle records, bit1: bin entries
   uint32_t error;     // bit0: texture slice out of range (scene/mod.rs:167 panic)
   uint32_t frags;     // covered fragments submitted to the visibility test
-  uint32_t n_emit;    // input triangles that emit at least one clipped triangle
+  uint32_t n_simple;  // input triangles emitted unclipped (inside the near and far planes)
+  uint32_t n_clip;    // input triangles that emit after near/far clipping
   uint32_t n_wide;    // triangles on the wide-triangle bin list
-  uint32_t pad;
 };
 
 // Material = the reference's `Texture` (scene/mod.rs:206-216) with device maps.

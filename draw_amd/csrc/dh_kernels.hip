@@ -273,16 +273,16 @@ __device__ inline void make_setup(const FrameParamsPod& fr, const FV& A, const F
   // `exact`: every edge evaluation at |x|,|y| <= R (the vertices and the scan
   // rectangle) has all partial sums below 2^24 in magnitude, so with integer
   // operands each one is exact and e_bc + e_ca + e_ab = f_alpha = f_beta =
-  // f_gama (twice the signed area) at every pixel.
-  double R = (double)x1 > (double)y1 ? (double)x1 : (double)y1;
+  // f_gama (twice the signed area) at every pixel.  S = |A|R + |B|R + |C| +
+  // |D| bounds every partial sum; its float evaluation (4 roundings to
+  // nearest) is at least S (1 - 2^-24)^4, so S_f < 16777000 gives S < 2^24.
+  // (A triangle this leaves out is only handled by the general paths.)
+  float R = (float)(x1 > y1 ? x1 : y1);
 #pragma unroll
-  for (int k = 0; k < 3; k++) {
-    R = fmax(R, fabs((double)X[k]));
-    R = fmax(R, fabs((double)Y[k]));
-  }
+  for (int k = 0; k < 3; k++) R = fmaxf(fmaxf(R, fabsf(X[k])), fabsf(Y[k]));
   auto exact_edge = [R](const float e[4]) {
-    const double S = fabs((double)e[0]) * R + fabs((double)e[1]) * R + fabs((double)e[2]) + fabs((double)e[3]);
-    return S < 16777216.0;
+    const float S = ((fabsf(e[0]) * R + fabsf(e[1]) * R) + fabsf(e[2])) + fabsf(e[3]);
+    return S < 16777000.0f;
   };
   const bool exact = isfinite(R) && exact_edge(ea) && exact_edge(eb) && exact_edge(eg);
   if (exact && __float_as_uint(fa) == 0u && __float_as_uint(fb) == 0u && __float_as_uint(fg) == 0u) flags |= kDrop;
@@ -500,13 +500,28 @@ __device__ inline void bin_count_one(const SetupRec& r, int TS, uint32_t i, uint
 }
 
 // ================================================================== geometry
+// cnt[t]: clipped-triangle count (bits 0-2, at most 4) | kCntClip when the
+// triangle crosses the near or far plane.
+constexpr uint32_t kCntMask = 7u, kCntClip = 8u;
+
+__device__ inline bool inside_plane(const float* pl, V3 a, V3 b, V3 c) {
+  return plane_f(pl, a) > 0.0f && plane_f(pl, b) > 0.0f && plane_f(pl, c) > 0.0f;
+}
+
+// Block-scan word: clipped triangles << 18 | unclipped emitters << 9 |
+// clipped emitters (a block has at most 256 * 4 of the first, 256 of each other).
+__device__ inline uint32_t pack_geom(uint32_t c, bool clip) {
+  return (c << 18) | (c && !clip ? (1u << 9) : 0u) | (c && clip ? 1u : 0u);
+}
+
 __global__ __launch_bounds__(256) void k_geom_count(const DevDraw* __restrict__ draws, uint32_t n_draws,
                                                     const DevView* __restrict__ views, uint32_t n_in,
                                                     uint8_t* __restrict__ cnt, uint32_t* __restrict__ blk,
-                                                    uint32_t* __restrict__ blk_emit) {
+                                                    uint32_t* __restrict__ blk_simple, uint32_t* __restrict__ blk_clip) {
   __shared__ uint32_t s_tmp[4];
   const uint32_t t = blockIdx.x * 256u + threadIdx.x;
   uint32_t c = 0;
+  bool clip = false;
   if (t < n_in) {
     const DevDraw& dr = draws[find_draw(draws, n_draws, t)];
     const FrameParamsPod& fr = views[dr.view].fr;
@@ -516,36 +531,48 @@ __global__ __launch_bounds__(256) void k_geom_count(const DevDraw* __restrict__ 
     b.p = ld3(dr.pos, ti[1]);
     cc.p = ld3(dr.pos, ti[2]);
     c = (uint32_t)cull_and_clip(fr, a, b, cc, !(dr.flags & kDrawNoCull), [](int, const PV&, const PV&, const PV&) {});
-    cnt[t] = (uint8_t)c;
+    // ViewPlane::clip keeps a triangle unchanged exactly when its three
+    // vertices are strictly inside the plane (clip_plane's first branch)
+    if (c) clip = !(inside_plane(fr.planes[0], a.p, b.p, cc.p) && inside_plane(fr.planes[1], a.p, b.p, cc.p));
+    cnt[t] = (uint8_t)(c | (clip ? kCntClip : 0u));
   }
-  // packed: clipped triangles << 16 | emitting triangles (a block has at
-  // most 256 * 4 of the former -- near and far clipping emit at most 4 per
-  // input triangle -- and 256 of the latter)
   uint32_t total;
-  block_exclusive_scan256((c << 16) | (c ? 1u : 0u), s_tmp, &total);
+  block_exclusive_scan256(pack_geom(c, clip), s_tmp, &total);
   if (threadIdx.x == 0) {
-    blk[blockIdx.x] = total >> 16;
-    blk_emit[blockIdx.x] = total & 0xFFFFu;
+    blk[blockIdx.x] = total >> 18;
+    blk_simple[blockIdx.x] = (total >> 9) & 0x1FFu;
+    blk_clip[blockIdx.x] = total & 0x1FFu;
   }
 }
 
-// Ordered list of the input triangles that emit: (triangle, first output id).
+// Ordered lists of the input triangles that emit, (triangle, first output
+// id): the unclipped ones in surv[0, n_simple), the clipped ones after them.
 __global__ __launch_bounds__(256) void k_geom_compact(uint32_t n_in, const uint8_t* __restrict__ cnt,
                                                       const uint32_t* __restrict__ blk_off,
-                                                      const uint32_t* __restrict__ emit_off,
-                                                      uint2* __restrict__ surv) {
+                                                      const uint32_t* __restrict__ simple_off,
+                                                      const uint32_t* __restrict__ clip_off,
+                                                      const Status* __restrict__ status, uint2* __restrict__ surv) {
   __shared__ uint32_t s_tmp[4];
   const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-  const uint32_t c = t < n_in ? (uint32_t)cnt[t] : 0u;
+  const uint32_t q = t < n_in ? (uint32_t)cnt[t] : 0u;
+  const uint32_t c = q & kCntMask;
+  const bool clip = (q & kCntClip) != 0;
   uint32_t total;
-  const uint32_t local = block_exclusive_scan256((c << 16) | (c ? 1u : 0u), s_tmp, &total);
-  if (c) surv[emit_off[blockIdx.x] + (local & 0xFFFFu)] = make_uint2(t, blk_off[blockIdx.x] + (local >> 16));
+  const uint32_t local = block_exclusive_scan256(pack_geom(c, clip), s_tmp, &total);
+  if (!c) return;
+  const uint2 rec = make_uint2(t, blk_off[blockIdx.x] + (local >> 18));
+  if (clip)
+    surv[status->n_simple + clip_off[blockIdx.x] + (local & 0x1FFu)] = rec;
+  else
+    surv[simple_off[blockIdx.x] + ((local >> 9) & 0x1FFu)] = rec;
 }
 
 
 
-// One thread per emitting input triangle (dense: culled triangles take no lane).
-template <bool BIN>
+// One thread per emitting input triangle of one list (dense: culled
+// triangles take no lane).  The unclipped list (CLIP false) skips the
+// clipper, whose attribute registers would otherwise size the whole kernel.
+template <bool BIN, bool CLIP>
 __global__ __launch_bounds__(256) void k_geom_emit(const DevDraw* __restrict__ draws, uint32_t n_draws,
                                                    const DevView* __restrict__ views,
                                                    const uint2* __restrict__ surv,
@@ -554,8 +581,8 @@ __global__ __launch_bounds__(256) void k_geom_emit(const DevDraw* __restrict__ d
                                                    Status* __restrict__ status, int TS, uint32_t* __restrict__ bincnt,
                                                    uint8_t* __restrict__ wflag) {
   const uint32_t j = blockIdx.x * 256u + threadIdx.x;
-  if (j >= status->n_emit) return;
-  const uint2 sv = surv[j];
+  if (j >= (CLIP ? status->n_clip : status->n_simple)) return;
+  const uint2 sv = surv[CLIP ? status->n_simple + j : j];
   const uint32_t t = sv.x, base = sv.y;
   const DevDraw& dr = draws[find_draw(draws, n_draws, t)];
   const FrameParamsPod& fr = views[dr.view].fr;
@@ -580,7 +607,7 @@ __global__ __launch_bounds__(256) void k_geom_emit(const DevDraw* __restrict__ d
     v[k].v = uv[1];
   }
   const uint32_t material = dr.material, view = dr.view, group = dr.group;
-  cull_and_clip(fr, v[0], v[1], v[2], !(dr.flags & kDrawNoCull), [&](int k, const FV& A, const FV& B, const FV& C) {
+  auto out = [&](int k, const FV& A, const FV& B, const FV& C) {
     const uint32_t idx = base + (uint32_t)k;
     if (idx >= cap_tri) {
       atomicOr(&status->overflow, 1u);
@@ -589,7 +616,11 @@ __global__ __launch_bounds__(256) void k_geom_emit(const DevDraw* __restrict__ d
     SetupRec rec;
     make_setup(fr, A, B, C, material, mats[material], view, group, setup + idx, shade + idx, rec);
     if (BIN) bin_count_one(rec, TS, idx, bincnt, wflag);
-  });
+  };
+  if constexpr (CLIP)
+    cull_and_clip(fr, v[0], v[1], v[2], !(dr.flags & kDrawNoCull), out);
+  else
+    out(0, v[0], v[1], v[2]);  // passed the back-face and lateral tests in k_geom_count
 }
 
 // First clipped-triangle id of each view: the exclusive prefix of the clip
@@ -608,7 +639,7 @@ __global__ void k_view_id_base(DevView* __restrict__ views, uint32_t n_views, ui
   } else {
     const uint32_t b = t0 >> 8;
     base = blk_off[b];
-    for (uint32_t t = b << 8; t < t0; t++) base += cnt[t];
+    for (uint32_t t = b << 8; t < t0; t++) base += cnt[t] & kCntMask;
   }
   views[v].id_base = base;
 }
@@ -683,9 +714,25 @@ __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__
 }
 
 // Scans of at most 4 * 1024 elements whose count the host knows: one block.
-__global__ __launch_bounds__(1024) void k_scan_small(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
-                                                     uint32_t n, uint32_t* total_out, uint32_t cap_check,
-                                                     uint32_t* overflow_word, uint32_t overflow_bit) {
+// Up to three independent scans of n <= 4096 values, one block each.
+struct ScanSeg {
+  const uint32_t* in;
+  uint32_t* out;
+  uint32_t* total_out;      // may be null
+  uint32_t* overflow_word;  // may be null
+  uint32_t cap_check, overflow_bit;
+};
+struct ScanSegs {
+  ScanSeg s[3];
+};
+
+__global__ __launch_bounds__(1024) void k_scan_small(ScanSegs segs, uint32_t n) {
+  const ScanSeg& g = segs.s[blockIdx.x];
+  const uint32_t* __restrict__ in = g.in;
+  uint32_t* __restrict__ out = g.out;
+  uint32_t* total_out = g.total_out;
+  uint32_t* overflow_word = g.overflow_word;
+  const uint32_t cap_check = g.cap_check, overflow_bit = g.overflow_bit;
   __shared__ uint32_t s_w[16];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t i0 = threadIdx.x * 4u;
@@ -722,8 +769,9 @@ static void scan_u32(const uint32_t* in, uint32_t* out, const uint32_t* n_ptr, u
                      uint32_t* partials, uint32_t* total_out, uint32_t cap_check, uint32_t* overflow_word,
                      uint32_t overflow_bit, hipStream_t s) {
   if (!n_ptr && n_const <= 4096u) {
-    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, s, in, out, n_const, total_out, cap_check,
-                       overflow_word, overflow_bit);
+    ScanSegs g{};
+    g.s[0] = ScanSeg{in, out, total_out, overflow_word, cap_check, overflow_bit};
+    hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, s, g, n_const);
     return;
   }
   hipLaunchKernelGGL(k_scan_reduce, dim3(kScanG), dim3(256), 0, s, in, n_ptr, n_const, cap, partials);
@@ -1019,7 +1067,7 @@ __global__ void k_group_base(const uint32_t* __restrict__ grp_first, uint32_t n_
   } else {
     const uint32_t b = t0 >> 8;
     base = blk_off[b];
-    for (uint32_t t = b << 8; t < t0; t++) base += cnt[t];
+    for (uint32_t t = b << 8; t < t0; t++) base += cnt[t] & kCntMask;
   }
   grp_base[g] = base;
 }
@@ -1094,21 +1142,37 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
   const uint32_t n_blk = cdiv(a.n_tri_in, 256);
   // 1-3 geometry with ordered compaction
   if (n_blk) {
-    uint32_t* blk_emit = a.blk + n_blk + 1;
+    uint32_t* blk_simple = a.blk + n_blk + 1;
+    uint32_t* blk_clip = a.blk + 2 * (n_blk + 1);
     hipLaunchKernelGGL(k_geom_count, dim3(n_blk), dim3(256), 0, s, a.draws, a.n_draws, a.views, a.n_tri_in, a.cnt,
-                       a.blk, blk_emit);
-    scan_u32(a.blk, a.blk, nullptr, n_blk, n_blk, a.partials, &a.status->n_out, a.cap_tri, &a.status->overflow, 1u,
-             s);
-    scan_u32(blk_emit, blk_emit, nullptr, n_blk, n_blk, a.partials, &a.status->n_emit, 0u, nullptr, 0u, s);
-    hipLaunchKernelGGL(k_geom_compact, dim3(n_blk), dim3(256), 0, s, a.n_tri_in, a.cnt, a.blk, blk_emit, a.surv);
+                       a.blk, blk_simple, blk_clip);
+    if (n_blk <= 4096u) {
+      ScanSegs g{};
+      g.s[0] = ScanSeg{a.blk, a.blk, &a.status->n_out, &a.status->overflow, a.cap_tri, 1u};
+      g.s[1] = ScanSeg{blk_simple, blk_simple, &a.status->n_simple, nullptr, 0u, 0u};
+      g.s[2] = ScanSeg{blk_clip, blk_clip, &a.status->n_clip, nullptr, 0u, 0u};
+      hipLaunchKernelGGL(k_scan_small, dim3(3), dim3(1024), 0, s, g, n_blk);
+    } else {
+      scan_u32(a.blk, a.blk, nullptr, n_blk, n_blk, a.partials, &a.status->n_out, a.cap_tri, &a.status->overflow,
+               1u, s);
+      scan_u32(blk_simple, blk_simple, nullptr, n_blk, n_blk, a.partials, &a.status->n_simple, 0u, nullptr, 0u, s);
+      scan_u32(blk_clip, blk_clip, nullptr, n_blk, n_blk, a.partials, &a.status->n_clip, 0u, nullptr, 0u, s);
+    }
+    hipLaunchKernelGGL(k_geom_compact, dim3(n_blk), dim3(256), 0, s, a.n_tri_in, a.cnt, a.blk, blk_simple, blk_clip,
+                       a.status, a.surv);
     // ordered bins: the per-triangle tile counts and wide flags come with the records
-    if (a.ordered_bins)
-      hipLaunchKernelGGL(k_geom_emit<true>, dim3(n_blk), dim3(256), 0, s, a.draws, a.n_draws, a.views, a.surv,
-                         a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, a.bincnt,
-                         reinterpret_cast<uint8_t*>(a.big));
-    else
-      hipLaunchKernelGGL(k_geom_emit<false>, dim3(n_blk), dim3(256), 0, s, a.draws, a.n_draws, a.views, a.surv,
-                         a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, nullptr, nullptr);
+    uint8_t* wf = reinterpret_cast<uint8_t*>(a.big);
+    if (a.ordered_bins) {
+      hipLaunchKernelGGL((k_geom_emit<true, false>), dim3(n_blk), dim3(256), 0, s, a.draws, a.n_draws, a.views,
+                         a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, a.bincnt, wf);
+      hipLaunchKernelGGL((k_geom_emit<true, true>), dim3(n_blk), dim3(256), 0, s, a.draws, a.n_draws, a.views,
+                         a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, a.bincnt, wf);
+    } else {
+      hipLaunchKernelGGL((k_geom_emit<false, false>), dim3(n_blk), dim3(256), 0, s, a.draws, a.n_draws, a.views,
+                         a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, nullptr, nullptr);
+      hipLaunchKernelGGL((k_geom_emit<false, true>), dim3(n_blk), dim3(256), 0, s, a.draws, a.n_draws, a.views,
+                         a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, nullptr, nullptr);
+    }
   }
   if (a.n_views > 1)
     hipLaunchKernelGGL(k_view_id_base, dim3(cdiv(a.n_views, 256)), dim3(256), 0, s, const_cast<DevView*>(a.views),

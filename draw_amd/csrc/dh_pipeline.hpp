@@ -23,9 +23,10 @@ struct PipeArgs {
   int raster_kind;     // 0: waves own 8x8 quadrants; 1: triangle-parallel rectangle sweeps
   // scratch
   uint8_t* cnt;        // [n_tri_in] clipped-triangle count per input triangle
-  uint32_t* blk;       // [2 * (ceil(n_tri_in/256) + 1)] per-block output counts -> offsets, then (at
-                       // ceil(n_tri_in/256) + 1) per-block emitting-triangle counts -> offsets
-  uint2* surv;         // [n_tri_in] (input triangle, first output id) of the emitting triangles, in order
+  uint32_t* blk;       // [3 * (ceil(n_tri_in/256) + 1)] per-block counts -> offsets: clipped triangles,
+                       // then (each ceil(n_tri_in/256) + 1 further) unclipped and clipped emitters
+  uint2* surv;         // [n_tri_in] (input triangle, first output id) of the emitting triangles: the
+                       // unclipped ones in order, then the clipped ones in order
   SetupRec* setup;     // [cap_tri]
   ShadeRec* shade;     // [cap_tri]
   uint32_t cap_tri;
