@@ -117,7 +117,7 @@ struct dh_ctx_s {
   DevBuf<uint32_t> hist, partials;
   DevBuf<uint2> ranges;
   DevBuf<uint32_t> tile_cnt, tile_start;
-  bool ordered_bins = true;
+  int bin_mode = 2;  // dh_set_bin_order: 0 unordered, 1 submission order, 2 near first
   DevBuf<uint32_t> vis;
   DevBuf<dh::Status> status;
   DevBuf<dh::DevDraw> draws;
@@ -317,7 +317,7 @@ extern "C" int dh_ctx_create_shared(dh_ctx parent, dh_ctx* out) {
     (*out)->TS = parent->TS;
     (*out)->waves = parent->waves;
     (*out)->raster_kind = parent->raster_kind;
-    (*out)->ordered_bins = parent->ordered_bins;
+    (*out)->bin_mode = parent->bin_mode;
   }
   return rc;
 }
@@ -376,9 +376,9 @@ extern "C" int dh_set_graph(dh_ctx c, int32_t enable) {
   return DH_OK;
 }
 
-extern "C" int dh_set_bin_order(dh_ctx c, int32_t ordered) {
-  if (!c) return DH_E_ARG;
-  c->ordered_bins = ordered != 0;
+extern "C" int dh_set_bin_order(dh_ctx c, int32_t mode) {
+  if (!c || mode < 0 || mode > 2) return DH_E_ARG;
+  c->bin_mode = mode;
   return DH_OK;
 }
 
@@ -592,8 +592,8 @@ int build_tables(dh_ctx c, const dh_view* views, uint32_t n_views, bool allow_tr
   dh_handle last_obj = 0;
   key->clear();
   key->push_back(n_views);
-  key->push_back((uint64_t)c->TS << 32 | (uint64_t)c->raster_kind << 24 | (uint64_t)c->waves << 1 |
-                 (c->ordered_bins ? 1u : 0u));
+  key->push_back((uint64_t)c->TS << 32 | (uint64_t)c->raster_kind << 24 | (uint64_t)c->waves << 2 |
+                 (uint64_t)c->bin_mode);
   uint64_t n_in = 0, n_tiles = 0, n_pixels = 0;
   for (uint32_t v = 0; v < n_views; v++) {
     const dh_view& vq = views[v];
@@ -616,6 +616,17 @@ int build_tables(dh_ctx c, const dh_view* views, uint32_t n_views, bool allow_tr
     dv.out_winner = cv.winner;
     dv.tri_begin = (uint32_t)n_in;
     dv.id_base = 0;
+    {
+      // half the camera's distance to its near plane (plane_f = n.p + d)
+      const float* pl = fr->planes[0];
+      const double nn = std::sqrt((double)pl[0] * pl[0] + (double)pl[1] * pl[1] + (double)pl[2] * pl[2]);
+      const double dn = std::fabs((double)pl[0] * fr->cam[0] + (double)pl[1] * fr->cam[1] +
+                                  (double)pl[2] * fr->cam[2] + (double)pl[3]) / nn;
+      const float h = (float)(0.5 * dn);
+      uint32_t hb = 0;
+      memcpy(&hb, &h, 4);
+      dv.depth_base = (std::isfinite(h) && h > 0.0f) ? hb : 0u;
+    }
     key->push_back((uint64_t)fr->W << 32 | (uint32_t)fr->H);
     key->push_back(vq.canvas);
     key->push_back(vq.n_meshes);
@@ -711,7 +722,8 @@ dh::PipeArgs make_args(dh_ctx c, const dh::DevMaterial* mats, uint32_t n_in, uin
   a.bincnt = c->bincnt.p;
   a.binoff = c->binoff.p;
   a.big = c->big.p;
-  a.ordered_bins = c->ordered_bins;
+  a.ordered_bins = c->bin_mode != 0;
+  a.depth_order = c->bin_mode == 2;
   a.tile_cnt = c->tile_cnt.p;
   a.tile_cur = c->tile_cnt.p + n_tiles;
   a.tile_start = c->tile_start.p;
@@ -824,6 +836,7 @@ int render_views(dh_ctx c, const dh_view* views, uint32_t n_views, bool use_grap
         ta.pass.n_draws = (uint32_t)c->h_tdraws.size();
         ta.pass.status = c->status.p + 1;
         ta.pass.ordered_bins = true;  // blending needs every tile list in submission order
+        ta.pass.depth_order = false;
         dh::enqueue_transparent_prep(oa, ta, c->stream);
         const dh::PipeArgs tb = dh::enqueue_front(ta.pass, c->stream);
         dh::enqueue_blend(tb, ta, c->stream);

@@ -77,6 +77,7 @@ struct DevView {
   uint32_t* out_winner;   // canvas winner ids (may be null), rows bottom-up
   uint32_t tri_begin;     // first input triangle of this view
   uint32_t id_base;       // first clipped-triangle id of this view (set on the device)
+  uint32_t depth_base;    // float bits of half the camera's near-plane distance (near-first bin order)
 };
 
 // Raster record of one clipped triangle: the edge-function constants of the

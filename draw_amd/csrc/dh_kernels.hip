@@ -500,6 +500,18 @@ __device__ inline void bin_count_one(const SetupRec& r, int TS, uint32_t i, uint
 }
 
 // ================================================================== geometry
+// Near-first binning order: an 8-bit key of the nearest vertex depth, 16
+// steps per octave above depth_base (the float bits of half the near-plane
+// distance), so 16 octaves of depth; monotonic for positive depths, others
+// sort first.  Only the speed of the raster depends on it.
+constexpr int kDepthKeyBits = 8;
+__device__ inline uint32_t depth_order_key(const SetupRec& r, uint32_t depth_base) {
+  const float d = fminf(fminf(r.d[0], r.d[1]), r.d[2]);
+  const uint32_t b = __float_as_uint(d);
+  if (!(d > 0.0f) || b <= depth_base) return 0u;
+  return min((b - depth_base) >> 19, (1u << kDepthKeyBits) - 1u);
+}
+
 // cnt[t]: clipped-triangle count (bits 0-2, at most 4) | kCntClip when the
 // triangle crosses the near or far plane.
 constexpr uint32_t kCntMask = 7u, kCntClip = 8u;
@@ -579,7 +591,8 @@ __global__ __launch_bounds__(256) void k_geom_emit(const DevDraw* __restrict__ d
                                                    const DevMaterial* __restrict__ mats, SetupRec* __restrict__ setup,
                                                    ShadeRec* __restrict__ shade, uint32_t cap_tri,
                                                    Status* __restrict__ status, int TS, uint32_t* __restrict__ bincnt,
-                                                   uint8_t* __restrict__ wflag) {
+                                                   uint8_t* __restrict__ wflag, uint32_t* __restrict__ dkey,
+                                                   uint32_t* __restrict__ dval) {
   const uint32_t j = blockIdx.x * 256u + threadIdx.x;
   if (j >= (CLIP ? status->n_clip : status->n_simple)) return;
   const uint2 sv = surv[CLIP ? status->n_simple + j : j];
@@ -616,6 +629,10 @@ __global__ __launch_bounds__(256) void k_geom_emit(const DevDraw* __restrict__ d
     SetupRec rec;
     make_setup(fr, A, B, C, material, mats[material], view, group, setup + idx, shade + idx, rec);
     if (BIN) bin_count_one(rec, TS, idx, bincnt, wflag);
+    if (dkey) {
+      dkey[idx] = depth_order_key(rec, views[view].depth_base);
+      dval[idx] = idx;
+    }
   };
   if constexpr (CLIP)
     cull_and_clip(fr, v[0], v[1], v[2], !(dr.flags & kDrawNoCull), out);
@@ -872,32 +889,56 @@ __global__ __launch_bounds__(256) void k_bin_count_wide(const SetupRec* __restri
 // Pass 2.  Ordered: (tile key, triangle) pairs at the triangle's scanned
 // offset, for the stable radix sort.  Unordered: the triangle id goes straight
 // into its tile's list at tile_start + atomicAdd(cursor).
-template <bool ORDERED>
+// SORTED (ordered bins, near first): position j of the depth order takes
+// triangle order[j] at the scanned offset sorted_off[j], and records the
+// offsets of wide triangles in binoff for k_bin_emit_wide.
+template <bool ORDERED, bool SORTED>
 __global__ __launch_bounds__(256) void k_bin_emit(const SetupRec* __restrict__ setup, const DevView* __restrict__ views,
                                                   const Status* status, uint32_t cap_tri, int TS,
-                                                  const uint8_t* __restrict__ wflag,
-                                                  const uint32_t* __restrict__ binoff,
+                                                  const uint8_t* __restrict__ wflag, uint32_t* __restrict__ binoff,
+                                                  const uint32_t* __restrict__ order,
+                                                  const uint32_t* __restrict__ sorted_off,
                                                   const uint32_t* __restrict__ tile_start,
                                                   uint32_t* __restrict__ tile_cur, uint32_t cap_pairs,
                                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   const uint32_t n = min(status->n_out, cap_tri);
-  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
-    if (wflag[i]) continue;
-    const SetupRec r = setup[i];
-    if (r.flags & kDrop) continue;
-    Spans g;
-    spans_init(r, TS, g);
-    const DevView& vw = views[r.view];
-    const uint32_t o = ORDERED ? binoff[i] : 0u;
-    thread_tiles(g, TS, [&](uint32_t e, int ty, int tx) {
+  for (uint32_t j = blockIdx.x * 256u + threadIdx.x; j < n; j += gridDim.x * 256u) {
+    const uint32_t i = SORTED ? order[j] : j;
+    if (wflag[i]) {
+      if (SORTED) binoff[i] = sorted_off[j];
+      continue;
+    }
+    const uint32_t o = SORTED ? sorted_off[j] : (ORDERED ? binoff[i] : 0u);
+    auto put = [&](const DevView& vw, uint32_t e, int ty, int tx) {
       const uint32_t key = vw.tile_base + (uint32_t)ty * vw.ntx + (uint32_t)tx;
       const uint32_t pos = ORDERED ? o + e : tile_start[key] + atomicAdd(&tile_cur[key], 1u);
       if (pos < cap_pairs) {
         if (ORDERED) keys[pos] = key;
         vals[pos] = i;
       }
-    });
+    };
+    // the record's last 16 bytes first: a triangle within one tile needs no edges
+    const uint4 h = reinterpret_cast<const uint4*>(setup + i)[5];
+    if (h.z & kDrop) continue;
+    const uint32_t cx0 = (h.x & 0xFFFFu) / (uint32_t)TS, cx1 = (h.x >> 16) / (uint32_t)TS;
+    const uint32_t cy0 = (h.y & 0xFFFFu) / (uint32_t)TS, cy1 = (h.y >> 16) / (uint32_t)TS;
+    if (cx0 == cx1 && cy0 == cy1) {
+      put(views[h.w], 0u, (int)cy0, (int)cx0);
+      continue;
+    }
+    const SetupRec r = setup[i];
+    Spans g;
+    spans_init(r, TS, g);
+    const DevView& vw = views[r.view];
+    thread_tiles(g, TS, [&](uint32_t e, int ty, int tx) { put(vw, e, ty, tx); });
   }
+}
+
+// out[j] = src[idx[j]] for j < min(*n_ptr, cap).
+__global__ __launch_bounds__(256) void k_gather_u32(const uint32_t* __restrict__ idx, const uint32_t* __restrict__ src,
+                                                    const uint32_t* n_ptr, uint32_t cap, uint32_t* __restrict__ out) {
+  const uint32_t n = min(*n_ptr, cap);
+  for (uint32_t j = blockIdx.x * 256u + threadIdx.x; j < n; j += gridDim.x * 256u) out[j] = src[idx[j]];
 }
 
 template <bool ORDERED>
@@ -1108,12 +1149,14 @@ __global__ __launch_bounds__(256) void k_permute_tris(const uint32_t* __restrict
 static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
 // Stable LSD radix sort of (keys[0], vals[0]) by the low key_bits bits of the
-// keys, *n_ptr items (at most cap): one 11-bit pass when key_bits <= 11, else
+// keys, *n_ptr items (at most cap): one 8-bit pass when key_bits <= 8, one
+// 11-bit pass when key_bits <= 11, else
 // 8-bit passes.  Returns the index of the buffer pair holding the result.
 int enqueue_radix_sort(uint32_t* const keys[2], uint32_t* const vals[2], const uint32_t* n_ptr, uint32_t cap,
                        int key_bits, uint32_t* hist, uint32_t* partials, hipStream_t s) {
   int cur = 0;
-  const bool one_pass = key_bits <= 11;
+  // one 8-bit pass for keys of up to 8 bits, one 11-bit pass up to 11 bits
+  const bool one_pass = key_bits > 8 && key_bits <= 11;
   const int step = one_pass ? 11 : 8;
   for (int shift = 0; shift < key_bits; shift += step) {
     const uint32_t nh = (one_pass ? 2048u : 256u) * kRadixW;
@@ -1136,6 +1179,9 @@ int enqueue_radix_sort(uint32_t* const keys[2], uint32_t* const vals[2], const u
 // Status reset, geometry, view bases and binning of one pipeline pass;
 // returns the arguments the per-tile stages read (tile lists in b.vals[0]).
 PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
+  // near-first lists keep the depth sort in keys[1] / vals[1] (behind the
+  // wide-triangle list), which needs cap_pairs >= 3 cap_tri
+  const bool depth_sorted = a.ordered_bins && a.depth_order && (uint64_t)a.cap_pairs >= 3ull * a.cap_tri;
   if (a.ev[0]) (void)hipEventRecord(a.ev[0], s);
   hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(a.status), 0u,
                      (uint32_t)(sizeof(Status) / 4));
@@ -1160,18 +1206,23 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
     }
     hipLaunchKernelGGL(k_geom_compact, dim3(n_blk), dim3(256), 0, s, a.n_tri_in, a.cnt, a.blk, blk_simple, blk_clip,
                        a.status, a.surv);
-    // ordered bins: the per-triangle tile counts and wide flags come with the records
+    // ordered bins: the per-triangle tile counts and wide flags come with the
+    // records, and for near-first lists each record's depth key
     uint8_t* wf = reinterpret_cast<uint8_t*>(a.big);
+    uint32_t* dk = depth_sorted ? a.keys[1] + a.cap_tri : nullptr;
+    uint32_t* dv = depth_sorted ? a.vals[1] : nullptr;
     if (a.ordered_bins) {
       hipLaunchKernelGGL((k_geom_emit<true, false>), dim3(n_blk), dim3(256), 0, s, a.draws, a.n_draws, a.views,
-                         a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, a.bincnt, wf);
+                         a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, a.bincnt, wf, dk, dv);
       hipLaunchKernelGGL((k_geom_emit<true, true>), dim3(n_blk), dim3(256), 0, s, a.draws, a.n_draws, a.views,
-                         a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, a.bincnt, wf);
+                         a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, a.bincnt, wf, dk, dv);
     } else {
       hipLaunchKernelGGL((k_geom_emit<false, false>), dim3(n_blk), dim3(256), 0, s, a.draws, a.n_draws, a.views,
-                         a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, nullptr, nullptr);
+                         a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, nullptr, nullptr, nullptr,
+                         nullptr);
       hipLaunchKernelGGL((k_geom_emit<false, true>), dim3(n_blk), dim3(256), 0, s, a.draws, a.n_draws, a.views,
-                         a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, nullptr, nullptr);
+                         a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, nullptr, nullptr, nullptr,
+                         nullptr);
     }
   }
   if (a.n_views > 1)
@@ -1191,14 +1242,34 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_wide_list, dim3(gl), dim3(256), 0, s, wflag, a.status, a.cap_tri, wide_list);
     hipLaunchKernelGGL(k_bin_count_wide<true>, dim3(gw), dim3(256), 0, s, a.setup, a.views, a.status, a.TS,
                        wide_list, a.bincnt, a.tile_cnt);
-    scan_u32(a.bincnt, a.binoff, &a.status->n_out, 0, a.cap_tri, a.partials, &a.status->n_pairs, a.cap_pairs,
-             &a.status->overflow, 2u, s);
-    hipLaunchKernelGGL(k_bin_emit<true>, dim3(gb), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri, a.TS,
-                       wflag, a.binoff, a.tile_start, a.tile_cur, a.cap_pairs, a.keys[0], a.vals[0]);
+    if (depth_sorted) {
+      // near first: the clipped triangles in depth-key order (one stable
+      // 8-bit pass over keys[1] + cap_tri / vals[1], results in keys[1] +
+      // 2 cap_tri / vals[1] + cap_tri), then the bin offsets in that order
+      uint32_t* const dkeys[2] = {a.keys[1] + a.cap_tri, a.keys[1] + 2ull * a.cap_tri};
+      uint32_t* const dvals[2] = {a.vals[1], a.vals[1] + a.cap_tri};
+      const int dc = enqueue_radix_sort(dkeys, dvals, &a.status->n_out, a.cap_tri, kDepthKeyBits, a.hist,
+                                        a.partials, s);
+      uint32_t* order = dvals[dc];
+      uint32_t* sorted_off = dkeys[dc];
+      hipLaunchKernelGGL(k_gather_u32, dim3(gb), dim3(256), 0, s, order, a.bincnt, &a.status->n_out, a.cap_tri,
+                         sorted_off);
+      scan_u32(sorted_off, sorted_off, &a.status->n_out, 0, a.cap_tri, a.partials, &a.status->n_pairs, a.cap_pairs,
+               &a.status->overflow, 2u, s);
+      hipLaunchKernelGGL((k_bin_emit<true, true>), dim3(gb), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri,
+                         a.TS, wflag, a.binoff, order, sorted_off, a.tile_start, a.tile_cur, a.cap_pairs, a.keys[0],
+                         a.vals[0]);
+    } else {
+      scan_u32(a.bincnt, a.binoff, &a.status->n_out, 0, a.cap_tri, a.partials, &a.status->n_pairs, a.cap_pairs,
+               &a.status->overflow, 2u, s);
+      hipLaunchKernelGGL((k_bin_emit<true, false>), dim3(gb), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri,
+                         a.TS, wflag, a.binoff, nullptr, nullptr, a.tile_start, a.tile_cur, a.cap_pairs, a.keys[0],
+                         a.vals[0]);
+    }
     hipLaunchKernelGGL(k_bin_emit_wide<true>, dim3(gw), dim3(256), 0, s, a.setup, a.views, a.status, wide_list,
                        a.TS, a.binoff, a.tile_start, a.tile_cur, a.cap_pairs, a.keys[0], a.vals[0]);
-    // stable radix sort by tile key (one 11-bit pass when the key fits, else
-    // 8-bit passes), then the tile ranges
+    // stable radix sort by tile key (one pass when the key has at most 11
+    // bits, else 8-bit passes), then the tile ranges
     const int cur = enqueue_radix_sort(a.keys, a.vals, &a.status->n_pairs, a.cap_pairs, a.key_bits, a.hist,
                                        a.partials, s);
     b.vals[0] = a.vals[cur];
@@ -1216,8 +1287,9 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
                        wide_list, a.bincnt, a.tile_cnt);
     scan_u32(a.tile_cnt, a.tile_start, nullptr, a.n_tiles, a.n_tiles, a.partials, &a.status->n_pairs, a.cap_pairs,
              &a.status->overflow, 2u, s);
-    hipLaunchKernelGGL(k_bin_emit<false>, dim3(gb), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri, a.TS,
-                       wflag, a.binoff, a.tile_start, a.tile_cur, a.cap_pairs, a.keys[0], a.vals[0]);
+    hipLaunchKernelGGL((k_bin_emit<false, false>), dim3(gb), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri,
+                       a.TS, wflag, a.binoff, nullptr, nullptr, a.tile_start, a.tile_cur, a.cap_pairs, a.keys[0],
+                       a.vals[0]);
     hipLaunchKernelGGL(k_bin_emit_wide<false>, dim3(gw), dim3(256), 0, s, a.setup, a.views, a.status, wide_list,
                        a.TS, a.binoff, a.tile_start, a.tile_cur, a.cap_pairs, a.keys[0], a.vals[0]);
     hipLaunchKernelGGL(k_ranges_from_counts, dim3(std::min<uint32_t>(cdiv(a.n_tiles, 256), 2048u)), dim3(256), 0, s,

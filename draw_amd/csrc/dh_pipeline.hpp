@@ -34,6 +34,7 @@ struct PipeArgs {
   uint32_t* binoff;    // [cap_tri]
   uint32_t* big;       // [cap_tri] used as one byte per triangle: 1 = wide, binned by a wave
   bool ordered_bins;   // stable radix sort (per-tile lists in submission order)
+  bool depth_order;    // ordered bins: per-tile lists near first (triangles sorted by depth key before binning)
   uint32_t* tile_cnt;  // [2 * n_tiles] per-tile counts, then per-tile cursors
   uint32_t* tile_cur;  // = tile_cnt + n_tiles
   uint32_t* tile_start;  // [n_tiles] exclusive scan of tile_cnt

@@ -75,8 +75,11 @@ class Renderer:
         with its tile size and waves per tile."""
         self._check(self.L.dh_set_raster_kernel(self.ctx, int(kind), int(tile), int(waves)), "dh_set_raster_kernel")
 
-    def set_bin_order(self, ordered: bool):
-        self._check(self.L.dh_set_bin_order(self.ctx, int(bool(ordered))), "dh_set_bin_order")
+    def set_bin_order(self, mode):
+        """dh_set_bin_order: 0 unordered, 1 submission order, 2 near first
+        (default); False/True mean 0/1."""
+        mode = int(mode) if not isinstance(mode, bool) else int(mode)
+        self._check(self.L.dh_set_bin_order(self.ctx, mode), "dh_set_bin_order")
 
     def set_graph(self, on: bool):
         """dh_set_graph: replay repeated frame structures from a hipGraph."""

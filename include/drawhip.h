@@ -134,11 +134,14 @@ int dh_set_raster_kernel(dh_ctx ctx, int32_t kind, int32_t tile, int32_t waves_p
  * sizes, tiling); uniforms may change.  1 (default) enables this, 0 launches
  * every kernel eagerly (per-stage profiling times need eager frames). */
 int dh_set_graph(dh_ctx ctx, int32_t enable);
-/* Bin order: 1 (default) sorts each tile's triangle list stably so that it
- * keeps the submission order; 0 builds the lists with atomics, in no
- * particular order.  The frame is the same either way: each pixel keeps the
- * minimum of (depth, submission index). */
-int dh_set_bin_order(dh_ctx ctx, int32_t ordered);
+/* Bin order of the per-tile triangle lists.  2 (default): near first -- the
+ * clipped triangles are ordered by their nearest vertex depth (11-bit key,
+ * stable) before binning, so that the raster's hierarchical depth test
+ * rejects most occluded triangles; 1: submission order (stable radix sort
+ * by tile); 0: built with atomics, in no particular order.  The frame is the
+ * same in every mode: each pixel keeps the minimum of (depth, submission
+ * index).  The transparent pass always bins in submission order. */
+int dh_set_bin_order(dh_ctx ctx, int32_t mode);
 
 /* ---- resources (Scene::add_obj, scene/mod.rs:788-801; Object::new,
  *      object.rs:34-71; TextureMap::new, scene/mod.rs:113-126) ---------- */

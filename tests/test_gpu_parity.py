@@ -88,22 +88,46 @@ def test_soup_1m_tiling_invariance(renderer):
 
 
 def test_bin_order_modes_identical(renderer):
-    """Unordered (atomic) and ordered (stable radix sort) bin lists give the
+    """Unordered (atomic), submission-ordered and near-first bin lists give the
     same frame: the per-pixel visibility key does not depend on list order."""
     s = workloads.soup_scene(5, n_tri=200_000, width=1280, height=720, tex_size=512)
-    renderer.set_bin_order(False)
-    a = _gpu(renderer, s)
-    renderer.set_bin_order(True)
+    frames = []
     try:
-        b = _gpu(renderer, s)
-        c = _gpu(renderer, s, 32, 4)
+        for mode in (0, 1, 2):
+            renderer.set_bin_order(mode)
+            frames.append(_gpu(renderer, s))
+            frames.append(_gpu(renderer, s, 32, 4))
+            frames.append(_gpu(renderer, s, 16, 4, kind=1))
     finally:
-        renderer.set_bin_order(False)
-    for x, y, z in zip(a, b, c):
-        assert np.array_equal(x.view(np.uint8), y.view(np.uint8))
-        assert np.array_equal(x.view(np.uint8), z.view(np.uint8))
+        renderer.set_bin_order(2)
+    for other in frames[1:]:
+        for x, y in zip(frames[0], other):
+            assert np.array_equal(x.view(np.uint8), y.view(np.uint8))
     ref = oracle(s, grid=(13, 11, 2, 3))
-    assert_same(*a, ref, mask=grid_mask(720, 1280, (13, 11, 2, 3)), label="soup200k sampled")
+    assert_same(*frames[0], ref, mask=grid_mask(720, 1280, (13, 11, 2, 3)), label="soup200k sampled")
+
+
+def test_near_first_back_to_front_submission(renderer):
+    """A soup submitted back to front (the near-first order's worst case for
+    submission-ordered lists) gives the oracle's frame in every bin mode."""
+    import copy
+    s = workloads.soup_scene(6, n_tri=60_000, width=640, height=480, tex_size=256)
+    o = s.objects[0]
+    tri = o.meshes[0].triangles
+    zc = o.vertices.astype(np.float64)[tri[:, :3]].mean(axis=1)[:, 2]
+    o2 = copy.copy(o)
+    m2 = copy.copy(o.meshes[0])
+    m2.triangles = np.ascontiguousarray(tri[np.argsort(zc, kind="stable")])
+    o2.meshes = [m2]
+    s.objects = [o2]
+    ref = oracle(s)
+    try:
+        for mode in (1, 2):
+            renderer.set_bin_order(mode)
+            f, d, w = _gpu(renderer, s)
+            assert_same(f, d, w, ref, label=f"back to front, bin mode {mode}")
+    finally:
+        renderer.set_bin_order(2)
 
 
 def test_multi_object_and_default_material(renderer):
