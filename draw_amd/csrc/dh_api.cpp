@@ -120,8 +120,15 @@ struct dh_ctx_s {
   int bin_mode = 2;  // dh_set_bin_order: 0 unordered, 1 submission order, 2 near first
   DevBuf<uint32_t> vis;
   DevBuf<dh::Status> status;
-  DevBuf<dh::DevDraw> draws;
-  DevBuf<dh::DevView> views;
+  // draw and view tables of a frame in one device buffer (one upload per
+  // frame from the pinned h_tables): draws at 0, views at tables_view_off
+  DevBuf<uint8_t> tables;
+  size_t tables_view_off = 0;
+  dh::DevDraw* draws = nullptr;
+  dh::DevView* views = nullptr;
+  uint8_t* h_tables = nullptr;  // pinned host staging of the same layout
+  size_t h_tables_n = 0;
+  dh::Status* d_status_host = nullptr;  // device address of h_status
   DevBuf<uint32_t> export_buf;  // RGBA staging of dh_canvas_read_rgba
   DevBuf<dh_gui_vertex> gui_vtx;  // GUI overlay draw list
   DevBuf<uint4> gui_tri, gui_clip;
@@ -201,6 +208,12 @@ dh::DevMaterial default_material(const uint8_t* white) {
   return m;
 }
 
+// The draw/view table layout: draws, then views at a 256-byte boundary.
+size_t draw_bytes(uint64_t n_draws) { return (std::max<uint64_t>(n_draws, 1) * sizeof(dh::DevDraw) + 255) & ~(size_t)255; }
+size_t tables_bytes(uint64_t n_draws, uint64_t n_views) {
+  return draw_bytes(n_draws) + std::max<uint64_t>(n_views, 1) * sizeof(dh::DevView);
+}
+
 // Bytes every scratch buffer would hold after ensure_scratch with these sizes.
 uint64_t scratch_bytes_after(dh_ctx c, uint64_t n_in, uint64_t cap_tri, uint64_t cap_pairs, uint64_t n_tiles,
                              uint64_t n_pixels, uint64_t n_draws, uint64_t n_views) {
@@ -211,8 +224,7 @@ uint64_t scratch_bytes_after(dh_ctx c, uint64_t n_in, uint64_t cap_tri, uint64_t
   b += 4 * (mx(c->keys0.n, cap_pairs) + mx(c->keys1.n, cap_pairs) + mx(c->vals0.n, cap_pairs) + mx(c->vals1.n, cap_pairs));
   b += 4 * (mx(c->hist.n, dh::kRadixHistEntries) + mx(c->partials.n, 1024)) + 8 * mx(c->ranges.n, n_tiles);
   b += 4 * (mx(c->tile_cnt.n, 2 * n_tiles) + mx(c->tile_start.n, n_tiles) + mx(c->vis.n, n_pixels));
-  b += sizeof(dh::Status) * mx(c->status.n, 2) + sizeof(dh::DevDraw) * mx(c->draws.n, n_draws) +
-       sizeof(dh::DevView) * mx(c->views.n, n_views);
+  b += sizeof(dh::Status) * mx(c->status.n, 2) + mx(c->tables.n, tables_bytes(n_draws, n_views));
   return b;
 }
 
@@ -238,8 +250,22 @@ int ensure_scratch(dh_ctx c, uint32_t n_in, uint32_t cap_tri, uint32_t cap_pairs
   HIPCHK(c, c->tile_start.ensure(n_tiles));
   HIPCHK(c, c->vis.ensure(n_pixels));
   HIPCHK(c, c->status.ensure(2));  // opaque pass, transparent pass
-  HIPCHK(c, c->draws.ensure(n_draws));
-  HIPCHK(c, c->views.ensure(n_views));
+  {
+    const size_t need = tables_bytes(n_draws, n_views);
+    if (c->tables.n < need || !c->tables.p) {
+      HIPCHK(c, c->tables.ensure(need));
+    }
+    if (c->h_tables_n < need) {
+      if (c->h_tables) hipHostFree(c->h_tables);
+      c->h_tables = nullptr;
+      c->h_tables_n = 0;
+      HIPCHK(c, hipHostMalloc(reinterpret_cast<void**>(&c->h_tables), need));
+      c->h_tables_n = need;
+    }
+    c->tables_view_off = draw_bytes(n_draws);
+    c->draws = reinterpret_cast<dh::DevDraw*>(c->tables.p);
+    c->views = reinterpret_cast<dh::DevView*>(c->tables.p + c->tables_view_off);
+  }
   c->cap_tri = (uint32_t)std::min<size_t>(c->setup.n, 0xFFFFFFF0u);
   c->cap_pairs = (uint32_t)std::min<size_t>(c->keys0.n, 0xFFFFFFF0u);
   // binning keeps the wide-triangle list (up to cap_tri entries) in keys[1]
@@ -277,7 +303,10 @@ int ctx_new(int32_t device, std::shared_ptr<ResPool> pool, dh_ctx* out) {
   c->device = device;
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
-  if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&c->h_status), 2 * sizeof(dh::Status));
+  // the frame's last kernel writes the status words here (coherent: no copy, no cache flush)
+  if (e == hipSuccess)
+    e = hipHostMalloc(reinterpret_cast<void**>(&c->h_status), 2 * sizeof(dh::Status), hipHostMallocCoherent);
+  if (e == hipSuccess) e = hipHostGetDevicePointer(reinterpret_cast<void**>(&c->d_status_host), c->h_status, 0);
   for (int i = 0; i < 5 && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
   if (e == hipSuccess && !pool) {
     pool = std::make_shared<ResPool>();
@@ -333,10 +362,11 @@ extern "C" int dh_ctx_destroy(dh_ctx c) {
   c->keys0.release(); c->keys1.release(); c->vals0.release(); c->vals1.release();
   c->hist.release(); c->partials.release(); c->ranges.release(); c->vis.release();
   c->tile_cnt.release(); c->tile_start.release();
-  c->status.release(); c->draws.release(); c->views.release(); c->export_buf.release();
+  c->status.release(); c->tables.release(); c->draws = nullptr; c->views = nullptr; c->export_buf.release();
   c->tdraws.release(); c->grp.release(); c->tperm.release(); c->sort_n.release();
   c->gui_vtx.release(); c->gui_tri.release(); c->gui_clip.release(); c->gui_rec.release(); c->gui_err.release();
   if (c->h_status) hipHostFree(c->h_status);
+  if (c->h_tables) hipHostFree(c->h_tables);
   for (int i = 0; i < 5; i++)
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -701,10 +731,10 @@ dh::PipeArgs make_args(dh_ctx c, const dh::DevMaterial* mats, uint32_t n_in, uin
                        bool events) {
   dh::PipeArgs a;
   memset(&a, 0, sizeof(a));
-  a.draws = c->draws.p;
+  a.draws = c->draws;
   a.n_draws = (uint32_t)c->h_draws.size();
   a.n_tri_in = n_in;
-  a.views = c->views.p;
+  a.views = c->views;
   a.n_views = (uint32_t)c->h_views.size();
   a.n_tiles = n_tiles;
   a.n_pixels = n_pixels;
@@ -786,9 +816,10 @@ int render_views(dh_ctx c, const dh_view* views, uint32_t n_views, bool use_grap
     rc = ensure_scratch(c, (uint32_t)n_max, (uint32_t)want_tri, (uint32_t)want_pairs, n_tiles, n_pixels,
                         (uint32_t)c->h_draws.size(), (uint32_t)c->h_views.size());
     if (rc) return rc;
-    HIPCHK(c, hipMemcpyAsync(c->draws.p, c->h_draws.data(), c->h_draws.size() * sizeof(dh::DevDraw),
-                             hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->views.p, c->h_views.data(), c->h_views.size() * sizeof(dh::DevView),
+    // one upload of both tables from pinned memory (the previous frame has completed)
+    memcpy(c->h_tables, c->h_draws.data(), c->h_draws.size() * sizeof(dh::DevDraw));
+    memcpy(c->h_tables + c->tables_view_off, c->h_views.data(), c->h_views.size() * sizeof(dh::DevView));
+    HIPCHK(c, hipMemcpyAsync(c->tables.p, c->h_tables, c->tables_view_off + c->h_views.size() * sizeof(dh::DevView),
                              hipMemcpyHostToDevice, c->stream));
     const uint32_t n_groups = (uint32_t)c->h_grp_first.size();
     if (transparent) {
@@ -805,11 +836,12 @@ int render_views(dh_ctx c, const dh_view* views, uint32_t n_views, bool use_grap
     std::vector<uint64_t> gkey = key;
     const void* ptrs[] = {c->cnt.p, c->blk.p, c->surv.p, c->setup.p, c->shade.p, c->bincnt.p, c->binoff.p, c->big.p,
                           c->keys0.p, c->keys1.p, c->vals0.p, c->vals1.p, c->hist.p, c->partials.p, c->ranges.p,
-                          c->tile_cnt.p, c->tile_start.p, c->vis.p, c->status.p, c->draws.p, c->views.p};
+                          c->tile_cnt.p, c->tile_start.p, c->vis.p, c->status.p, c->draws, c->views};
     for (const void* p : ptrs) gkey.push_back((uint64_t)(uintptr_t)p);
     gkey.push_back((uint64_t)c->cap_tri << 32 | c->cap_pairs);
     gkey.push_back(n_in);
     const bool replay = use_graph && c->graph_exec && gkey == c->graph_key;
+    memset(c->h_status, 0, 2 * sizeof(dh::Status));
     if (replay) {
       if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[0], c->stream));
       HIPCHK(c, hipGraphLaunch(c->graph_exec, c->stream));
@@ -842,9 +874,9 @@ int render_views(dh_ctx c, const dh_view* views, uint32_t n_views, bool use_grap
         dh::enqueue_blend(tb, ta, c->stream);
         if (c->profiling) HIPCHK(c, hipEventRecord(c->ev[4], c->stream));
       }
+      dh::enqueue_status_out(c->status.p, c->d_status_host, 2, c->stream);
       HIPCHK(c, hipGetLastError());
     }
-    HIPCHK(c, hipMemcpyAsync(c->h_status, c->status.p, 2 * sizeof(dh::Status), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const dh::Status st = c->h_status[0];
     const dh::Status st2 = transparent ? c->h_status[1] : dh::Status{};
@@ -882,6 +914,7 @@ int render_views(dh_ctx c, const dh_view* views, uint32_t n_views, bool use_grap
       drop_graph(c);
       HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       dh::enqueue_frame(make_args(c, mats, (uint32_t)n_in, n_tiles, n_pixels, false), c->stream);
+      dh::enqueue_status_out(c->status.p, c->d_status_host, 2, c->stream);
       hipGraph_t g = nullptr;
       HIPCHK(c, hipStreamEndCapture(c->stream, &g));
       c->graph = g;
@@ -1047,7 +1080,7 @@ uint64_t ctx_scratch_bytes(dh_ctx c) {
   b += c->cnt.n + 4 * (c->blk.n + c->surv.n) + sizeof(SetupRec) * c->setup.n + sizeof(ShadeRec) * c->shade.n;
   b += 4 * (c->bincnt.n + c->binoff.n + c->big.n + c->keys0.n + c->keys1.n + c->vals0.n + c->vals1.n);
   b += 4 * (c->hist.n + c->partials.n + c->tile_cnt.n + c->tile_start.n + c->vis.n) + 8 * c->ranges.n;
-  b += sizeof(Status) * c->status.n + sizeof(DevDraw) * c->draws.n + sizeof(DevView) * c->views.n;
+  b += sizeof(Status) * c->status.n + c->tables.n;
   return b;
 }
 
@@ -1089,7 +1122,7 @@ void ctx_release_scratch(dh_ctx c) {
   c->keys0.release(); c->keys1.release(); c->vals0.release(); c->vals1.release();
   c->hist.release(); c->partials.release(); c->ranges.release(); c->vis.release();
   c->tile_cnt.release(); c->tile_start.release();
-  c->status.release(); c->draws.release(); c->views.release(); c->export_buf.release();
+  c->status.release(); c->tables.release(); c->draws = nullptr; c->views = nullptr; c->export_buf.release();
   c->tdraws.release(); c->grp.release(); c->tperm.release(); c->sort_n.release();
   c->gui_vtx.release(); c->gui_tri.release(); c->gui_clip.release(); c->gui_rec.release(); c->gui_err.release();
   c->cap_tri = c->cap_pairs = 0;

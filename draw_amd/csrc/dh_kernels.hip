@@ -1321,6 +1321,18 @@ void enqueue_transparent_prep(const PipeArgs& opaque, const TransArgs& t, hipStr
   }
 }
 
+// The frame's status words into coherent pinned host memory (one wave, in
+// stream order after every kernel that writes them).
+__global__ void k_status_out(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, uint32_t n_words) {
+  for (uint32_t i = threadIdx.x; i < n_words; i += blockDim.x) dst[i] = src[i];
+  __threadfence_system();
+}
+
+void enqueue_status_out(const Status* dev, Status* host_dev_ptr, uint32_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_status_out, dim3(1), dim3(64), 0, s, reinterpret_cast<const uint32_t*>(dev),
+                     reinterpret_cast<uint32_t*>(host_dev_ptr), (uint32_t)(n * sizeof(Status) / 4));
+}
+
 void enqueue_frame(const PipeArgs& a, hipStream_t s) {
   const PipeArgs b = enqueue_front(a, s);
   // raster, shade (dh_raster.hip)

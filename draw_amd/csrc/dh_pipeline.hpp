@@ -71,6 +71,10 @@ struct TransArgs {
 
 PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s);
 void enqueue_frame(const PipeArgs& a, hipStream_t s);
+// Copies n Status words from device memory to coherent pinned host memory
+// (host_dev_ptr: its device address) with a kernel, so that the copy can sit
+// inside a captured frame graph.
+void enqueue_status_out(const Status* dev, Status* host_dev_ptr, uint32_t n, hipStream_t s);
 int enqueue_radix_sort(uint32_t* const keys[2], uint32_t* const vals[2], const uint32_t* n_ptr, uint32_t cap,
                        int key_bits, uint32_t* hist, uint32_t* partials, hipStream_t s);
 void enqueue_transparent_prep(const PipeArgs& opaque, const TransArgs& t, hipStream_t s);
