@@ -2,24 +2,31 @@
 //
 // Pipeline (one stream, no host synchronisation inside a frame):
 //   1 k_geom_count   per input triangle: back-face cull, frustum clip count
-//                    (scene/mod.rs:1016-1033, 43-90, 641-746), positions only
-//   2 scan           block counts -> ordered output offsets (submission order)
-//   3 k_geom_emit    per input triangle: per-vertex lighting (917-926), clip
-//                    with attributes, projection (1047-1063), raster setup
-//                    (canvas.rs:585-666) -> SetupRec + ShadeRec at its
-//                    compacted index, and (ordered bins) the triangle's
-//                    conservative tile overlap count
-//   4 k_bin_count_wide  the counts of triangles spanning many tiles, a wave
-//                    each (k_bin_count counts per tile for unordered bins)
-//   5 scan           -> bin entry offsets
-//   6 k_bin_emit     (tile, triangle) pairs in triangle order
-//   7 radix sort     stable LSD sort of the pairs by tile -> per-tile lists
-//                    that keep the submission order
-//   8 k_tile_ranges  [start, end) of every tile's list
-//   9 k_raster       one workgroup per screen tile: coverage and the strict
+//                    (scene/mod.rs:1016-1033, 43-90, 641-746), positions only,
+//                    and whether it crosses the near or far plane
+//   2 scan, k_geom_compact  block counts -> ordered output offsets (submission
+//                    order) and two dense lists of emitting input triangles:
+//                    unclipped, clipped
+//   3 k_geom_emit    per listed triangle: per-vertex lighting (917-926), clip
+//                    with attributes (clipped list only), projection
+//                    (1047-1063), raster setup (canvas.rs:585-666) -> SetupRec +
+//                    ShadeRec at its output id; with ordered bins also the
+//                    conservative tile count, and the near-first depth key
+//   4 k_wide_list, k_bin_count_wide  triangles spanning many tiles: listed,
+//                    then counted a wave each (k_bin_count counts per tile for
+//                    unordered bins)
+//   5 near first     stable 8-bit radix pass of the triangles by depth key,
+//                    their counts gathered in that order
+//   6 scan           -> bin entry offsets
+//   7 k_bin_emit(_wide)  (tile, triangle) pairs in that order
+//   8 radix sort     stable LSD sort of the pairs by tile -> per-tile lists,
+//                    near first (or in submission order)
+//   9 k_tile_ranges  [start, end) of every tile's list
+//  10 k_raster       one workgroup per screen tile: coverage and the strict
 //                    depth test (canvas.rs:668-682, 923) into an LDS visibility
-//                    buffer of (depth key << 32 | triangle id)
-//  10 k_shade        deferred Phong + texture shading of each pixel's winner
+//                    buffer of (depth key << 32 | triangle id); the result does
+//                    not depend on list order
+//  11 k_shade        deferred Phong + texture shading of each pixel's winner
 //                    (canvas.rs:684-745), BGRA8 + f32 depth
 #include <math.h>
 #include <stdio.h>

@@ -53,7 +53,8 @@ __device__ inline uint32_t wave_max_u32(uint32_t v) {
   return v;
 }
 
-// Per-tile visibility.  The tile's bin list is in submission order; every
+// Per-tile visibility.  The tile's bin list is near first (or in submission
+// order, or unordered: the result is the same); every
 // covered fragment with depth < clear_depth contributes its key
 // (depth_key(d) << 32 | id) to an LDS atomicMin, which leaves at each pixel the
 // first fragment of minimum depth -- the winner of the reference's sequential
