@@ -357,9 +357,18 @@ __device__ inline void make_setup(const FrameParamsPod& fr, const FV& A, const F
 struct Spans {
   int cx0, cx1, cy0, cy1;
   bool exact;
-  double a[3], b[3], k[3], m[3];
+  float fa[3], fb[3], fk[3], fm[3];  // per-tile corner test (tile_may_cover)
+  double a[3], b[3], k[3], m[3];     // per-row spans of wide triangles (spans_row)
 };
 
+// Bounds and the per-tile test's constants.  The test evaluates
+// G_f = fl(fl(fl(a xc) + fl(b yc)) + fl(s (C - D))) at the tile corner
+// (xc, yc) maximising the edge's exact function G = a x + b y + s (C - D).
+// With M = |A| (x1 + TS) + |B| (y1 + TS) + |C| + |D|, which bounds the
+// magnitudes at every corner of a tile of the rectangle, |G_f - G| <= 3.02 u M
+// (u = 2^-24).  A covered pixel p of the tile has G >= g(p) >= -4.01 u M (the
+// pixel evaluation bound above), so G_f >= -7.03 u M; rejecting only when
+// G_f < -(1 + 9 u M) keeps every tile with a covered pixel.
 __device__ inline void spans_init(const SetupRec& r, int TS, Spans& g) {
   const uint32_t x0 = r.rx & 0xFFFFu, x1 = r.rx >> 16, y0 = r.ry & 0xFFFFu, y1 = r.ry >> 16;
   g.cx0 = (int)(x0 / TS); g.cx1 = (int)(x1 / TS);
@@ -369,6 +378,25 @@ __device__ inline void spans_init(const SetupRec& r, int TS, Spans& g) {
     g.exact = false;
     return;
   }
+  const float* es[3] = {r.ea, r.eb, r.eg};
+  const float xm = (float)x1 + (float)TS, ym = (float)y1 + (float)TS;
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    const float A = es[i][0], B = es[i][1], C = es[i][2], D = es[i][3];
+    const float s = r.f[i] > 0.0f ? 1.0f : -1.0f;
+    const float M = ((fabsf(A) * xm + fabsf(B) * ym) + fabsf(C)) + fabsf(D);
+    g.fa[i] = s * A;
+    g.fb[i] = s * B;
+    g.fk[i] = s * (C - D);
+    g.fm[i] = 1.0f + 5.4e-7f * M;  // 1 + 9 u M, rounded up
+    if (!isfinite(M)) g.exact = false;
+  }
+}
+
+// The per-row constants of a wide triangle (after spans_init), in double.
+__device__ inline void spans_init_rows(const SetupRec& r, Spans& g) {
+  if (!g.exact) return;
+  const uint32_t x1 = r.rx >> 16, y1 = r.ry >> 16;
   const float* es[3] = {r.ea, r.eb, r.eg};
 #pragma unroll
   for (int i = 0; i < 3; i++) {
@@ -427,17 +455,16 @@ __device__ inline bool bin_is_wide(const Spans& g) {
 
 __device__ inline uint64_t lanemask_lt_u64(int lane) { return lane == 0 ? 0ull : (~0ull >> (64 - lane)); }
 
-// Tile (tx, ty) may hold a covered pixel: every edge's exact function,
-// widened by m, reaches 0 at the tile corner maximising it.  Products of a
-// float and a pixel coordinate below 2^16 are exact in double.
+// Tile (tx, ty) may hold a covered pixel: every edge's function, widened by
+// fm (see spans_init), reaches 0 at the tile corner maximising it.
 __device__ inline bool tile_may_cover(const Spans& g, int TS, int tx, int ty) {
   if (!g.exact) return true;
-  const double xa = (double)(tx * TS), xb = (double)(tx * TS + TS - 1);
-  const double ya = (double)(ty * TS), yb = (double)(ty * TS + TS - 1);
+  const float xa = (float)(tx * TS), xb = (float)(tx * TS + TS - 1);
+  const float ya = (float)(ty * TS), yb = (float)(ty * TS + TS - 1);
 #pragma unroll
   for (int i = 0; i < 3; i++) {
-    const double v = (g.a[i] * (g.a[i] > 0.0 ? xb : xa) + g.b[i] * (g.b[i] > 0.0 ? yb : ya)) + g.k[i];
-    if (v < -g.m[i]) return false;
+    const float v = (g.fa[i] * (g.fa[i] > 0.0f ? xb : xa) + g.fb[i] * (g.fb[i] > 0.0f ? yb : ya)) + g.fk[i];
+    if (v < -g.fm[i]) return false;
   }
   return true;
 }
@@ -886,6 +913,7 @@ __global__ __launch_bounds__(256) void k_bin_count_wide(const SetupRec* __restri
     const DevView& vw = views[r.view];
     Spans g;
     spans_init(r, TS, g);
+    spans_init_rows(r, g);
     const uint32_t c = wave_tiles(g, TS, lane, s_lo[wave], s_off[wave], [&](uint32_t, int ty, int tx) {
       if (!ORDERED) atomicAdd(&tile_cnt[vw.tile_base + (uint32_t)ty * vw.ntx + (uint32_t)tx], 1u);
     });
@@ -964,6 +992,7 @@ __global__ __launch_bounds__(256) void k_bin_emit_wide(const SetupRec* __restric
     const DevView& vw = views[r.view];
     Spans g;
     spans_init(r, TS, g);
+    spans_init_rows(r, g);
     const uint32_t o = ORDERED ? binoff[i] : 0u;
     wave_tiles(g, TS, lane, s_lo[wave], s_off[wave], [&](uint32_t e, int ty, int tx) {
       const uint32_t key = vw.tile_base + (uint32_t)ty * vw.ntx + (uint32_t)tx;
