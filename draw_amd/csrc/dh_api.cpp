@@ -124,9 +124,11 @@ struct dh_ctx_s {
   // frame from the pinned h_tables): draws at 0, views at tables_view_off
   DevBuf<uint8_t> tables;
   size_t tables_view_off = 0;
+  size_t tables_used = 0;  // bytes of the current frame's tables
   dh::DevDraw* draws = nullptr;
   dh::DevView* views = nullptr;
-  uint8_t* h_tables = nullptr;  // pinned host staging of the same layout
+  uint8_t* h_tables = nullptr;  // coherent pinned host staging of the same layout
+  uint32_t* d_tables_host = nullptr;  // its device address (read by the frame's first kernel)
   size_t h_tables_n = 0;
   dh::Status* d_status_host = nullptr;  // device address of h_status
   DevBuf<uint32_t> export_buf;  // RGBA staging of dh_canvas_read_rgba
@@ -208,7 +210,9 @@ dh::DevMaterial default_material(const uint8_t* white) {
   return m;
 }
 
-// The draw/view table layout: draws, then views at a 256-byte boundary.
+// The draw/view table layout: draws, then views at a 256-byte boundary
+// (copied as 4-byte words by the frame's first kernel).
+static_assert(sizeof(dh::DevView) % 4 == 0, "view table copied in words");
 size_t draw_bytes(uint64_t n_draws) { return (std::max<uint64_t>(n_draws, 1) * sizeof(dh::DevDraw) + 255) & ~(size_t)255; }
 size_t tables_bytes(uint64_t n_draws, uint64_t n_views) {
   return draw_bytes(n_draws) + std::max<uint64_t>(n_views, 1) * sizeof(dh::DevView);
@@ -259,7 +263,8 @@ int ensure_scratch(dh_ctx c, uint32_t n_in, uint32_t cap_tri, uint32_t cap_pairs
       if (c->h_tables) hipHostFree(c->h_tables);
       c->h_tables = nullptr;
       c->h_tables_n = 0;
-      HIPCHK(c, hipHostMalloc(reinterpret_cast<void**>(&c->h_tables), need));
+      HIPCHK(c, hipHostMalloc(reinterpret_cast<void**>(&c->h_tables), need, hipHostMallocCoherent));
+      HIPCHK(c, hipHostGetDevicePointer(reinterpret_cast<void**>(&c->d_tables_host), c->h_tables, 0));
       c->h_tables_n = need;
     }
     c->tables_view_off = draw_bytes(n_draws);
@@ -733,6 +738,9 @@ dh::PipeArgs make_args(dh_ctx c, const dh::DevMaterial* mats, uint32_t n_in, uin
   memset(&a, 0, sizeof(a));
   a.draws = c->draws;
   a.n_draws = (uint32_t)c->h_draws.size();
+  a.tables_src = c->d_tables_host;
+  a.tables_dst = reinterpret_cast<uint32_t*>(c->tables.p);
+  a.tables_words = (uint32_t)(c->tables_used / 4);
   a.n_tri_in = n_in;
   a.views = c->views;
   a.n_views = (uint32_t)c->h_views.size();
@@ -816,11 +824,11 @@ int render_views(dh_ctx c, const dh_view* views, uint32_t n_views, bool use_grap
     rc = ensure_scratch(c, (uint32_t)n_max, (uint32_t)want_tri, (uint32_t)want_pairs, n_tiles, n_pixels,
                         (uint32_t)c->h_draws.size(), (uint32_t)c->h_views.size());
     if (rc) return rc;
-    // one upload of both tables from pinned memory (the previous frame has completed)
+    // both tables into the pinned staging (the previous frame has completed);
+    // the frame's first kernel copies them to the device
     memcpy(c->h_tables, c->h_draws.data(), c->h_draws.size() * sizeof(dh::DevDraw));
     memcpy(c->h_tables + c->tables_view_off, c->h_views.data(), c->h_views.size() * sizeof(dh::DevView));
-    HIPCHK(c, hipMemcpyAsync(c->tables.p, c->h_tables, c->tables_view_off + c->h_views.size() * sizeof(dh::DevView),
-                             hipMemcpyHostToDevice, c->stream));
+    c->tables_used = c->tables_view_off + c->h_views.size() * sizeof(dh::DevView);
     const uint32_t n_groups = (uint32_t)c->h_grp_first.size();
     if (transparent) {
       HIPCHK(c, c->tdraws.ensure(c->h_tdraws.size()));
@@ -836,7 +844,8 @@ int render_views(dh_ctx c, const dh_view* views, uint32_t n_views, bool use_grap
     std::vector<uint64_t> gkey = key;
     const void* ptrs[] = {c->cnt.p, c->blk.p, c->surv.p, c->setup.p, c->shade.p, c->bincnt.p, c->binoff.p, c->big.p,
                           c->keys0.p, c->keys1.p, c->vals0.p, c->vals1.p, c->hist.p, c->partials.p, c->ranges.p,
-                          c->tile_cnt.p, c->tile_start.p, c->vis.p, c->status.p, c->draws, c->views};
+                          c->tile_cnt.p, c->tile_start.p, c->vis.p, c->status.p, c->draws, c->views,
+                          c->d_tables_host};
     for (const void* p : ptrs) gkey.push_back((uint64_t)(uintptr_t)p);
     gkey.push_back((uint64_t)c->cap_tri << 32 | c->cap_pairs);
     gkey.push_back(n_in);
@@ -865,6 +874,7 @@ int render_views(dh_ctx c, const dh_view* views, uint32_t n_views, bool use_grap
         ta.n_groups = n_groups;
         ta.pass = make_args(c, mats, (uint32_t)c->n_tin, n_tiles, n_pixels, false);
         ta.pass.draws = c->tdraws.p;
+        ta.pass.tables_src = nullptr;  // copied by the opaque pass
         ta.pass.n_draws = (uint32_t)c->h_tdraws.size();
         ta.pass.status = c->status.p + 1;
         ta.pass.ordered_bins = true;  // blending needs every tile list in submission order

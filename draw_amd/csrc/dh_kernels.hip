@@ -1115,6 +1115,15 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
   }
 }
 
+// A pass's first kernel: its Status words cleared and (opaque pass) the
+// draw/view tables copied in from pinned host memory.
+__global__ __launch_bounds__(256) void k_frame_begin(uint32_t* __restrict__ status, uint32_t status_words,
+                                                     const uint32_t* __restrict__ src, uint32_t* __restrict__ dst,
+                                                     uint32_t n_words) {
+  if (blockIdx.x == 0 && threadIdx.x < status_words) status[threadIdx.x] = 0u;
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n_words; i += gridDim.x * 256u) dst[i] = src[i];
+}
+
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint32_t n) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = v;
 }
@@ -1219,8 +1228,9 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
   // wide-triangle list), which needs cap_pairs >= 3 cap_tri
   const bool depth_sorted = a.ordered_bins && a.depth_order && (uint64_t)a.cap_pairs >= 3ull * a.cap_tri;
   if (a.ev[0]) (void)hipEventRecord(a.ev[0], s);
-  hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, s, reinterpret_cast<uint32_t*>(a.status), 0u,
-                     (uint32_t)(sizeof(Status) / 4));
+  hipLaunchKernelGGL(k_frame_begin, dim3(a.tables_src ? std::min<uint32_t>(cdiv(a.tables_words, 1024), 64u) : 1u),
+                     dim3(256), 0, s, reinterpret_cast<uint32_t*>(a.status), (uint32_t)(sizeof(Status) / 4),
+                     a.tables_src, a.tables_dst, a.tables_src ? a.tables_words : 0u);
   const uint32_t n_blk = cdiv(a.n_tri_in, 256);
   // 1-3 geometry with ordered compaction
   if (n_blk) {

@@ -46,6 +46,11 @@ struct PipeArgs {
   uint2* ranges;       // [n_tiles]
   uint32_t* vis;       // [n_pixels] winner id per pixel
   Status* status;
+  // the frame's draw/view tables, copied by the frame's first kernel from
+  // coherent pinned host memory (tables_src: its device address; null: none)
+  const uint32_t* tables_src;
+  uint32_t* tables_dst;
+  uint32_t tables_words;
   hipEvent_t ev[5];    // optional stage events (all null when not profiling)
 };
 
