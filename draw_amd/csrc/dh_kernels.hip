@@ -706,15 +706,21 @@ __device__ inline uint32_t scan_count(const uint32_t* n_ptr, uint32_t n_const, u
 }
 __device__ inline uint32_t scan_chunk(uint32_t n) { return (((n + kScanG - 1u) / kScanG) + 1023u) & ~1023u; }
 
+// The scans' input: in[i], or in[idx[i]] (a gather fused into the scan).
+__device__ inline uint32_t scan_in(const uint32_t* in, const uint32_t* idx, uint32_t i) {
+  return idx ? in[idx[i]] : in[i];
+}
+
 __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict__ in, const uint32_t* n_ptr,
-                                                     uint32_t n_const, uint32_t cap, uint32_t* __restrict__ partials) {
+                                                     uint32_t n_const, uint32_t cap, uint32_t* __restrict__ partials,
+                                                     const uint32_t* __restrict__ idx) {
   __shared__ uint32_t s_tmp[4];
   const uint32_t n = scan_count(n_ptr, n_const, cap);
   const uint32_t C = scan_chunk(n);
   const uint32_t b0 = blockIdx.x * C;
   const uint32_t b1 = min(b0 + C, n);
   uint32_t s = 0;
-  for (uint32_t i = b0 + threadIdx.x; i < b1; i += 256) s += in[i];
+  for (uint32_t i = b0 + threadIdx.x; i < b1; i += 256) s += scan_in(in, idx, i);
   uint32_t total;
   block_exclusive_scan256(s, s_tmp, &total);
   if (threadIdx.x == 0) partials[blockIdx.x] = total;
@@ -727,7 +733,7 @@ __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__
                                                     const uint32_t* n_ptr, uint32_t n_const, uint32_t cap,
                                                     const uint32_t* __restrict__ partials, uint32_t* total_out,
                                                     uint32_t cap_check, uint32_t* overflow_word,
-                                                    uint32_t overflow_bit) {
+                                                    uint32_t overflow_bit, const uint32_t* __restrict__ idx) {
   __shared__ uint32_t s_tmp[4];
   const uint32_t n = scan_count(n_ptr, n_const, cap);
   const uint32_t C = scan_chunk(n);
@@ -751,7 +757,7 @@ __global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__
     const uint32_t i0 = tb + threadIdx.x * 4u;
     uint32_t v[4];
 #pragma unroll
-    for (int k = 0; k < 4; k++) v[k] = (i0 + k < b1) ? in[i0 + k] : 0u;
+    for (int k = 0; k < 4; k++) v[k] = (i0 + k < b1) ? scan_in(in, idx, i0 + k) : 0u;
     const uint32_t s4 = v[0] + v[1] + v[2] + v[3];
     uint32_t total;
     uint32_t ex = block_exclusive_scan256(s4, s_tmp, &total) + running;
@@ -816,18 +822,19 @@ __global__ __launch_bounds__(1024) void k_scan_small(ScanSegs segs, uint32_t n) 
   }
 }
 
+// out = exclusive scan of in (or of in[idx[i]] when idx is given).
 static void scan_u32(const uint32_t* in, uint32_t* out, const uint32_t* n_ptr, uint32_t n_const, uint32_t cap,
                      uint32_t* partials, uint32_t* total_out, uint32_t cap_check, uint32_t* overflow_word,
-                     uint32_t overflow_bit, hipStream_t s) {
-  if (!n_ptr && n_const <= 4096u) {
+                     uint32_t overflow_bit, hipStream_t s, const uint32_t* idx = nullptr) {
+  if (!n_ptr && n_const <= 4096u && !idx) {
     ScanSegs g{};
     g.s[0] = ScanSeg{in, out, total_out, overflow_word, cap_check, overflow_bit};
     hipLaunchKernelGGL(k_scan_small, dim3(1), dim3(1024), 0, s, g, n_const);
     return;
   }
-  hipLaunchKernelGGL(k_scan_reduce, dim3(kScanG), dim3(256), 0, s, in, n_ptr, n_const, cap, partials);
+  hipLaunchKernelGGL(k_scan_reduce, dim3(kScanG), dim3(256), 0, s, in, n_ptr, n_const, cap, partials, idx);
   hipLaunchKernelGGL(k_scan_apply, dim3(kScanG), dim3(256), 0, s, in, out, n_ptr, n_const, cap, partials, total_out,
-                     cap_check, overflow_word, overflow_bit);
+                     cap_check, overflow_word, overflow_bit, idx);
 }
 
 // Pass 1 of unordered bins, one thread per clipped triangle: flag wide
@@ -861,9 +868,12 @@ __global__ __launch_bounds__(256) void k_bin_count(const SetupRec* __restrict__ 
 // bin entries go to its own scanned offset (or through per-tile atomics for
 // unordered bins).
 __global__ __launch_bounds__(256) void k_wide_list(const uint8_t* __restrict__ wflag, Status* status,
-                                                   uint32_t cap_tri, uint32_t* __restrict__ list) {
+                                                   uint32_t cap_tri, uint32_t* __restrict__ list,
+                                                   uint32_t* __restrict__ zero_dst, uint32_t zero_words) {
   __shared__ uint32_t s_tmp[4];
   __shared__ uint32_t s_base;
+  // also clears the tile ranges that k_tile_ranges fills later
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < zero_words; i += gridDim.x * 256u) zero_dst[i] = 0u;
   const uint32_t n = min(status->n_out, cap_tri);
   for (uint32_t c0 = blockIdx.x * 4096u; c0 < n; c0 += gridDim.x * 4096u) {
     const uint32_t t0 = c0 + threadIdx.x * 16u;
@@ -969,12 +979,6 @@ __global__ __launch_bounds__(256) void k_bin_emit(const SetupRec* __restrict__ s
   }
 }
 
-// out[j] = src[idx[j]] for j < min(*n_ptr, cap).
-__global__ __launch_bounds__(256) void k_gather_u32(const uint32_t* __restrict__ idx, const uint32_t* __restrict__ src,
-                                                    const uint32_t* n_ptr, uint32_t cap, uint32_t* __restrict__ out) {
-  const uint32_t n = min(*n_ptr, cap);
-  for (uint32_t j = blockIdx.x * 256u + threadIdx.x; j < n; j += gridDim.x * 256u) out[j] = src[idx[j]];
-}
 
 template <bool ORDERED>
 __global__ __launch_bounds__(256) void k_bin_emit_wide(const SetupRec* __restrict__ setup,
@@ -1285,7 +1289,8 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
   const uint32_t gw = std::min<uint32_t>(cdiv(a.cap_tri, 64), 4096u);
   PipeArgs b = a;
   if (a.ordered_bins) {
-    hipLaunchKernelGGL(k_wide_list, dim3(gl), dim3(256), 0, s, wflag, a.status, a.cap_tri, wide_list);
+    hipLaunchKernelGGL(k_wide_list, dim3(gl), dim3(256), 0, s, wflag, a.status, a.cap_tri, wide_list,
+                       reinterpret_cast<uint32_t*>(a.ranges), 2u * a.n_tiles);
     hipLaunchKernelGGL(k_bin_count_wide<true>, dim3(gw), dim3(256), 0, s, a.setup, a.views, a.status, a.TS,
                        wide_list, a.bincnt, a.tile_cnt);
     if (depth_sorted) {
@@ -1298,10 +1303,8 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
                                         a.partials, s);
       uint32_t* order = dvals[dc];
       uint32_t* sorted_off = dkeys[dc];
-      hipLaunchKernelGGL(k_gather_u32, dim3(gb), dim3(256), 0, s, order, a.bincnt, &a.status->n_out, a.cap_tri,
-                         sorted_off);
-      scan_u32(sorted_off, sorted_off, &a.status->n_out, 0, a.cap_tri, a.partials, &a.status->n_pairs, a.cap_pairs,
-               &a.status->overflow, 2u, s);
+      scan_u32(a.bincnt, sorted_off, &a.status->n_out, 0, a.cap_tri, a.partials, &a.status->n_pairs, a.cap_pairs,
+               &a.status->overflow, 2u, s, order);
       hipLaunchKernelGGL((k_bin_emit<true, true>), dim3(gb), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri,
                          a.TS, wflag, a.binoff, order, sorted_off, a.tile_start, a.tile_cur, a.cap_pairs, a.keys[0],
                          a.vals[0]);
@@ -1319,8 +1322,6 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
     const int cur = enqueue_radix_sort(a.keys, a.vals, &a.status->n_pairs, a.cap_pairs, a.key_bits, a.hist,
                                        a.partials, s);
     b.vals[0] = a.vals[cur];
-    hipLaunchKernelGGL(k_fill_u32, dim3(std::min<uint32_t>(cdiv(2ull * a.n_tiles, 256), 2048u)), dim3(256), 0, s,
-                       reinterpret_cast<uint32_t*>(a.ranges), 0u, 2u * a.n_tiles);
     hipLaunchKernelGGL(k_tile_ranges, dim3(std::min<uint32_t>(cdiv(a.cap_pairs, 256), 4096u)), dim3(256), 0, s,
                        a.keys[cur], &a.status->n_pairs, a.cap_pairs, a.ranges);
   } else {
@@ -1328,7 +1329,7 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
                        a.tile_cnt, 0u, 2u * a.n_tiles);  // tile_cnt and tile_cur are adjacent
     hipLaunchKernelGGL(k_bin_count, dim3(gb), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri, a.TS,
                        a.tile_cnt, wflag);
-    hipLaunchKernelGGL(k_wide_list, dim3(gl), dim3(256), 0, s, wflag, a.status, a.cap_tri, wide_list);
+    hipLaunchKernelGGL(k_wide_list, dim3(gl), dim3(256), 0, s, wflag, a.status, a.cap_tri, wide_list, nullptr, 0u);
     hipLaunchKernelGGL(k_bin_count_wide<false>, dim3(gw), dim3(256), 0, s, a.setup, a.views, a.status, a.TS,
                        wide_list, a.bincnt, a.tile_cnt);
     scan_u32(a.tile_cnt, a.tile_start, nullptr, a.n_tiles, a.n_tiles, a.partials, &a.status->n_pairs, a.cap_pairs,
