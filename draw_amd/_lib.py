@@ -115,6 +115,13 @@ SIGNATURES = {
     "dh_set_profiling": [_P, C.c_int32],
     "dh_set_bin_order": [_P, C.c_int32],
     "dh_set_graph": [_P, C.c_int32],
+    "dh_render_shard": [_P, C.POINTER(DhFrameParams), C.POINTER(C.c_uint64), C.c_uint32, C.c_uint32, C.c_uint64,
+                        C.c_void_p],
+    "dh_vis_merge_min": [_P, C.c_void_p, C.c_void_p, C.c_uint64],
+    "dh_shade_shard": [_P, C.c_void_p, C.c_uint32],
+    "dh_device_alloc": [_P, C.c_uint64, C.POINTER(C.c_void_p)],
+    "dh_device_free": [_P, C.c_void_p],
+    "dh_device_read": [_P, C.c_void_p, C.c_void_p, C.c_uint64],
     "dh_set_raster_kernel": [_P, C.c_int32, C.c_int32, C.c_int32],
     "dh_texture_create": [_P, _P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(_H)],
     "dh_material_create": [_P, C.POINTER(DhMaterialDesc), C.POINTER(_H)],

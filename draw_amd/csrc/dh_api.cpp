@@ -125,6 +125,13 @@ struct dh_ctx_s {
   DevBuf<uint8_t> tables;
   size_t tables_view_off = 0;
   size_t tables_used = 0;  // bytes of the current frame's tables
+  // sort-last shard frames: dh_render_shard sets shard_out for its call; the
+  // frame it leaves (records, tables) is what dh_shade_shard shades
+  uint64_t* shard_out = nullptr;
+  uint32_t shard = 0;
+  bool shard_frame = false;
+  uint64_t shard_n_in = 0;
+  uint32_t shard_n_tiles = 0, shard_n_pixels = 0;
   dh::DevDraw* draws = nullptr;
   dh::DevView* views = nullptr;
   uint8_t* h_tables = nullptr;  // coherent pinned host staging of the same layout
@@ -803,7 +810,8 @@ int render_views(dh_ctx c, const dh_view* views, uint32_t n_views, bool use_grap
     mats = c->res->mat_dev.p;
   }
   const bool transparent = !c->h_tdraws.empty();
-  if (transparent) use_graph = false;
+  if (transparent || c->shard_out) use_graph = false;
+  c->shard_frame = false;
   const uint64_t n_max = std::max<uint64_t>(n_in, c->n_tin);
   key.push_back((uint64_t)(uintptr_t)mats);
   uint64_t want_tri = std::max<uint64_t>(c->cap_tri, n_max + n_max / 4 + 1024);
@@ -858,7 +866,10 @@ int render_views(dh_ctx c, const dh_view* views, uint32_t n_views, bool use_grap
     } else {
       dh::PipeArgs oa = make_args(c, mats, (uint32_t)n_in, n_tiles, n_pixels, c->profiling);
       if (transparent) oa.ev[4] = nullptr;  // the frame ends after the transparent pass
-      dh::enqueue_frame(oa, c->stream);
+      if (c->shard_out)
+        dh::enqueue_visibility(oa, c->shard, c->shard_out, c->stream);
+      else
+        dh::enqueue_frame(oa, c->stream);
       if (transparent) {
         // transparent pass (scene/mod.rs:1088-1247): sort, geometry without
         // culling, ordered binning, per-tile blend
@@ -931,6 +942,14 @@ int render_views(dh_ctx c, const dh_view* views, uint32_t n_views, bool use_grap
       HIPCHK(c, hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0));
       c->graph_key = gkey;
     }
+    if (c->shard_out) {
+      if (st.n_out >= (1u << dh::kShardShift))
+        return fail(c, DH_E_CAPACITY, "a shard emits at most 2^29 clipped triangles");
+      c->shard_frame = true;
+      c->shard_n_in = n_in;
+      c->shard_n_tiles = n_tiles;
+      c->shard_n_pixels = n_pixels;
+    }
     if ((st.error | st2.error) & 1u)
       return fail(c, DH_E_TEXCOORD, "texture coordinate out of range (get_rgb_slice would panic)");
     return DH_OK;
@@ -950,6 +969,77 @@ extern "C" int dh_render(dh_ctx c, const dh_frame_params* fr, const dh_handle* m
   v.meshes = meshes;
   v.n_meshes = n_meshes;
   return render_views(c, &v, 1, c->graph_frames, true);
+}
+
+extern "C" int dh_device_alloc(dh_ctx c, uint64_t bytes, void** out_dev) {
+  if (!c || !out_dev) return DH_E_ARG;
+  *out_dev = nullptr;
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, hipMalloc(out_dev, bytes ? bytes : 1));
+  return DH_OK;
+}
+
+extern "C" int dh_device_free(dh_ctx c, void* dev) {
+  if (!c) return DH_E_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (dev) HIPCHK(c, hipFree(dev));
+  return DH_OK;
+}
+
+extern "C" int dh_device_read(dh_ctx c, void* host, const void* dev, uint64_t bytes) {
+  if (!c || (bytes && (!host || !dev))) return DH_E_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (bytes) HIPCHK(c, hipMemcpyAsync(host, dev, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return DH_OK;
+}
+
+extern "C" int dh_render_shard(dh_ctx c, const dh_frame_params* fr, const dh_handle* meshes, uint32_t n_meshes,
+                               uint32_t shard, dh_handle canvas, uint64_t* vis_dev) {
+  if (!c || !fr || (n_meshes && !meshes) || !vis_dev) return DH_E_ARG;
+  if (shard >= dh::kShardMax) return fail(c, DH_E_ARG, "shard index must be below 8");
+  dh_view v;
+  memset(&v, 0, sizeof(v));
+  v.fr = *fr;
+  v.canvas = canvas;
+  v.meshes = meshes;
+  v.n_meshes = n_meshes;
+  c->shard_out = vis_dev;
+  c->shard = shard;
+  const int rc = render_views(c, &v, 1, false, false);
+  c->shard_out = nullptr;
+  return rc;
+}
+
+extern "C" int dh_vis_merge_min(dh_ctx c, uint64_t* acc_dev, const uint64_t* src_dev, uint64_t n) {
+  if (!c || (n && (!acc_dev || !src_dev))) return DH_E_ARG;
+  HIPCHK(c, hipSetDevice(c->device));
+  if (n) dh::enqueue_vis_min(acc_dev, src_dev, n, c->stream);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return DH_OK;
+}
+
+extern "C" int dh_shade_shard(dh_ctx c, const uint64_t* merged_dev, uint32_t shard) {
+  if (!c || !merged_dev) return DH_E_ARG;
+  if (!c->shard_frame || shard != c->shard)
+    return fail(c, DH_E_ARG, "dh_shade_shard needs this shard's dh_render_shard frame on the same context");
+  HIPCHK(c, hipSetDevice(c->device));
+  dh::DevMaterial* mats = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(c->res->mu);
+    mats = c->res->mat_dev.p;
+  }
+  dh::PipeArgs a = make_args(c, mats, (uint32_t)c->shard_n_in, c->shard_n_tiles, c->shard_n_pixels, false);
+  HIPCHK(c, hipMemsetAsync(c->status.p, 0, sizeof(dh::Status), c->stream));
+  dh::enqueue_vis_unpack(merged_dev, c->shard_n_pixels, shard, c->vis.p, c->stream);
+  dh::enqueue_shade(a, c->stream);
+  dh::enqueue_status_out(c->status.p, c->d_status_host, 2, c->stream);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (c->h_status[0].error & 1u)
+    return fail(c, DH_E_TEXCOORD, "texture coordinate out of range (get_rgb_slice would panic)");
+  return DH_OK;
 }
 
 extern "C" int dh_render_batch(dh_ctx c, const dh_view* views, uint32_t n_views, uint32_t flags) {

@@ -16,6 +16,9 @@ typedef dh_frame_params FrameParamsPod;
 namespace dh {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr uint32_t kSkip = 0xFFFFFFFEu;  // a pixel another shard shades (dh_shade_shard)
+constexpr int kShardShift = 29;          // packed visibility id: shard << 29 | clipped-triangle id
+constexpr uint32_t kShardMax = 8;
 constexpr float kEps = 0.0000001f;  // linalg.rs:6
 
 // Setup record flags.

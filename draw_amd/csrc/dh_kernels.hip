@@ -1380,6 +1380,16 @@ void enqueue_status_out(const Status* dev, Status* host_dev_ptr, uint32_t n, hip
                      reinterpret_cast<uint32_t*>(host_dev_ptr), (uint32_t)(n * sizeof(Status) / 4));
 }
 
+// One shard of a sort-last frame: geometry, binning, raster, then the packed
+// u64 visibility (no shading: dh_shade_shard shades after the merge).
+void enqueue_visibility(const PipeArgs& a, uint32_t shard, uint64_t* out, hipStream_t s) {
+  const PipeArgs b = enqueue_front(a, s);
+  enqueue_raster(b, s);
+  if (a.ev[3]) (void)hipEventRecord(a.ev[3], s);
+  enqueue_vis_pack(a, shard, out, s);
+  if (a.ev[4]) (void)hipEventRecord(a.ev[4], s);
+}
+
 void enqueue_frame(const PipeArgs& a, hipStream_t s) {
   const PipeArgs b = enqueue_front(a, s);
   // raster, shade (dh_raster.hip)

@@ -76,6 +76,7 @@ struct TransArgs {
 
 PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s);
 void enqueue_frame(const PipeArgs& a, hipStream_t s);
+void enqueue_visibility(const PipeArgs& a, uint32_t shard, uint64_t* out, hipStream_t s);
 // Copies n Status words from device memory to coherent pinned host memory
 // (host_dev_ptr: its device address) with a kernel, so that the copy can sit
 // inside a captured frame graph.
@@ -88,5 +89,9 @@ bool raster_config_supported(int TS, int waves, int kind);
 // Per-pixel stages (dh_raster.hip): a.vals[0] must hold the sorted bin values.
 void enqueue_raster(const PipeArgs& a, hipStream_t s);
 void enqueue_shade(const PipeArgs& a, hipStream_t s);
+// Sort-last shards (dh_render_shard / dh_shade_shard / dh_vis_merge_min).
+void enqueue_vis_pack(const PipeArgs& a, uint32_t shard, uint64_t* out, hipStream_t s);
+void enqueue_vis_unpack(const uint64_t* merged, uint32_t n_pixels, uint32_t shard, uint32_t* vis, hipStream_t s);
+void enqueue_vis_min(uint64_t* acc, const uint64_t* src, uint64_t n, hipStream_t s);
 
 }  // namespace dh

@@ -536,6 +536,7 @@ __global__ __launch_bounds__(256) void k_shade(const DevView* __restrict__ views
   const uint32_t W = (uint32_t)vw.fr.W, H = (uint32_t)vw.fr.H;
   const uint32_t y = p / W, x = p - y * W;
   const uint32_t id = vis[g];
+  if (id == kSkip) return;  // shaded by another shard
   uint32_t bgra;
   float depth;
   if (id == kNone) {
@@ -737,6 +738,70 @@ void enqueue_blend(const PipeArgs& b, const TransArgs& t, hipStream_t s) {
   else
     hipLaunchKernelGGL(k_blend<32>, g, blk, 0, s, b.views, b.ranges, b.vals[0], b.setup, b.shade, b.mats,
                        t.grp_base, t.n_groups, b.status);
+}
+
+// Sort-last shards.  k_vis_pack writes a shard's full-screen visibility as
+// u64 (depth key << 32 | shard << 29 | clipped-triangle id): the same key the
+// raster's atomicMin kept (bary and depth_key at the winner's pixel), with
+// ids ordered by (shard, submission) -- so the u64 minimum over the shards
+// of a frame split by submission range is the single-GPU winner, ties
+// included.  No winner: ~0.
+__global__ __launch_bounds__(256) void k_vis_pack(const DevView* __restrict__ views, uint32_t n_pixels,
+                                                  const uint32_t* __restrict__ vis, const SetupRec* __restrict__ setup,
+                                                  uint32_t tag, unsigned long long* __restrict__ out) {
+  const uint32_t g = blockIdx.x * 256u + threadIdx.x;
+  if (g >= n_pixels) return;
+  const DevView& vw = views[0];
+  const uint32_t W = (uint32_t)vw.fr.W;
+  const uint32_t y = g / W, x = g - y * W;
+  const uint32_t id = vis[g];
+  unsigned long long k = ~0ull;
+  if (id != kNone) {
+    const FragOut o = bary(setup[id], (float)x, (float)y);
+    k = ((unsigned long long)depth_key(o.d) << 32) | (tag | id);
+  }
+  out[g] = k;
+}
+
+// The merged visibility back to this shard's winner ids: its own pixels keep
+// their id, pixels without a winner go to shard 0 (kNone: the clear colour),
+// the rest are kSkip.
+__global__ __launch_bounds__(256) void k_vis_unpack(const unsigned long long* __restrict__ merged, uint32_t n_pixels,
+                                                    uint32_t shard, uint32_t* __restrict__ vis) {
+  const uint32_t g = blockIdx.x * 256u + threadIdx.x;
+  if (g >= n_pixels) return;
+  const unsigned long long k = merged[g];
+  uint32_t v;
+  if (k == ~0ull) {
+    v = shard == 0 ? kNone : kSkip;
+  } else {
+    const uint32_t lo = (uint32_t)k;
+    v = (lo >> kShardShift) == shard ? (lo & ((1u << kShardShift) - 1u)) : kSkip;
+  }
+  vis[g] = v;
+}
+
+__global__ __launch_bounds__(256) void k_vis_min(unsigned long long* __restrict__ acc,
+                                                 const unsigned long long* __restrict__ src, uint64_t n) {
+  for (uint64_t i = blockIdx.x * 256ull + threadIdx.x; i < n; i += (uint64_t)gridDim.x * 256ull)
+    acc[i] = min(acc[i], src[i]);
+}
+
+void enqueue_vis_pack(const PipeArgs& a, uint32_t shard, uint64_t* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_vis_pack, dim3(cdiv(a.n_pixels, 256)), dim3(256), 0, s, a.views, a.n_pixels, a.vis, a.setup,
+                     shard << kShardShift, reinterpret_cast<unsigned long long*>(out));
+}
+
+void enqueue_vis_unpack(const uint64_t* merged, uint32_t n_pixels, uint32_t shard, uint32_t* vis, hipStream_t s) {
+  hipLaunchKernelGGL(k_vis_unpack, dim3(cdiv(n_pixels, 256)), dim3(256), 0, s,
+                     reinterpret_cast<const unsigned long long*>(merged), n_pixels, shard, vis);
+}
+
+void enqueue_vis_min(uint64_t* acc, const uint64_t* src, uint64_t n, hipStream_t s) {
+  const uint64_t nb = (n + 255) / 256;
+  const uint32_t blocks = nb > 8192u ? 8192u : (nb ? (uint32_t)nb : 1u);
+  hipLaunchKernelGGL(k_vis_min, dim3(blocks), dim3(256), 0, s,
+                     reinterpret_cast<unsigned long long*>(acc), reinterpret_cast<const unsigned long long*>(src), n);
 }
 
 void enqueue_shade(const PipeArgs& a, hipStream_t s) {

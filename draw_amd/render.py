@@ -81,6 +81,46 @@ class Renderer:
         mode = int(mode) if not isinstance(mode, bool) else int(mode)
         self._check(self.L.dh_set_bin_order(self.ctx, mode), "dh_set_bin_order")
 
+    def shard_context(self) -> "Renderer":
+        """Another context (own stream and scratch) on this one's resources
+        (dh_ctx_create_shared): one per shard of a sort-last frame."""
+        out = C.c_void_p()
+        self._check(self.L.dh_ctx_create_shared(self.ctx, C.byref(out)), "dh_ctx_create_shared")
+        child = Renderer.__new__(Renderer)
+        child.L = self.L
+        child.ctx = out
+        child._tex, child._mat, child._obj, child._canvas, child._keep = (
+            self._tex, self._mat, self._obj, self._canvas, self._keep)
+        return child
+
+    # ------------------------------------------------- sort-last shards
+    def device_alloc(self, nbytes: int) -> int:
+        p = C.c_void_p()
+        self._check(self.L.dh_device_alloc(self.ctx, int(nbytes), C.byref(p)), "dh_device_alloc")
+        return p.value
+
+    def device_free(self, dev: int):
+        self._check(self.L.dh_device_free(self.ctx, C.c_void_p(dev)), "dh_device_free")
+
+    def device_read(self, dev: int, nbytes: int, dtype=np.uint8) -> np.ndarray:
+        out = np.empty(int(nbytes) // np.dtype(dtype).itemsize, dtype)
+        self._check(self.L.dh_device_read(self.ctx, out.ctypes.data, C.c_void_p(dev), int(nbytes)), "dh_device_read")
+        return out
+
+    def render_shard(self, frame: DhFrameParams, meshes: List[int], shard: int, canvas: int, vis_dev: int):
+        """dh_render_shard: this shard's submission range into the packed u64
+        visibility buffer vis_dev (W*H entries)."""
+        arr = (C.c_uint64 * max(1, len(meshes)))(*meshes)
+        self._check(self.L.dh_render_shard(self.ctx, C.byref(frame), arr, len(meshes), int(shard), canvas,
+                                           C.c_void_p(vis_dev)), "dh_render_shard")
+
+    def vis_merge_min(self, acc_dev: int, src_dev: int, n: int):
+        self._check(self.L.dh_vis_merge_min(self.ctx, C.c_void_p(acc_dev), C.c_void_p(src_dev), int(n)),
+                    "dh_vis_merge_min")
+
+    def shade_shard(self, merged_dev: int, shard: int):
+        self._check(self.L.dh_shade_shard(self.ctx, C.c_void_p(merged_dev), int(shard)), "dh_shade_shard")
+
     def set_graph(self, on: bool):
         """dh_set_graph: replay repeated frame structures from a hipGraph."""
         self._check(self.L.dh_set_graph(self.ctx, int(bool(on))), "dh_set_graph")

@@ -200,6 +200,35 @@ enum { DH_BATCH_GRAPH = 1u };
  * sort is a per-frame sequence. */
 int dh_render_batch(dh_ctx ctx, const dh_view* views, uint32_t n_views, uint32_t flags);
 
+/* ---- sort-last shards: one frame split by submission range over GPUs ----
+ * No counterpart in the reference (its render is one sequential loop); this
+ * is the per-GPU part of a multi-GPU frame.  A caller splits the frame's mesh
+ * list into consecutive submission ranges, one per GPU (shard < 8), and:
+ *  1. dh_render_shard renders its range into vis_dev, W*H u64 on its device:
+ *     (depth key << 32 | shard << 29 | clipped-triangle id), ~0 where no
+ *     triangle covers the pixel (at most 2^29 clipped triangles per shard);
+ *  2. the callers reduce the shards' buffers with an element-wise u64
+ *     minimum (an RCCL reduce-scatter or all-reduce with ncclMin on
+ *     ncclUint64; dh_vis_merge_min does the same on one device) -- the result
+ *     is the single-GPU winner of every pixel, submission-order ties included;
+ *  3. dh_shade_shard, on the context that rendered the shard and with no
+ *     other frame in between, shades the pixels whose winner is its own into
+ *     the canvas given to dh_render_shard (shard 0 also writes the clear
+ *     colour of pixels without a winner).  The winner ids it writes are the
+ *     shard's clipped-triangle ids.
+ * Opaque meshes only (alpha < 1: DH_E_UNSUPPORTED). */
+int dh_render_shard(dh_ctx ctx, const dh_frame_params* fr, const dh_handle* meshes, uint32_t n_meshes,
+                    uint32_t shard, dh_handle canvas, uint64_t* vis_dev);
+int dh_vis_merge_min(dh_ctx ctx, uint64_t* acc_dev, const uint64_t* src_dev, uint64_t n);
+int dh_shade_shard(dh_ctx ctx, const uint64_t* merged_dev, uint32_t shard);
+
+/* Device memory for callers without their own allocator (visibility
+ * buffers above): hipMalloc/hipFree on the context's device, and a copy to
+ * host memory in the context's stream order. */
+int dh_device_alloc(dh_ctx ctx, uint64_t bytes, void** out_dev);
+int dh_device_free(dh_ctx ctx, void* dev);
+int dh_device_read(dh_ctx ctx, void* host, const void* dev, uint64_t bytes);
+
 /* Copy a canvas to host memory: bgra W*H*4 bytes, rows top-down
  * (Canvas::as_bytes_slice, canvas.rs:974-978); depth W*H floats, rows
  * bottom-up (canvas.rs:415); winner W*H ids, rows bottom-up, 0xFFFFFFFF =
