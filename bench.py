@@ -187,6 +187,58 @@ def bench_batched(reps: int) -> dict:
     }
 
 
+def bench_shard(scene, steps: int = 5) -> dict:
+    """One GPU's share of a sort-last frame at its real shape: the 1080p
+    1M-triangle soup cut into 8 equal submission ranges (125k input
+    triangles each), each shard rendered on its own context into its
+    full-screen packed u64 visibility buffer (dh_render_shard), one u64-min
+    merge of two such buffers (the per-buffer work of the cross-GPU
+    reduction, done here on one device) and one shard's shading of the merged
+    winners (dh_shade_shard).  Times: host wall clock around each synchronous
+    call, median over ``steps``.  With 8 GPUs the frame waits for the slowest
+    shard; the soup puts its near-plane straddlers first, so equal triangle
+    counts are not equal work."""
+    from draw_amd import workloads
+    from draw_amd.render import Renderer, frame_uniforms
+    s8 = workloads.split_submission(scene, 8)
+    W, H = s8.width, s8.height
+    r = Renderer(0)
+    meshes = r.mesh_handles(s8)
+    fr = frame_uniforms(s8)
+    ctxs = [r.shard_context() for _ in range(8)]
+    cv = r.canvas(W, H)
+    bufs = [r.device_alloc(W * H * 8) for _ in range(8)]
+    med = lambda v: round(float(np.median(v)) * 1e3, 4)
+    try:
+        per_shard = []
+        for k, cx in enumerate(ctxs):
+            t = []
+            for i in range(steps + 1):
+                t0 = time.perf_counter()
+                cx.render_shard(fr, [meshes[k]], k, cv, bufs[k])
+                if i:
+                    t.append(time.perf_counter() - t0)
+            per_shard.append(med(t))
+        t_m, t_s = [], []
+        for i in range(steps + 1):
+            t0 = time.perf_counter()
+            r.vis_merge_min(bufs[0], bufs[1], W * H)
+            t1 = time.perf_counter()
+            ctxs[1].shade_shard(bufs[0], 1)
+            if i:
+                t_m.append(t1 - t0)
+                t_s.append(time.perf_counter() - t1)
+    finally:
+        for p in bufs:
+            r.device_free(p)
+        for cx in ctxs:
+            cx.close()
+        r.close()
+    return {"workload": "8 submission ranges of the 1920x1080 1M-triangle soup, 125k input triangles each",
+            "render_ms_per_shard": per_shard, "render_ms_max": max(per_shard),
+            "merge_ms_per_buffer": med(t_m), "shade_ms_one_shard": med(t_s), "visibility_bytes": W * H * 8}
+
+
 def bench_serving(seconds: float, rate_hz: float, n_distinct: int = 512) -> dict:
     """Config 5: a device-resident pool of 64 seeded meshes (1k-2M triangles,
     log-uniform, 1024^2 textures) and requests (mesh, random camera,
@@ -291,6 +343,8 @@ def main():
     ap.add_argument("--n-tri", type=int, default=1_000_000)
     ap.add_argument("--cpu-sample", type=int, default=400_000, help="0 disables the CPU baseline")
     ap.add_argument("--batched-reps", type=int, default=5, help="timed batched launches; 0 skips the batched run")
+    ap.add_argument("--shard-steps", type=int, default=5,
+                    help="timed calls of the 1-of-8 sort-last shard leg; 0 skips it")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="length of the Poisson serving stream (the serving config names 60 s); 0 skips serving")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate of the serving stream")
@@ -375,6 +429,9 @@ def main():
         "roofline_shade": roof_shade,
     }
     r.close()
+    if args.shard_steps > 0:
+        log("sort-last shard (1 of 8) ...")
+        out["shard_1_of_8"] = bench_shard(scene, args.shard_steps)
     if args.batched_reps > 0:
         out["batched"] = bench_batched(args.batched_reps)
     if args.serve_seconds > 0:

@@ -429,3 +429,22 @@ def _ball_soup(n: int, radius: float, tex: Texture, rng: np.random.Generator) ->
     return Object(pc.reshape(-1, 3).astype(F32), vn.reshape(-1, 3).astype(F32),
                   rng.random((3 * n, 2)).astype(F32), [IndexedMesh(tri, 0, "ball_soup")], [tex],
                   name="ball_soup")
+
+
+def split_submission(scene: Scene, n_shards: int) -> Scene:
+    """The scene's first object with its first mesh cut into ``n_shards``
+    consecutive submission ranges (meshes over the same vertex arrays): the
+    shards of a sort-last frame, in submission order."""
+    import copy
+    o = scene.objects[0]
+    tri = o.meshes[0].triangles
+    cuts = np.linspace(0, tri.shape[0], n_shards + 1).astype(int)
+    o2 = copy.copy(o)
+    o2.meshes = []
+    for k in range(n_shards):
+        m = copy.copy(o.meshes[0])
+        m.triangles = np.ascontiguousarray(tri[cuts[k]:cuts[k + 1]])
+        o2.meshes.append(m)
+    s2 = copy.copy(scene)
+    s2.objects = [o2]
+    return s2

@@ -3,8 +3,6 @@ shards, each rendered on its own context into a packed u64 visibility buffer,
 merged with an element-wise minimum (what an RCCL ncclMin reduction does
 across GPUs) and shaded per shard, equals the single-context frame byte for
 byte; the merged winners map back to the single frame's winner ids."""
-import copy
-
 import numpy as np
 import pytest
 
@@ -14,26 +12,10 @@ from draw_amd.render import frame_uniforms
 pytestmark = pytest.mark.gpu
 
 
-def _split(scene, n_shards):
-    """The scene's one mesh as n_shards consecutive submission ranges of the same object."""
-    o = scene.objects[0]
-    tri = o.meshes[0].triangles
-    cuts = np.linspace(0, tri.shape[0], n_shards + 1).astype(int)
-    o2 = copy.copy(o)
-    o2.meshes = []
-    for k in range(n_shards):
-        m = copy.copy(o.meshes[0])
-        m.triangles = np.ascontiguousarray(tri[cuts[k]:cuts[k + 1]])
-        o2.meshes.append(m)
-    s2 = copy.copy(scene)
-    s2.objects = [o2]
-    return s2
-
-
 @pytest.mark.parametrize("n_shards,seed", [(4, 8), (8, 9), (2, 10)])
 def test_sort_last_shards_match_single_render(renderer, n_shards, seed):
-    s = _split(workloads.soup_scene(seed, n_tri=120_000, width=640, height=480, tex_size=256, cross_frac=0.1),
-               n_shards)
+    s = workloads.split_submission(
+        workloads.soup_scene(seed, n_tri=120_000, width=640, height=480, tex_size=256, cross_frac=0.1), n_shards)
     W, H = s.width, s.height
     fr = frame_uniforms(s)
     meshes = renderer.mesh_handles(s)
