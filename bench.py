@@ -187,7 +187,7 @@ def bench_batched(reps: int) -> dict:
     }
 
 
-def bench_shard(scene, steps: int = 5) -> dict:
+def bench_shard(scene, raster: tuple, steps: int = 5) -> dict:
     """One GPU's share of a sort-last frame at its real shape: the 1080p
     1M-triangle soup cut into 8 equal submission ranges (125k input
     triangles each), each shard rendered on its own context into its
@@ -197,15 +197,17 @@ def bench_shard(scene, steps: int = 5) -> dict:
     winners (dh_shade_shard).  Times: host wall clock around each synchronous
     call, median over ``steps``.  With 8 GPUs the frame waits for the slowest
     shard; the soup puts its near-plane straddlers first, so equal triangle
-    counts are not equal work."""
+    counts are not equal work.  ``raster``: (kind, tile, waves) of the plain
+    path's autotuned choice, used by every shard context."""
     from draw_amd import workloads
     from draw_amd.render import Renderer, frame_uniforms
     s8 = workloads.split_submission(scene, 8)
     W, H = s8.width, s8.height
     r = Renderer(0)
+    r.set_raster(*raster)
     meshes = r.mesh_handles(s8)
     fr = frame_uniforms(s8)
-    ctxs = [r.shard_context() for _ in range(8)]
+    ctxs = [r.shard_context() for _ in range(8)]  # inherit the raster configuration
     cv = r.canvas(W, H)
     bufs = [r.device_alloc(W * H * 8) for _ in range(8)]
     med = lambda v: round(float(np.median(v)) * 1e3, 4)
@@ -235,6 +237,7 @@ def bench_shard(scene, steps: int = 5) -> dict:
             cx.close()
         r.close()
     return {"workload": "8 submission ranges of the 1920x1080 1M-triangle soup, 125k input triangles each",
+            "raster_kind": raster[0], "tile": raster[1], "waves_per_tile": raster[2],
             "render_ms_per_shard": per_shard, "render_ms_max": max(per_shard),
             "merge_ms_per_buffer": med(t_m), "shade_ms_one_shard": med(t_s), "visibility_bytes": W * H * 8}
 
@@ -431,7 +434,7 @@ def main():
     r.close()
     if args.shard_steps > 0:
         log("sort-last shard (1 of 8) ...")
-        out["shard_1_of_8"] = bench_shard(scene, args.shard_steps)
+        out["shard_1_of_8"] = bench_shard(scene, (args.raster_kind, args.tile, args.waves), args.shard_steps)
     if args.batched_reps > 0:
         out["batched"] = bench_batched(args.batched_reps)
     if args.serve_seconds > 0:

@@ -359,6 +359,7 @@ extern "C" int dh_ctx_create_shared(dh_ctx parent, dh_ctx* out) {
     (*out)->waves = parent->waves;
     (*out)->raster_kind = parent->raster_kind;
     (*out)->bin_mode = parent->bin_mode;
+    (*out)->graph_frames = parent->graph_frames;
   }
   return rc;
 }
@@ -810,7 +811,7 @@ int render_views(dh_ctx c, const dh_view* views, uint32_t n_views, bool use_grap
     mats = c->res->mat_dev.p;
   }
   const bool transparent = !c->h_tdraws.empty();
-  if (transparent || c->shard_out) use_graph = false;
+  if (transparent) use_graph = false;
   c->shard_frame = false;
   const uint64_t n_max = std::max<uint64_t>(n_in, c->n_tin);
   key.push_back((uint64_t)(uintptr_t)mats);
@@ -857,6 +858,8 @@ int render_views(dh_ctx c, const dh_view* views, uint32_t n_views, bool use_grap
     for (const void* p : ptrs) gkey.push_back((uint64_t)(uintptr_t)p);
     gkey.push_back((uint64_t)c->cap_tri << 32 | c->cap_pairs);
     gkey.push_back(n_in);
+    gkey.push_back((uint64_t)(uintptr_t)c->shard_out);  // shard frames: visibility output and shard index
+    gkey.push_back(c->shard);
     const bool replay = use_graph && c->graph_exec && gkey == c->graph_key;
     memset(c->h_status, 0, 2 * sizeof(dh::Status));
     if (replay) {
@@ -934,7 +937,11 @@ int render_views(dh_ctx c, const dh_view* views, uint32_t n_views, bool use_grap
       // capture the same launch sequence (no stage events) for the next call
       drop_graph(c);
       HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      dh::enqueue_frame(make_args(c, mats, (uint32_t)n_in, n_tiles, n_pixels, false), c->stream);
+      if (c->shard_out)
+        dh::enqueue_visibility(make_args(c, mats, (uint32_t)n_in, n_tiles, n_pixels, false), c->shard, c->shard_out,
+                               c->stream);
+      else
+        dh::enqueue_frame(make_args(c, mats, (uint32_t)n_in, n_tiles, n_pixels, false), c->stream);
       dh::enqueue_status_out(c->status.p, c->d_status_host, 2, c->stream);
       hipGraph_t g = nullptr;
       HIPCHK(c, hipStreamEndCapture(c->stream, &g));
@@ -1006,7 +1013,7 @@ extern "C" int dh_render_shard(dh_ctx c, const dh_frame_params* fr, const dh_han
   v.n_meshes = n_meshes;
   c->shard_out = vis_dev;
   c->shard = shard;
-  const int rc = render_views(c, &v, 1, false, false);
+  const int rc = render_views(c, &v, 1, c->graph_frames, false);
   c->shard_out = nullptr;
   return rc;
 }

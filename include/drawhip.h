@@ -115,8 +115,9 @@ int dh_ctx_create(int32_t device, dh_ctx* out);
 /* A second context (own HIP stream and scratch) on the parent's device that
  * shares the parent's resources: handles created through either are valid in
  * both, so frames can render concurrently from one resident mesh pool.  The
- * resources live until the last context sharing them is destroyed.  No
- * counterpart in the reference (one canvas, one thread). */
+ * resources live until the last context sharing them is destroyed.  It
+ * starts with the parent's raster kernel, tiling, bin order and graph
+ * settings.  No counterpart in the reference (one canvas, one thread). */
 int dh_ctx_create_shared(dh_ctx parent, dh_ctx* out);
 int dh_ctx_destroy(dh_ctx ctx);
 /* Human-readable text of the last error on this context (never NULL). */

@@ -76,3 +76,25 @@ def test_shard_errors(renderer):
     finally:
         renderer.device_free(buf)
         cx.close()
+
+
+def test_shard_graph_replay_identical(renderer):
+    """A repeated shard frame replays its captured launch sequence; the
+    visibility it writes is the eager frame's, bit for bit."""
+    s = workloads.split_submission(workloads.soup_scene(11, n_tri=40_000, width=320, height=240, tex_size=256), 3)
+    W, H = s.width, s.height
+    fr = frame_uniforms(s)
+    meshes = renderer.mesh_handles(s)
+    cx = renderer.shard_context()
+    bufs = [renderer.device_alloc(W * H * 8) for _ in range(2)]
+    try:
+        out = []
+        for rep in range(3):
+            cx.render_shard(fr, [meshes[1]], 1, renderer.canvas(W, H), bufs[rep % 2])
+            out.append(renderer.device_read(bufs[rep % 2], W * H * 8, np.uint64))
+    finally:
+        for p in bufs:
+            renderer.device_free(p)
+        cx.close()
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
+    assert (out[0] != np.uint64(0xFFFFFFFFFFFFFFFF)).any()
