@@ -729,7 +729,8 @@ __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict_
 // Second pass: each block takes the exclusive prefix of the partial sums
 // before it (at most kScanG values, read straight from the partials) and
 // scans its chunk; block 0 also publishes the total and the overflow flag.
-__global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+// in and out may be the same array (in-place scans): no __restrict__ on them.
+__global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* in, uint32_t* out,
                                                     const uint32_t* n_ptr, uint32_t n_const, uint32_t cap,
                                                     const uint32_t* __restrict__ partials, uint32_t* total_out,
                                                     uint32_t cap_check, uint32_t* overflow_word,
@@ -785,8 +786,8 @@ struct ScanSegs {
 
 __global__ __launch_bounds__(1024) void k_scan_small(ScanSegs segs, uint32_t n) {
   const ScanSeg& g = segs.s[blockIdx.x];
-  const uint32_t* __restrict__ in = g.in;
-  uint32_t* __restrict__ out = g.out;
+  const uint32_t* in = g.in;  // may be out (in-place scans)
+  uint32_t* out = g.out;
   uint32_t* total_out = g.total_out;
   uint32_t* overflow_word = g.overflow_word;
   const uint32_t cap_check = g.cap_check, overflow_bit = g.overflow_bit;
