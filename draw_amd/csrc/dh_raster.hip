@@ -246,7 +246,8 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
             const float w0 = fmaf(A[0], dxf[qq], fmaf(B[0], dyf[qq], E[0]));
             const float w1 = fmaf(A[1], dxf[qq], fmaf(B[1], dyf[qq], E[1]));
             const float w2 = fmaf(A[2], dxf[qq], fmaf(B[2], dyf[qq], E[2]));
-            const bool cov = fminf(fminf(w0, w1), w2) >= 0.0f && onscr[qq] && !(zkey > zk[qq]);
+            // bitwise, not short-circuit: straight-line code, no exec-mask branches
+            const bool cov = (fminf(fminf(w0, w1), w2) >= 0.0f) & onscr[qq] & (zkey <= zk[qq]);
             const uint64_t m = __ballot(cov);
             if (m) {
               if (cov) {
