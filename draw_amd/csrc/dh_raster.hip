@@ -75,7 +75,9 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
     const DevMaterial* __restrict__ mats, uint32_t* __restrict__ vis_out, Status* __restrict__ status) {
   constexpr int QS = TS / 8, NQ = QS * QS, NW = NQ / QPW, NT = 64 * NW;
   constexpr int CH = 128;    // records staged per round
-  constexpr int QCAP = 256;  // fragment queue per wave
+  // fragment queue per wave: drained once it holds more than kQFlush entries,
+  // checked before each triangle, which appends at most 64 per quadrant
+  constexpr int kQFlush = 192, QCAP = kQFlush + 64 * QPW;
   __shared__ SetupRec s_rec[CH];
   __shared__ uint4 s_hdr[CH];  // scan rectangle x, y; flags; depth-bound key
   __shared__ float4 s_pre[CH][3];  // per edge: s*A, s*B, s*(C-D), |C|+|D| with s = sign(f)
@@ -222,6 +224,7 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
         const int kl = (int)__builtin_ctzll(hm);
         const int k = g0 + kl;
         hm &= hm - 1;
+        if (qn > kQFlush) flush();  // room for this triangle's visits
         const SetupRec& r = s_rec[k];
         // the record's header, from the registers of the lane that pre-filtered it
         const uint32_t hx = (uint32_t)__builtin_amdgcn_readlane((int)h.x, kl);
@@ -255,7 +258,6 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
                 s_q[wave][pos] = ((uint32_t)k << 16) | (uint32_t)lp[qq];
               }
               qn += (uint32_t)__popcll(m);
-              if (qn > QCAP - 64) flush();
             }
           }
           continue;
@@ -284,7 +286,6 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
                 s_q[wave][pos] = ((uint32_t)k << 16) | (uint32_t)lp[qq];
               }
               qn += (uint32_t)__popcll(m);
-              if (qn > QCAP - 64) flush();
             }
           } else {
             // Any other triangle: the reference's predicate on the divided values.
