@@ -218,50 +218,60 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
           }
         }
       }
-      uint64_t hm = __ballot(hit);
-      const uint64_t em = __ballot(hit && wexact);
-      while (hm) {
-        const int kl = (int)__builtin_ctzll(hm);
+      // Exact two-FMA hits in a tight loop, then the rest (the visibility
+      // minimum does not depend on the order of the visits).
+      uint64_t hf = __ballot(hit && wexact);
+      uint64_t hs = __ballot(hit) & ~hf;
+      while (hf) {
+        const int kl = (int)__builtin_ctzll(hf);
         const int k = g0 + kl;
-        hm &= hm - 1;
+        hf &= hf - 1;
         if (qn > kQFlush) flush();  // room for this triangle's visits
-        const SetupRec& r = s_rec[k];
         // the record's header, from the registers of the lane that pre-filtered it
+        const uint32_t hx = (uint32_t)__builtin_amdgcn_readlane((int)h.x, kl);
+        const uint32_t hy = (uint32_t)__builtin_amdgcn_readlane((int)h.y, kl);
+        const int rx0 = (int)(hx & 0xFFFFu), rx1 = (int)(hx >> 16);
+        const int ry0 = (int)(hy & 0xFFFFu), ry1 = (int)(hy >> 16);
+        const uint32_t zkey = (uint32_t)__builtin_amdgcn_readlane((int)h.w, kl);
+        float A[3], B[3], E[3];
+#pragma unroll
+        for (int e = 0; e < 3; e++) {
+          A[e] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pA[e]), kl));
+          B[e] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pB[e]), kl));
+          E[e] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pE[e]), kl));
+        }
+#pragma unroll
+        for (int qq = 0; qq < QPW; qq++) {
+          const int Q = wq0 + qq;
+          const int qx0 = tx0 + (Q % QS) * 8, qy0 = ty0 + (Q / QS) * 8;
+          if (rx1 < qx0 || rx0 > qx0 + 7 || ry1 < qy0 || ry0 > qy0 + 7) continue;
+          const float w0 = fmaf(A[0], dxf[qq], fmaf(B[0], dyf[qq], E[0]));
+          const float w1 = fmaf(A[1], dxf[qq], fmaf(B[1], dyf[qq], E[1]));
+          const float w2 = fmaf(A[2], dxf[qq], fmaf(B[2], dyf[qq], E[2]));
+          // bitwise, not short-circuit: straight-line code, no exec-mask branches
+          const bool cov = (fminf(fminf(w0, w1), w2) >= 0.0f) & onscr[qq] & (zkey <= zk[qq]);
+          const uint64_t m = __ballot(cov);
+          if (m) {
+            if (cov) {
+              const uint32_t pos = qn + (uint32_t)__popcll(m & lanemask_lt);
+              s_q[wave][pos] = ((uint32_t)k << 16) | (uint32_t)lp[qq];
+            }
+            qn += (uint32_t)__popcll(m);
+          }
+        }
+      }
+      while (hs) {
+        const int kl = (int)__builtin_ctzll(hs);
+        const int k = g0 + kl;
+        hs &= hs - 1;
+        if (qn > kQFlush) flush();
+        const SetupRec& r = s_rec[k];
         const uint32_t hx = (uint32_t)__builtin_amdgcn_readlane((int)h.x, kl);
         const uint32_t hy = (uint32_t)__builtin_amdgcn_readlane((int)h.y, kl);
         const int rx0 = (int)(hx & 0xFFFFu), rx1 = (int)(hx >> 16);
         const int ry0 = (int)(hy & 0xFFFFu), ry1 = (int)(hy >> 16);
         const uint32_t flags = (uint32_t)__builtin_amdgcn_readlane((int)h.z, kl);
         const uint32_t zkey = (uint32_t)__builtin_amdgcn_readlane((int)h.w, kl);
-        if ((em >> kl) & 1ull) {
-          float A[3], B[3], E[3];
-#pragma unroll
-          for (int e = 0; e < 3; e++) {
-            A[e] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pA[e]), kl));
-            B[e] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pB[e]), kl));
-            E[e] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pE[e]), kl));
-          }
-#pragma unroll
-          for (int qq = 0; qq < QPW; qq++) {
-            const int Q = wq0 + qq;
-            const int qx0 = tx0 + (Q % QS) * 8, qy0 = ty0 + (Q / QS) * 8;
-            if (rx1 < qx0 || rx0 > qx0 + 7 || ry1 < qy0 || ry0 > qy0 + 7) continue;
-            const float w0 = fmaf(A[0], dxf[qq], fmaf(B[0], dyf[qq], E[0]));
-            const float w1 = fmaf(A[1], dxf[qq], fmaf(B[1], dyf[qq], E[1]));
-            const float w2 = fmaf(A[2], dxf[qq], fmaf(B[2], dyf[qq], E[2]));
-            // bitwise, not short-circuit: straight-line code, no exec-mask branches
-            const bool cov = (fminf(fminf(w0, w1), w2) >= 0.0f) & onscr[qq] & (zkey <= zk[qq]);
-            const uint64_t m = __ballot(cov);
-            if (m) {
-              if (cov) {
-                const uint32_t pos = qn + (uint32_t)__popcll(m & lanemask_lt);
-                s_q[wave][pos] = ((uint32_t)k << 16) | (uint32_t)lp[qq];
-              }
-              qn += (uint32_t)__popcll(m);
-            }
-          }
-          continue;
-        }
 #pragma unroll
         for (int qq = 0; qq < QPW; qq++) {
           const int Q = wq0 + qq;
