@@ -179,6 +179,7 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
     for (int g0 = 0; g0 < n; g0 += 64) {
       bool hit = false, wexact = false;
       float pA[3] = {0.0f, 0.0f, 0.0f}, pB[3] = {0.0f, 0.0f, 0.0f}, pE[3] = {0.0f, 0.0f, 0.0f};
+      uint32_t qm = 0;  // exact hits: the wave's quadrants the record's rectangle meets
       uint4 h = make_uint4(0u, 0u, 0u, 0u);
       if (g0 + lane < n) {
         h = s_hdr[g0 + lane];
@@ -216,6 +217,22 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
             pB[e] = q.y;
             pE[e] = ((q.x * (float)wx0 + q.y * (float)wy0) + q.z) - ((h.z & (kTieA << e)) ? 0.0f : 1.0f);
           }
+          // the wave's quadrants the scan rectangle meets, one bit each:
+          // quadrant column c meets [x0, x1] iff floor((x0 - tx0) / 8) <= c <=
+          // floor((x1 - tx0) / 8), rows likewise (the rectangle meets the
+          // wave's, so the clamped ranges are not empty)
+          const int c0 = max(((int)(h.x & 0xFFFFu) - tx0) >> 3, 0), c1 = min(((int)(h.x >> 16) - tx0) >> 3, QS - 1);
+          const int r0 = max(((int)(h.y & 0xFFFFu) - ty0) >> 3, 0), r1 = min(((int)(h.y >> 16) - ty0) >> 3, QS - 1);
+          const uint32_t cols = (2u << c1) - (1u << c0), rows = (2u << r1) - (1u << r0);
+          if constexpr (QPW <= QS) {
+            // part of one quadrant row: columns wq0 % QS .. + QPW - 1, row wq0 / QS
+            qm = ((cols >> (wq0 % QS)) & ((1u << QPW) - 1u)) & (0u - ((rows >> (wq0 / QS)) & 1u));
+          } else {
+            // whole quadrant rows wq0 / QS .. + QPW / QS - 1
+#pragma unroll
+            for (int rr = 0; rr < QPW / QS; rr++)
+              if ((rows >> (wq0 / QS + rr)) & 1u) qm |= (cols & ((1u << QS) - 1u)) << (rr * QS);
+          }
         }
       }
       // Exact two-FMA hits in a tight loop, then the rest (the visibility
@@ -227,11 +244,8 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
         const int k = g0 + kl;
         hf &= hf - 1;
         if (qn > kQFlush) flush();  // room for this triangle's visits
-        // the record's header, from the registers of the lane that pre-filtered it
-        const uint32_t hx = (uint32_t)__builtin_amdgcn_readlane((int)h.x, kl);
-        const uint32_t hy = (uint32_t)__builtin_amdgcn_readlane((int)h.y, kl);
-        const int rx0 = (int)(hx & 0xFFFFu), rx1 = (int)(hx >> 16);
-        const int ry0 = (int)(hy & 0xFFFFu), ry1 = (int)(hy >> 16);
+        // the record's values, from the registers of the lane that pre-filtered it
+        const uint32_t qmk = (uint32_t)__builtin_amdgcn_readlane((int)qm, kl);
         const uint32_t zkey = (uint32_t)__builtin_amdgcn_readlane((int)h.w, kl);
         float A[3], B[3], E[3];
 #pragma unroll
@@ -242,9 +256,7 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
         }
 #pragma unroll
         for (int qq = 0; qq < QPW; qq++) {
-          const int Q = wq0 + qq;
-          const int qx0 = tx0 + (Q % QS) * 8, qy0 = ty0 + (Q / QS) * 8;
-          if (rx1 < qx0 || rx0 > qx0 + 7 || ry1 < qy0 || ry0 > qy0 + 7) continue;
+          if (!((qmk >> qq) & 1u)) continue;
           const float w0 = fmaf(A[0], dxf[qq], fmaf(B[0], dyf[qq], E[0]));
           const float w1 = fmaf(A[1], dxf[qq], fmaf(B[1], dyf[qq], E[1]));
           const float w2 = fmaf(A[2], dxf[qq], fmaf(B[2], dyf[qq], E[2]));
