@@ -47,10 +47,22 @@ __device__ inline bool uv_ok(const ShadeRec* __restrict__ shade, const DevMateri
 }
 
 
+// Maximum over the wave with DPP moves (no LDS): quads, half rows and rows
+// by permutes and mirrors, then rows 1 and 3 take lane 15 of the row before
+// (row_bcast:15) and rows 2 and 3 lane 31 (row_bcast:31), so lane 63 holds
+// the maximum.  Lanes a move does not write get 0, the identity of max.
+template <int CTRL, int ROW_MASK>
+__device__ inline uint32_t dpp_max(uint32_t v) {
+  return max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xF, false));
+}
 __device__ inline uint32_t wave_max_u32(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
-  return v;
+  v = dpp_max<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
+  v = dpp_max<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
+  v = dpp_max<0x141, 0xF>(v);  // row_half_mirror
+  v = dpp_max<0x140, 0xF>(v);  // row_mirror
+  v = dpp_max<0x142, 0xA>(v);  // row_bcast:15
+  v = dpp_max<0x143, 0xC>(v);  // row_bcast:31
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 
 // Per-tile visibility.  The tile's bin list is near first (or in submission
