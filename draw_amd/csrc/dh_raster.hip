@@ -264,7 +264,10 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
         for (int e = 0; e < 3; e++) {
           A[e] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pA[e]), kl));
           B[e] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pB[e]), kl));
-          E[e] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pE[e]), kl));
+          const float Es = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pE[e]), kl));
+          // once per triangle into a vector register: a VALU instruction reads
+          // at most one scalar register, and each fma below already takes A or B
+          asm("v_mov_b32 %0, %1" : "=v"(E[e]) : "s"(Es));
         }
 #pragma unroll
         for (int qq = 0; qq < QPW; qq++) {
