@@ -98,6 +98,10 @@ struct ResPool {
 struct dh_ctx_s {
   int device = 0;
   hipStream_t stream = nullptr;
+  // a second stream (default priority) for independent kernels of a frame,
+  // forked from and joined back to `stream` with these events
+  hipStream_t side = nullptr;
+  hipEvent_t xev[4] = {nullptr, nullptr, nullptr, nullptr};
   std::string err;
   std::shared_ptr<ResPool> res;
   // raster configuration
@@ -315,6 +319,8 @@ int ctx_new(int32_t device, std::shared_ptr<ResPool> pool, dh_ctx* out) {
   c->device = device;
   hipError_t e = hipSetDevice(device);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking);
+  for (int i = 0; i < 4 && e == hipSuccess; i++) e = hipEventCreateWithFlags(&c->xev[i], hipEventDisableTiming);
   // the frame's last kernel writes the status words here (coherent: no copy, no cache flush)
   if (e == hipSuccess)
     e = hipHostMalloc(reinterpret_cast<void**>(&c->h_status), 2 * sizeof(dh::Status), hipHostMallocCoherent);
@@ -382,6 +388,9 @@ extern "C" int dh_ctx_destroy(dh_ctx c) {
   if (c->h_tables) hipHostFree(c->h_tables);
   for (int i = 0; i < 5; i++)
     if (c->ev[i]) hipEventDestroy(c->ev[i]);
+  for (int i = 0; i < 4; i++)
+    if (c->xev[i]) hipEventDestroy(c->xev[i]);
+  if (c->side) hipStreamDestroy(c->side);
   if (c->stream) hipStreamDestroy(c->stream);
   c->res.reset();  // the pool goes with its last context
   delete c;
@@ -784,6 +793,8 @@ dh::PipeArgs make_args(dh_ctx c, const dh::DevMaterial* mats, uint32_t n_in, uin
   a.vis = c->vis.p;
   a.status = c->status.p;
   for (int i = 0; i < 5; i++) a.ev[i] = events ? c->ev[i] : nullptr;
+  a.side = c->side;
+  for (int i = 0; i < 4; i++) a.xev[i] = c->xev[i];
   return a;
 }
 

@@ -1226,6 +1226,22 @@ int enqueue_radix_sort(uint32_t* const keys[2], uint32_t* const vals[2], const u
   return cur;
 }
 
+// Fork: work launched on the returned stream runs beside the main stream's
+// (the side stream waits for everything enqueued on `s` so far).  Without a
+// side stream it is `s` itself.
+static hipStream_t fork(hipStream_t s, const PipeArgs& a, int i) {
+  if (!a.side) return s;
+  (void)hipEventRecord(a.xev[i], s);
+  (void)hipStreamWaitEvent(a.side, a.xev[i], 0);
+  return a.side;
+}
+// Join: the main stream waits for everything enqueued on the side stream.
+static void join(hipStream_t s, const PipeArgs& a, int i) {
+  if (!a.side) return;
+  (void)hipEventRecord(a.xev[i], a.side);
+  (void)hipStreamWaitEvent(s, a.xev[i], 0);
+}
+
 // Status reset, geometry, view bases and binning of one pipeline pass;
 // returns the arguments the per-tile stages read (tile lists in b.vals[0]).
 PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
@@ -1262,19 +1278,23 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
     uint8_t* wf = reinterpret_cast<uint8_t*>(a.big);
     uint32_t* dk = depth_sorted ? a.keys[1] + a.cap_tri : nullptr;
     uint32_t* dv = depth_sorted ? a.vals[1] : nullptr;
+    // the unclipped and clipped emits write disjoint outputs: the clipped
+    // one runs beside the other on the side stream
+    const hipStream_t s2 = fork(s, a, 0);
     if (a.ordered_bins) {
       hipLaunchKernelGGL((k_geom_emit<true, false>), dim3(n_blk), dim3(256), 0, s, a.draws, a.n_draws, a.views,
                          a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, a.bincnt, wf, dk, dv);
-      hipLaunchKernelGGL((k_geom_emit<true, true>), dim3(n_blk), dim3(256), 0, s, a.draws, a.n_draws, a.views,
+      hipLaunchKernelGGL((k_geom_emit<true, true>), dim3(n_blk), dim3(256), 0, s2, a.draws, a.n_draws, a.views,
                          a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, a.bincnt, wf, dk, dv);
     } else {
       hipLaunchKernelGGL((k_geom_emit<false, false>), dim3(n_blk), dim3(256), 0, s, a.draws, a.n_draws, a.views,
                          a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, nullptr, nullptr, nullptr,
                          nullptr);
-      hipLaunchKernelGGL((k_geom_emit<false, true>), dim3(n_blk), dim3(256), 0, s, a.draws, a.n_draws, a.views,
+      hipLaunchKernelGGL((k_geom_emit<false, true>), dim3(n_blk), dim3(256), 0, s2, a.draws, a.n_draws, a.views,
                          a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, nullptr, nullptr, nullptr,
                          nullptr);
     }
+    join(s, a, 1);
   }
   if (a.n_views > 1)
     hipLaunchKernelGGL(k_view_id_base, dim3(cdiv(a.n_views, 256)), dim3(256), 0, s, const_cast<DevView*>(a.views),
@@ -1290,9 +1310,12 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
   const uint32_t gw = std::min<uint32_t>(cdiv(a.cap_tri, 64), 4096u);
   PipeArgs b = a;
   if (a.ordered_bins) {
-    hipLaunchKernelGGL(k_wide_list, dim3(gl), dim3(256), 0, s, wflag, a.status, a.cap_tri, wide_list,
+    // the wide-triangle list and counts (keys[1][0, cap_tri), bincnt) beside
+    // the depth sort (keys[1] + cap_tri .., vals[1], hist, partials)
+    const hipStream_t s2 = depth_sorted ? fork(s, a, 2) : s;
+    hipLaunchKernelGGL(k_wide_list, dim3(gl), dim3(256), 0, s2, wflag, a.status, a.cap_tri, wide_list,
                        reinterpret_cast<uint32_t*>(a.ranges), 2u * a.n_tiles);
-    hipLaunchKernelGGL(k_bin_count_wide<true>, dim3(gw), dim3(256), 0, s, a.setup, a.views, a.status, a.TS,
+    hipLaunchKernelGGL(k_bin_count_wide<true>, dim3(gw), dim3(256), 0, s2, a.setup, a.views, a.status, a.TS,
                        wide_list, a.bincnt, a.tile_cnt);
     if (depth_sorted) {
       // near first: the clipped triangles in depth-key order (one stable
@@ -1302,6 +1325,7 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
       uint32_t* const dvals[2] = {a.vals[1], a.vals[1] + a.cap_tri};
       const int dc = enqueue_radix_sort(dkeys, dvals, &a.status->n_out, a.cap_tri, kDepthKeyBits, a.hist,
                                         a.partials, s);
+      join(s, a, 3);
       uint32_t* order = dvals[dc];
       uint32_t* sorted_off = dkeys[dc];
       scan_u32(a.bincnt, sorted_off, &a.status->n_out, 0, a.cap_tri, a.partials, &a.status->n_pairs, a.cap_pairs,

@@ -52,6 +52,11 @@ struct PipeArgs {
   uint32_t* tables_dst;
   uint32_t tables_words;
   hipEvent_t ev[5];    // optional stage events (all null when not profiling)
+  // independent kernels fork onto `side` (null: everything on the main
+  // stream) and join back with xev[] (fork 0 / join 1: the two geometry
+  // emits; fork 2 / join 3: the wide-triangle counts beside the depth sort)
+  hipStream_t side;
+  hipEvent_t xev[4];
 };
 
 // One transparent mesh (device pointers): its triangles, sorted in place.
