@@ -47,6 +47,11 @@ __device__ inline bool uv_ok(const ShadeRec* __restrict__ shade, const DevMateri
 }
 
 
+// popcount(m & lanes below this one) + base (v_mbcnt_lo / v_mbcnt_hi).
+__device__ inline uint32_t lanes_below(uint64_t m, uint32_t base) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, base));
+}
+
 // Maximum over the wave with DPP moves (no LDS): quads, half rows and rows
 // by permutes and mirrors, then rows 1 and 3 take lane 15 of the row before
 // (row_bcast:15) and rows 2 and 3 lane 31 (row_bcast:31), so lane 63 holds
@@ -108,7 +113,11 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);  // wave-uniform: keeps the wave's geometry scalar
   const uint2 rg = ranges[tile];
 
-  for (int p = tid; p < TS * TS; p += NT) s_vis[p] = ~0ull;
+  // Pixels off the screen start at key 0, below every fragment's (depth
+  // keys of recorded fragments and staged depth bounds are >= 1), so the
+  // early-Z test alone keeps them uncovered.
+  for (int p = tid; p < TS * TS; p += NT)
+    s_vis[p] = (tx0 + p % TS < W && ty0 + p / TS < H) ? ~0ull : 0ull;
 
   const int lx = lane & 7, ly = lane >> 3;
   const uint64_t lanemask_lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -119,18 +128,16 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
   const int wx1 = wx0 + (QPW < QS ? QPW : QS) * 8 - 1;
   const int wy1 = wy0 + (QPW < QS ? 1 : QPW / QS) * 8 - 1;
   int lp[QPW];        // local index of the lane's pixel in each of its quadrants
-  uint32_t zk[QPW];   // recorded depth key at that pixel (0xFFFFFFFF: none yet)
+  uint32_t zk[QPW];   // recorded depth key at that pixel (0xFFFFFFFF: none yet; 0: off the screen)
   float dxf[QPW], dyf[QPW];  // the pixel's offset from the wave rectangle's origin
-  bool onscr[QPW];           // the pixel lies on the screen
 #pragma unroll
   for (int qq = 0; qq < QPW; qq++) {
     const int Q = wq0 + qq;
     lp[qq] = ((Q / QS) * 8 + ly) * TS + (Q % QS) * 8 + lx;
-    zk[qq] = 0xFFFFFFFFu;
     const int px = tx0 + (Q % QS) * 8 + lx, py = ty0 + (Q / QS) * 8 + ly;
     dxf[qq] = (float)(px - wx0);
     dyf[qq] = (float)(py - wy0);
-    onscr[qq] = px < W && py < H;
+    zk[qq] = (px < W && py < H) ? 0xFFFFFFFFu : 0u;
   }
   uint32_t wmax = 0xFFFFFFFFu;  // largest recorded depth key over the wave's pixels
   uint32_t qn = 0;
@@ -177,7 +184,9 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
       for (int q = 0; q < 6; q++) w[q] = src[q];
 #pragma unroll
       for (int q = 0; q < 6; q++) dst[q] = w[q];
-      s_hdr[i] = make_uint4(w[5].x, w[5].y, w[5].z, depth_key(__uint_as_float(w[4].w)));
+      // depth bound key, at least 1: only off-screen pixels hold key 0 (a
+      // fragment key of 0 is a NaN depth, which never passes the depth test)
+      s_hdr[i] = make_uint4(w[5].x, w[5].y, w[5].z, max(depth_key(__uint_as_float(w[4].w)), 1u));
       const float fs[3] = {__uint_as_float(w[3].x), __uint_as_float(w[3].y), __uint_as_float(w[3].z)};
 #pragma unroll
       for (int e = 0; e < 3; e++) {
@@ -245,6 +254,22 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
             for (int rr = 0; rr < QPW / QS; rr++)
               if ((rows >> (wq0 / QS + rr)) & 1u) qm |= (cols & ((1u << QS) - 1u)) << (rr * QS);
           }
+          // ...less the quadrants an edge rules out.  A visit computes
+          // E + A dx + B dy at the quadrant's pixels, exactly (integers below
+          // S); its largest value is at the corner dx = ox + (A > 0 ? 7 : 0),
+          // dy = oy + (B > 0 ? 7 : 0), computed here by the same exact fmas.
+          // Below zero there, no pixel of the quadrant is covered.
+#pragma unroll
+          for (int e = 0; e < 3; e++) {
+            const float c00 = fmaf(pA[e], pA[e] > 0.0f ? 7.0f : 0.0f, fmaf(pB[e], pB[e] > 0.0f ? 7.0f : 0.0f, pE[e]));
+#pragma unroll
+            for (int qq = 0; qq < QPW; qq++) {
+              constexpr int kRowQ = QPW < QS ? QPW : QS;  // quadrants per row of the wave's rectangle
+              const float ox = (float)((qq % kRowQ) * 8), oy = (float)((qq / kRowQ) * 8);
+              const float cmax = fmaf(pA[e], ox, fmaf(pB[e], oy, c00));
+              if (cmax < 0.0f) qm &= ~(1u << qq);
+            }
+          }
         }
       }
       // Exact two-FMA hits in a tight loop, then the rest (the visibility
@@ -275,14 +300,15 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
           const float w0 = fmaf(A[0], dxf[qq], fmaf(B[0], dyf[qq], E[0]));
           const float w1 = fmaf(A[1], dxf[qq], fmaf(B[1], dyf[qq], E[1]));
           const float w2 = fmaf(A[2], dxf[qq], fmaf(B[2], dyf[qq], E[2]));
-          // bitwise, not short-circuit: straight-line code, no exec-mask branches
-          const bool cov = (fminf(fminf(w0, w1), w2) >= 0.0f) & onscr[qq] & (zkey <= zk[qq]);
-          const uint64_t m = __ballot(cov);
+          // bitwise, not short-circuit: straight-line code, no exec-mask
+          // branches.  The ballot of each comparison is its lane mask; the
+          // masks are ANDed on the scalar unit.
+          const bool inside = fminf(fminf(w0, w1), w2) >= 0.0f, nearer = zkey <= zk[qq];
+          const bool cov = inside & nearer;
+          const uint64_t m = __ballot(inside) & __ballot(nearer);
           if (m) {
-            if (cov) {
-              const uint32_t pos = qn + (uint32_t)__popcll(m & lanemask_lt);
-              s_q[wave][pos] = ((uint32_t)k << 16) | (uint32_t)lp[qq];
-            }
+            uint32_t* const qb = &s_q[wave][qn];
+            if (cov) qb[lanes_below(m, 0u)] = ((uint32_t)k << 16) | (uint32_t)lp[qq];
             qn += (uint32_t)__popcll(m);
           }
         }
