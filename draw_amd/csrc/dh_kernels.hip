@@ -21,7 +21,9 @@
 //   7 k_bin_emit(_wide)  (tile, triangle) pairs in that order
 //   8 radix sort     stable LSD sort of the pairs by tile -> per-tile lists,
 //                    near first (or in submission order)
-//   9 k_tile_ranges  [start, end) of every tile's list
+//   9 tile ranges    [start, end) of every tile's list: written by the sort's
+//                    scatter from its scanned histogram when the tile key
+//                    takes one pass, else by k_tile_ranges from the sorted keys
 //  10 k_raster       one workgroup per screen tile: coverage and the strict
 //                    depth test (canvas.rs:668-682, 923) into an LDS visibility
 //                    buffer of (depth key << 32 | triangle id); the result does
@@ -1070,11 +1072,16 @@ __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* __restrict__
     for (int w = 0; w < 4; w++) hist[d * kRadixW + lb * 4u + (uint32_t)w] = h[w][d];
 }
 
-template <int BITS>
+// KEYS_OUT false: only the values are written (a last pass whose sorted keys
+// nobody reads).  ranges (may be null): [start, end) of each of the first
+// n_ranges digits in the output, straight from the scanned histogram -- the
+// per-tile lists when this pass sorts by the whole key.
+template <int BITS, bool KEYS_OUT>
 __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
                                                        uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
                                                        const uint32_t* n_ptr, uint32_t cap, int shift,
-                                                       const uint32_t* __restrict__ hist_off) {
+                                                       const uint32_t* __restrict__ hist_off,
+                                                       uint2* __restrict__ ranges, uint32_t n_ranges) {
   constexpr uint32_t ND = 1u << BITS;
   __shared__ uint32_t run[4][ND];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -1082,6 +1089,9 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
   const uint32_t C = radix_chunk(n);
   const uint32_t gw = xcd_remap(blockIdx.x, gridDim.x) * 4u + (uint32_t)wave;
   const uint32_t b0 = min(gw * C, n), b1 = min(b0 + C, n);
+  if (ranges)
+    for (uint32_t d = blockIdx.x * 256u + (uint32_t)tid; d < n_ranges; d += gridDim.x * 256u)
+      ranges[d] = make_uint2(hist_off[d * kRadixW], d + 1u < ND ? hist_off[(d + 1u) * kRadixW] : n);
   for (uint32_t d = (uint32_t)lane; d < ND; d += 64) run[wave][d] = hist_off[d * kRadixW + gw];
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -1111,7 +1121,7 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
       if (valid) base = run[wave][dgt];
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       if (valid) {
-        kout[base + rank] = key;
+        if (KEYS_OUT) kout[base + rank] = key;
         vout[base + rank] = vv[u];
         if (rank == 0) run[wave][dgt] = base + (uint32_t)__popcll(peers);
       }
@@ -1202,25 +1212,42 @@ static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b -
 // keys, *n_ptr items (at most cap): one 8-bit pass when key_bits <= 8, one
 // 11-bit pass when key_bits <= 11, else
 // 8-bit passes.  Returns the index of the buffer pair holding the result.
+// keys_out false: the sorted keys are not written (the values are).  ranges:
+// when the sort is one pass (key_bits <= 11), [start, end) of each key below
+// n_ranges in the sorted order; not written otherwise.
 int enqueue_radix_sort(uint32_t* const keys[2], uint32_t* const vals[2], const uint32_t* n_ptr, uint32_t cap,
-                       int key_bits, uint32_t* hist, uint32_t* partials, hipStream_t s) {
+                       int key_bits, uint32_t* hist, uint32_t* partials, hipStream_t s, bool keys_out,
+                       uint2* ranges, uint32_t n_ranges) {
   int cur = 0;
   // one 8-bit pass for keys of up to 8 bits, one 11-bit pass up to 11 bits
   const bool one_pass = key_bits > 8 && key_bits <= 11;
   const int step = one_pass ? 11 : 8;
   for (int shift = 0; shift < key_bits; shift += step) {
     const uint32_t nh = (one_pass ? 2048u : 256u) * kRadixW;
+    const bool last = shift + step >= key_bits;
+    const bool kout = !last || keys_out;
+    uint2* const rg = (last && shift == 0) ? ranges : nullptr;
+    const uint32_t nr = rg ? n_ranges : 0u;
     if (one_pass)
       hipLaunchKernelGGL(k_radix_hist<11>, dim3(kRadixG), dim3(256), 0, s, keys[cur], n_ptr, cap, shift, hist);
     else
       hipLaunchKernelGGL(k_radix_hist<8>, dim3(kRadixG), dim3(256), 0, s, keys[cur], n_ptr, cap, shift, hist);
     scan_u32(hist, hist, nullptr, nh, nh, partials, nullptr, 0, nullptr, 0, s);
-    if (one_pass)
-      hipLaunchKernelGGL(k_radix_scatter<11>, dim3(kRadixG), dim3(256), 0, s, keys[cur], vals[cur], keys[cur ^ 1],
-                         vals[cur ^ 1], n_ptr, cap, shift, hist);
-    else
-      hipLaunchKernelGGL(k_radix_scatter<8>, dim3(kRadixG), dim3(256), 0, s, keys[cur], vals[cur], keys[cur ^ 1],
-                         vals[cur ^ 1], n_ptr, cap, shift, hist);
+    if (one_pass) {
+      if (kout)
+        hipLaunchKernelGGL((k_radix_scatter<11, true>), dim3(kRadixG), dim3(256), 0, s, keys[cur], vals[cur],
+                           keys[cur ^ 1], vals[cur ^ 1], n_ptr, cap, shift, hist, rg, nr);
+      else
+        hipLaunchKernelGGL((k_radix_scatter<11, false>), dim3(kRadixG), dim3(256), 0, s, keys[cur], vals[cur],
+                           keys[cur ^ 1], vals[cur ^ 1], n_ptr, cap, shift, hist, rg, nr);
+    } else {
+      if (kout)
+        hipLaunchKernelGGL((k_radix_scatter<8, true>), dim3(kRadixG), dim3(256), 0, s, keys[cur], vals[cur],
+                           keys[cur ^ 1], vals[cur ^ 1], n_ptr, cap, shift, hist, rg, nr);
+      else
+        hipLaunchKernelGGL((k_radix_scatter<8, false>), dim3(kRadixG), dim3(256), 0, s, keys[cur], vals[cur],
+                           keys[cur ^ 1], vals[cur ^ 1], n_ptr, cap, shift, hist, rg, nr);
+    }
     cur ^= 1;
   }
   return cur;
@@ -1323,8 +1350,9 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
       // 2 cap_tri / vals[1] + cap_tri), then the bin offsets in that order
       uint32_t* const dkeys[2] = {a.keys[1] + a.cap_tri, a.keys[1] + 2ull * a.cap_tri};
       uint32_t* const dvals[2] = {a.vals[1], a.vals[1] + a.cap_tri};
+      // (the sorted keys are not needed: their buffer takes the scan below)
       const int dc = enqueue_radix_sort(dkeys, dvals, &a.status->n_out, a.cap_tri, kDepthKeyBits, a.hist,
-                                        a.partials, s);
+                                        a.partials, s, false, nullptr, 0u);
       join(s, a, 3);
       uint32_t* order = dvals[dc];
       uint32_t* sorted_off = dkeys[dc];
@@ -1343,12 +1371,15 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_bin_emit_wide<true>, dim3(gw), dim3(256), 0, s, a.setup, a.views, a.status, wide_list,
                        a.TS, a.binoff, a.tile_start, a.tile_cur, a.cap_pairs, a.keys[0], a.vals[0]);
     // stable radix sort by tile key (one pass when the key has at most 11
-    // bits, else 8-bit passes), then the tile ranges
+    // bits, else 8-bit passes), then the tile ranges: from the one pass's
+    // scanned histogram, else from the sorted keys
+    const bool one_pass = a.key_bits <= 11;
     const int cur = enqueue_radix_sort(a.keys, a.vals, &a.status->n_pairs, a.cap_pairs, a.key_bits, a.hist,
-                                       a.partials, s);
+                                       a.partials, s, !one_pass, one_pass ? a.ranges : nullptr, a.n_tiles);
     b.vals[0] = a.vals[cur];
-    hipLaunchKernelGGL(k_tile_ranges, dim3(std::min<uint32_t>(cdiv(a.cap_pairs, 256), 4096u)), dim3(256), 0, s,
-                       a.keys[cur], &a.status->n_pairs, a.cap_pairs, a.ranges);
+    if (!one_pass)
+      hipLaunchKernelGGL(k_tile_ranges, dim3(std::min<uint32_t>(cdiv(a.cap_pairs, 256), 4096u)), dim3(256), 0, s,
+                         a.keys[cur], &a.status->n_pairs, a.cap_pairs, a.ranges);
   } else {
     hipLaunchKernelGGL(k_fill_u32, dim3(std::min<uint32_t>(cdiv(2ull * a.n_tiles, 256), 2048u)), dim3(256), 0, s,
                        a.tile_cnt, 0u, 2u * a.n_tiles);  // tile_cnt and tile_cur are adjacent
@@ -1386,7 +1417,7 @@ void enqueue_transparent_prep(const PipeArgs& opaque, const TransArgs& t, hipStr
     hipLaunchKernelGGL(k_tsort_keys, dim3(cdiv(tm.n_tri, 256)), dim3(256), 0, s, tm.tri, tm.n_tri, tm.pos,
                        t.cam[0], t.cam[1], t.cam[2], opaque.keys[0], opaque.vals[0]);
     const int cur = enqueue_radix_sort(opaque.keys, opaque.vals, t.sort_n, opaque.cap_pairs, 32, opaque.hist,
-                                       opaque.partials, s);
+                                       opaque.partials, s, false, nullptr, 0u);
     hipLaunchKernelGGL(k_permute_tris, dim3(cdiv(tm.n_tri, 256)), dim3(256), 0, s, tm.tri, opaque.vals[cur],
                        tm.n_tri, t.perm_tmp);
     (void)hipMemcpyAsync(tm.tri, t.perm_tmp, 36ull * tm.n_tri, hipMemcpyDeviceToDevice, s);
