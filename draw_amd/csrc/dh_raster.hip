@@ -171,32 +171,67 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
     wmax = wave_max_u32(m);
   };
 
+  // A record's 96 bytes as six uint4 (named values, not an array, so that
+  // a record held across the raster loop stays in registers).
+  auto load_rec = [&](uint32_t id, uint4& w0, uint4& w1, uint4& w2, uint4& w3, uint4& w4, uint4& w5) {
+    const uint4* src = reinterpret_cast<const uint4*>(setup + id);
+    w0 = src[0]; w1 = src[1]; w2 = src[2]; w3 = src[3]; w4 = src[4]; w5 = src[5];
+  };
+  // Record i of a chunk into LDS.
+  auto stage = [&](int i, uint32_t id, uint4 w0, uint4 w1, uint4 w2, uint4 w3, uint4 w4, uint4 w5) {
+    s_id[i] = id;
+    uint4* dst = reinterpret_cast<uint4*>(s_rec + i);
+    dst[0] = w0; dst[1] = w1; dst[2] = w2; dst[3] = w3; dst[4] = w4; dst[5] = w5;
+    // depth bound key, at least 1: only off-screen pixels hold key 0 (a
+    // fragment key of 0 is a NaN depth, which never passes the depth test)
+    s_hdr[i] = make_uint4(w5.x, w5.y, w5.z, max(depth_key(__uint_as_float(w4.w)), 1u));
+    auto pre = [&](int e, uint4 edge, uint32_t fbits) {
+      const float sg = __uint_as_float(fbits) > 0.0f ? 1.0f : -1.0f;
+      const float A = __uint_as_float(edge.x), B = __uint_as_float(edge.y);
+      const float C = __uint_as_float(edge.z), D = __uint_as_float(edge.w);
+      s_pre[i][e] = make_float4(sg * A, sg * B, sg * (C - D), fabsf(C) + fabsf(D));
+    };
+    pre(0, w0, w3.x);
+    pre(1, w1, w3.y);
+    pre(2, w2, w3.z);
+  };
+  // With one thread per staged record (CH <= NT <= 256) the loads run ahead
+  // of the raster: while a chunk is rasterized, the next chunk's records
+  // (their ids loaded a chunk earlier) and the ids of the one after are in
+  // flight in registers.
+  constexpr bool kPrefetch = NT >= CH && NT <= 256;
+  uint32_t pf_id = 0, pf_id2 = 0;  // ids of the next chunk and of the one after it
+  uint4 p0{}, p1{}, p2{}, p3{}, p4{}, p5{};  // the next chunk's record
+  if constexpr (kPrefetch) {
+    if (tid < CH && rg.x + (uint32_t)tid < rg.y) {
+      pf_id = vals[rg.x + tid];
+      load_rec(pf_id, p0, p1, p2, p3, p4, p5);
+    }
+    if (tid < CH && rg.x + CH + (uint32_t)tid < rg.y) pf_id2 = vals[rg.x + CH + tid];
+  }
+
   for (uint32_t base = rg.x; base < rg.y; base += CH) {
     const int n = (int)min((uint32_t)CH, rg.y - base);
     __syncthreads();
-    for (int i = tid; i < n; i += NT) {
-      const uint32_t id = vals[base + i];
-      s_id[i] = id;
-      const uint4* src = reinterpret_cast<const uint4*>(setup + id);
-      uint4* dst = reinterpret_cast<uint4*>(s_rec + i);
-      uint4 w[6];
-#pragma unroll
-      for (int q = 0; q < 6; q++) w[q] = src[q];
-#pragma unroll
-      for (int q = 0; q < 6; q++) dst[q] = w[q];
-      // depth bound key, at least 1: only off-screen pixels hold key 0 (a
-      // fragment key of 0 is a NaN depth, which never passes the depth test)
-      s_hdr[i] = make_uint4(w[5].x, w[5].y, w[5].z, max(depth_key(__uint_as_float(w[4].w)), 1u));
-      const float fs[3] = {__uint_as_float(w[3].x), __uint_as_float(w[3].y), __uint_as_float(w[3].z)};
-#pragma unroll
-      for (int e = 0; e < 3; e++) {
-        const float sg = fs[e] > 0.0f ? 1.0f : -1.0f;
-        const float A = __uint_as_float(w[e].x), B = __uint_as_float(w[e].y);
-        const float C = __uint_as_float(w[e].z), D = __uint_as_float(w[e].w);
-        s_pre[i][e] = make_float4(sg * A, sg * B, sg * (C - D), fabsf(C) + fabsf(D));
+    if constexpr (kPrefetch) {
+      if (tid < n) stage(tid, pf_id, p0, p1, p2, p3, p4, p5);
+    } else {
+      for (int i = tid; i < n; i += NT) {
+        const uint32_t id = vals[base + i];
+        uint4 w0, w1, w2, w3, w4, w5;
+        load_rec(id, w0, w1, w2, w3, w4, w5);
+        stage(i, id, w0, w1, w2, w3, w4, w5);
       }
     }
     __syncthreads();
+    if constexpr (kPrefetch) {
+      // the next chunk's records, the ids of the chunk after it
+      const uint32_t b1 = base + CH, b2 = base + 2u * CH;
+      pf_id = pf_id2;
+      if (tid < CH && b1 + (uint32_t)tid < rg.y) load_rec(pf_id, p0, p1, p2, p3, p4, p5);
+      pf_id2 = 0;
+      if (tid < CH && b2 + (uint32_t)tid < rg.y) pf_id2 = vals[b2 + tid];
+    }
     for (int g0 = 0; g0 < n; g0 += 64) {
       bool hit = false, wexact = false;
       float pA[3] = {0.0f, 0.0f, 0.0f}, pB[3] = {0.0f, 0.0f, 0.0f}, pE[3] = {0.0f, 0.0f, 0.0f};
