@@ -94,7 +94,7 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
   constexpr int CH = 128;    // records staged per round
   // fragment queue per wave: drained once it holds more than kQFlush entries,
   // checked before each triangle, which appends at most 64 per quadrant
-  constexpr int kQFlush = 192, QCAP = kQFlush + 64 * QPW;
+  constexpr int kQFlush = 256, QCAP = kQFlush + 64 * QPW;
   __shared__ SetupRec s_rec[CH];
   __shared__ uint4 s_hdr[CH];  // scan rectangle x, y; flags; depth-bound key
   __shared__ float4 s_pre[CH][3];  // per edge: s*A, s*B, s*(C-D), |C|+|D| with s = sign(f)
@@ -341,11 +341,9 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
           const bool inside = fminf(fminf(w0, w1), w2) >= 0.0f, nearer = zkey <= zk[qq];
           const bool cov = inside & nearer;
           const uint64_t m = __ballot(inside) & __ballot(nearer);
-          if (m) {
-            uint32_t* const qb = &s_q[wave][qn];
-            if (cov) qb[lanes_below(m, 0u)] = ((uint32_t)k << 16) | (uint32_t)lp[qq];
-            qn += (uint32_t)__popcll(m);
-          }
+          uint32_t* const qb = &s_q[wave][qn];
+          if (cov) qb[lanes_below(m, 0u)] = ((uint32_t)k << 16) | (uint32_t)lp[qq];
+          qn += (uint32_t)__popcll(m);
         }
       }
       while (hs) {
