@@ -44,6 +44,7 @@ struct ObjRes {
 struct MeshRes {
   dh_handle obj = 0;
   uint32_t* tri = nullptr;
+  float* soa = nullptr;  // per-triangle vertex streams (opaque material), or null
   uint32_t n_tri = 0;
   dh_handle mat = 0;
 };
@@ -300,7 +301,10 @@ ResPool::~ResPool() {
     hipFree(kv.second.nrm);
     hipFree(kv.second.uv);
   }
-  for (auto& kv : meshes) hipFree(kv.second.tri);
+  for (auto& kv : meshes) {
+    hipFree(kv.second.tri);
+    if (kv.second.soa) hipFree(kv.second.soa);
+  }
   for (auto& kv : canvases) {
     hipFree(kv.second.bgra);
     hipFree(kv.second.depth);
@@ -541,6 +545,24 @@ extern "C" int dh_mesh_create(dh_ctx c, dh_handle object, const uint32_t* tri, u
   m.mat = material;
   HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&m.tri), std::max<size_t>(1, 9ull * n_tri) * 4));
   if (n_tri) HIPCHK(c, hipMemcpy(m.tri, tri, 36ull * n_tri, hipMemcpyHostToDevice));
+  // Opaque meshes (their triangle order never changes) get per-triangle vertex
+  // streams for the geometry kernels; transparent ones are re-sorted every
+  // frame and keep reading through their index triplets.
+  const bool opaque = !material || c->res->mats.at(material & ((1ull << 56) - 1)).alpha >= 1.0f;
+  if (opaque && n_tri) {
+    const hipError_t e = hipMalloc(reinterpret_cast<void**>(&m.soa), (size_t)dh::kMeshStreams * 4ull * n_tri);
+    if (e != hipSuccess) {
+      hipFree(m.tri);
+      return hip_fail(c, e, "hipMalloc(mesh streams)");
+    }
+    dh::enqueue_mesh_streams(m.tri, n_tri, o.pos, o.nrm, o.uv, m.soa, c->stream);
+    const hipError_t e2 = hipStreamSynchronize(c->stream);
+    if (e2 != hipSuccess) {
+      hipFree(m.soa);
+      hipFree(m.tri);
+      return hip_fail(c, e2, "mesh streams");
+    }
+  }
   const uint64_t id = c->res->next_id++;
   c->res->meshes[id] = m;
   *out = make_handle(kMesh, id);
@@ -612,6 +634,7 @@ extern "C" int dh_release(dh_ctx c, dh_handle h) {
       auto it = c->res->meshes.find(id);
       if (it == c->res->meshes.end()) break;
       hipFree(it->second.tri);
+      if (it->second.soa) hipFree(it->second.soa);
       c->res->meshes.erase(it);
       return DH_OK;
     }
@@ -715,6 +738,7 @@ int build_tables(dh_ctx c, const dh_view* views, uint32_t n_views, bool allow_tr
       d.nrm = ot->second.nrm;
       d.uv = ot->second.uv;
       d.tri = m.tri;
+      d.soa = alpha < 1.0f ? nullptr : m.soa;
       d.n_tri = m.n_tri;
       d.material = slot;
       d.view = v;

@@ -55,12 +55,19 @@ struct DevMaterial {
 
 // One draw = one IndexedMesh of one Object, in submission order.
 constexpr uint32_t kDrawNoCull = 1u;
+// Per-triangle vertex streams of an opaque mesh, built once when the mesh is
+// created: stream s of triangle t at soa[s * n_tri + t]; streams 0-8 hold the
+// positions of vertices a, b, c (x, y, z each), 9-17 their normals, 18-23
+// their texture coordinates (u, v each).  The geometry kernels read each value
+// with one coalesced load instead of gathering through the index triplets.
+constexpr uint32_t kMeshStreams = 24;
 
 struct DevDraw {
   const float* pos;       // xyz
   const float* nrm;       // xyz
   const float* uv;        // uv
   const uint32_t* tri;    // 9 per triangle (pos a,b,c; uv a,b,c; nrm a,b,c)
+  const float* soa;       // opaque meshes: per-triangle vertex streams (kMeshStreams x n_tri), else null
   uint32_t n_tri;
   uint32_t tri_begin;     // first input-triangle index of this draw
   uint32_t material;      // slot in the material table

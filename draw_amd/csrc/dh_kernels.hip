@@ -55,6 +55,69 @@ __device__ inline V3 ld3(const float* a, uint32_t i) {
   return v3(p[0], p[1], p[2]);
 }
 
+// Vertex positions of a draw's triangle lt (a, b, c): from the mesh's
+// per-triangle streams when it has them, else through its index triplet.
+__device__ inline void tri_positions(const DevDraw& dr, uint32_t lt, V3& a, V3& b, V3& c) {
+  if (dr.soa) {
+    const uint64_t n = dr.n_tri;
+    const float* s = dr.soa + lt;
+    a = v3(s[0], s[n], s[2 * n]);
+    b = v3(s[3 * n], s[4 * n], s[5 * n]);
+    c = v3(s[6 * n], s[7 * n], s[8 * n]);
+  } else {
+    const uint32_t* ti = dr.tri + 9ull * lt;
+    a = ld3(dr.pos, ti[0]);
+    b = ld3(dr.pos, ti[1]);
+    c = ld3(dr.pos, ti[2]);
+  }
+}
+
+// Normals and texture coordinates of the vertices of triangle lt.
+__device__ inline void tri_attributes(const DevDraw& dr, uint32_t lt, V3 (&nrm)[3], float (&u)[3], float (&v)[3]) {
+  if (dr.soa) {
+    const uint64_t n = dr.n_tri;
+    const float* s = dr.soa + lt;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      nrm[k] = v3(s[(9 + 3 * k) * n], s[(10 + 3 * k) * n], s[(11 + 3 * k) * n]);
+      u[k] = s[(18 + 2 * k) * n];
+      v[k] = s[(19 + 2 * k) * n];
+    }
+  } else {
+    const uint32_t* ti = dr.tri + 9ull * lt;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      nrm[k] = ld3(dr.nrm, ti[6 + k]);
+      const float* uv = dr.uv + 2ull * ti[3 + k];
+      u[k] = uv[0];
+      v[k] = uv[1];
+    }
+  }
+}
+
+// Builds a mesh's per-triangle vertex streams (once, when the mesh is created).
+__global__ __launch_bounds__(256) void k_mesh_streams(const uint32_t* __restrict__ tri, uint32_t n_tri,
+                                                      const float* __restrict__ pos, const float* __restrict__ nrm,
+                                                      const float* __restrict__ uv, float* __restrict__ soa) {
+  const uint64_t n = n_tri;
+  for (uint32_t t = blockIdx.x * 256u + threadIdx.x; t < n_tri; t += gridDim.x * 256u) {
+    const uint32_t* q = tri + 9ull * t;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const float* p = pos + 3ull * q[k];
+      const float* m = nrm + 3ull * q[6 + k];
+      const float* w = uv + 2ull * q[3 + k];
+#pragma unroll
+      for (int j = 0; j < 3; j++) {
+        soa[(uint64_t)(3 * k + j) * n + t] = p[j];
+        soa[(uint64_t)(9 + 3 * k + j) * n + t] = m[j];
+      }
+      soa[(uint64_t)(18 + 2 * k) * n + t] = w[0];
+      soa[(uint64_t)(19 + 2 * k) * n + t] = w[1];
+    }
+  }
+}
+
 // Block of 256 threads (4 waves): exclusive scan of one value per thread.
 __device__ inline uint32_t block_exclusive_scan256(uint32_t v, uint32_t* s_tmp /*[4]*/, uint32_t* total) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -573,11 +636,8 @@ __global__ __launch_bounds__(256) void k_geom_count(const DevDraw* __restrict__ 
   if (t < n_in) {
     const DevDraw& dr = draws[find_draw(draws, n_draws, t)];
     const FrameParamsPod& fr = views[dr.view].fr;
-    const uint32_t* ti = dr.tri + 9ull * (t - dr.tri_begin);
     PV a, b, cc;
-    a.p = ld3(dr.pos, ti[0]);
-    b.p = ld3(dr.pos, ti[1]);
-    cc.p = ld3(dr.pos, ti[2]);
+    tri_positions(dr, t - dr.tri_begin, a.p, b.p, cc.p);
     c = (uint32_t)cull_and_clip(fr, a, b, cc, !(dr.flags & kDrawNoCull), [](int, const PV&, const PV&, const PV&) {});
     // ViewPlane::clip keeps a triangle unchanged exactly when its three
     // vertices are strictly inside the plane (clip_plane's first branch)
@@ -635,13 +695,17 @@ __global__ __launch_bounds__(256) void k_geom_emit(const DevDraw* __restrict__ d
   const uint32_t t = sv.x, base = sv.y;
   const DevDraw& dr = draws[find_draw(draws, n_draws, t)];
   const FrameParamsPod& fr = views[dr.view].fr;
-  const uint32_t* ti = dr.tri + 9ull * (t - dr.tri_begin);
+  const uint32_t lt = t - dr.tri_begin;
   const V3 cam = v3(fr.cam[0], fr.cam[1], fr.cam[2]);
   const V3 L = v3(fr.light[0], fr.light[1], fr.light[2]);
   FV v[3];
+  V3 P[3], N[3];
+  float U[3], W[3];
+  tri_positions(dr, lt, P[0], P[1], P[2]);
+  tri_attributes(dr, lt, N, U, W);
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    const V3 p = ld3(dr.pos, ti[k]);
+    const V3 p = P[k];
     v[k].p = p;
     // per-vertex visual info, scene/mod.rs:920-925
     const V3 eye_dir = sub(p, cam);
@@ -650,10 +714,9 @@ __global__ __launch_bounds__(256) void k_geom_emit(const DevDraw* __restrict__ d
     v[k].d = norm(eye_dir);
     v[k].l = light;
     v[k].h = normalized(add(light, eye));
-    v[k].n = ld3(dr.nrm, ti[6 + k]);
-    const float* uv = dr.uv + 2ull * ti[3 + k];
-    v[k].u = uv[0];
-    v[k].v = uv[1];
+    v[k].n = N[k];
+    v[k].u = U[k];
+    v[k].v = W[k];
   }
   const uint32_t material = dr.material, view = dr.view, group = dr.group;
   auto out = [&](int k, const FV& A, const FV& B, const FV& C) {
@@ -1207,6 +1270,13 @@ __global__ __launch_bounds__(256) void k_permute_tris(const uint32_t* __restrict
 
 // ============================================================= orchestration
 static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+void enqueue_mesh_streams(const uint32_t* tri, uint32_t n_tri, const float* pos, const float* nrm, const float* uv,
+                          float* soa, hipStream_t s) {
+  if (n_tri == 0) return;
+  hipLaunchKernelGGL(k_mesh_streams, dim3(std::min<uint32_t>(cdiv(n_tri, 256), 65535u)), dim3(256), 0, s, tri, n_tri,
+                     pos, nrm, uv, soa);
+}
 
 // Stable LSD radix sort of (keys[0], vals[0]) by the low key_bits bits of the
 // keys, *n_ptr items (at most cap): one 8-bit pass when key_bits <= 8, one
