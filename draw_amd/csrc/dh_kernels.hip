@@ -681,16 +681,12 @@ __global__ __launch_bounds__(256) void k_geom_compact(uint32_t n_in, const uint8
 // triangles take no lane).  The unclipped list (CLIP false) skips the
 // clipper, whose attribute registers would otherwise size the whole kernel.
 template <bool BIN, bool CLIP>
-__global__ __launch_bounds__(256) void k_geom_emit(const DevDraw* __restrict__ draws, uint32_t n_draws,
-                                                   const DevView* __restrict__ views,
-                                                   const uint2* __restrict__ surv,
-                                                   const DevMaterial* __restrict__ mats, SetupRec* __restrict__ setup,
-                                                   ShadeRec* __restrict__ shade, uint32_t cap_tri,
-                                                   Status* __restrict__ status, int TS, uint32_t* __restrict__ bincnt,
-                                                   uint8_t* __restrict__ wflag, uint32_t* __restrict__ dkey,
-                                                   uint32_t* __restrict__ dval) {
-  const uint32_t j = blockIdx.x * 256u + threadIdx.x;
-  if (j >= (CLIP ? status->n_clip : status->n_simple)) return;
+__device__ inline void geom_emit_one(uint32_t j, const DevDraw* __restrict__ draws, uint32_t n_draws,
+                                     const DevView* __restrict__ views, const uint2* __restrict__ surv,
+                                     const DevMaterial* __restrict__ mats, SetupRec* __restrict__ setup,
+                                     ShadeRec* __restrict__ shade, uint32_t cap_tri, Status* __restrict__ status,
+                                     int TS, uint32_t* __restrict__ bincnt, uint8_t* __restrict__ wflag,
+                                     uint32_t* __restrict__ dkey) {
   const uint2 sv = surv[CLIP ? status->n_simple + j : j];
   const uint32_t t = sv.x, base = sv.y;
   const DevDraw& dr = draws[find_draw(draws, n_draws, t)];
@@ -728,15 +724,28 @@ __global__ __launch_bounds__(256) void k_geom_emit(const DevDraw* __restrict__ d
     SetupRec rec;
     make_setup(fr, A, B, C, material, mats[material], view, group, setup + idx, shade + idx, rec);
     if (BIN) bin_count_one(rec, TS, idx, bincnt, wflag);
-    if (dkey) {
-      dkey[idx] = depth_order_key(rec, views[view].depth_base);
-      dval[idx] = idx;
-    }
+    if (dkey) dkey[idx] = depth_order_key(rec, views[view].depth_base);
   };
   if constexpr (CLIP)
     cull_and_clip(fr, v[0], v[1], v[2], !(dr.flags & kDrawNoCull), out);
   else
     out(0, v[0], v[1], v[2]);  // passed the back-face and lateral tests in k_geom_count
+}
+
+// A grid-stride loop over one list (its length is on the device), so that the
+// grid can be capped well below one thread per input triangle.
+template <bool BIN, bool CLIP>
+__global__ __launch_bounds__(256) void k_geom_emit(const DevDraw* __restrict__ draws, uint32_t n_draws,
+                                                   const DevView* __restrict__ views,
+                                                   const uint2* __restrict__ surv,
+                                                   const DevMaterial* __restrict__ mats, SetupRec* __restrict__ setup,
+                                                   ShadeRec* __restrict__ shade, uint32_t cap_tri,
+                                                   Status* __restrict__ status, int TS, uint32_t* __restrict__ bincnt,
+                                                   uint8_t* __restrict__ wflag, uint32_t* __restrict__ dkey) {
+  const uint32_t n = CLIP ? status->n_clip : status->n_simple;
+  for (uint32_t j = blockIdx.x * 256u + threadIdx.x; j < n; j += gridDim.x * 256u)
+    geom_emit_one<BIN, CLIP>(j, draws, n_draws, views, surv, mats, setup, shade, cap_tri, status, TS, bincnt, wflag,
+                             dkey);
 }
 
 // First clipped-triangle id of each view: the exclusive prefix of the clip
@@ -1136,7 +1145,7 @@ __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* __restrict__
 }
 
 // KEYS_OUT false: only the values are written (a last pass whose sorted keys
-// nobody reads).  ranges (may be null): [start, end) of each of the first
+// nobody reads).  vin null: the values are the input positions.  ranges (may be null): [start, end) of each of the first
 // n_ranges digits in the output, straight from the scanned histogram -- the
 // per-tile lists when this pass sorts by the whole key.
 template <int BITS, bool KEYS_OUT>
@@ -1165,7 +1174,7 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
     for (int u = 0; u < kRadixU; u++) {
       const uint32_t i = tb + 64u * (uint32_t)u + (uint32_t)lane;
       kk[u] = i < b1 ? kin[i] : 0u;
-      vv[u] = i < b1 ? vin[i] : 0u;
+      vv[u] = i < b1 ? (vin ? vin[i] : i) : 0u;
     }
 #pragma unroll
     for (int u = 0; u < kRadixU; u++) {
@@ -1279,7 +1288,8 @@ void enqueue_mesh_streams(const uint32_t* tri, uint32_t n_tri, const float* pos,
 }
 
 // Stable LSD radix sort of (keys[0], vals[0]) by the low key_bits bits of the
-// keys, *n_ptr items (at most cap): one 8-bit pass when key_bits <= 8, one
+// keys (vals[0] may be null: the values are then the input positions), *n_ptr
+// items (at most cap): one 8-bit pass when key_bits <= 8, one
 // 11-bit pass when key_bits <= 11, else
 // 8-bit passes.  Returns the index of the buffer pair holding the result.
 // keys_out false: the sorted keys are not written (the values are).  ranges:
@@ -1374,22 +1384,22 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
     // records, and for near-first lists each record's depth key
     uint8_t* wf = reinterpret_cast<uint8_t*>(a.big);
     uint32_t* dk = depth_sorted ? a.keys[1] + a.cap_tri : nullptr;
-    uint32_t* dv = depth_sorted ? a.vals[1] : nullptr;
     // the unclipped and clipped emits write disjoint outputs: the clipped
-    // one runs beside the other on the side stream
+    // one runs beside the other on the side stream.  Capped grids (the lists'
+    // lengths are on the device): 2048 and 1024 blocks, a grid-stride loop
+    // beyond that.
+    const uint32_t g_simple = std::min<uint32_t>(n_blk, 2048u), g_clip = std::min<uint32_t>(n_blk, 1024u);
     const hipStream_t s2 = fork(s, a, 0);
     if (a.ordered_bins) {
-      hipLaunchKernelGGL((k_geom_emit<true, false>), dim3(n_blk), dim3(256), 0, s, a.draws, a.n_draws, a.views,
-                         a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, a.bincnt, wf, dk, dv);
-      hipLaunchKernelGGL((k_geom_emit<true, true>), dim3(n_blk), dim3(256), 0, s2, a.draws, a.n_draws, a.views,
-                         a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, a.bincnt, wf, dk, dv);
+      hipLaunchKernelGGL((k_geom_emit<true, false>), dim3(g_simple), dim3(256), 0, s, a.draws, a.n_draws, a.views,
+                         a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, a.bincnt, wf, dk);
+      hipLaunchKernelGGL((k_geom_emit<true, true>), dim3(g_clip), dim3(256), 0, s2, a.draws, a.n_draws, a.views,
+                         a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, a.bincnt, wf, dk);
     } else {
-      hipLaunchKernelGGL((k_geom_emit<false, false>), dim3(n_blk), dim3(256), 0, s, a.draws, a.n_draws, a.views,
-                         a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, nullptr, nullptr, nullptr,
-                         nullptr);
-      hipLaunchKernelGGL((k_geom_emit<false, true>), dim3(n_blk), dim3(256), 0, s2, a.draws, a.n_draws, a.views,
-                         a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, nullptr, nullptr, nullptr,
-                         nullptr);
+      hipLaunchKernelGGL((k_geom_emit<false, false>), dim3(g_simple), dim3(256), 0, s, a.draws, a.n_draws, a.views,
+                         a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, nullptr, nullptr, nullptr);
+      hipLaunchKernelGGL((k_geom_emit<false, true>), dim3(g_clip), dim3(256), 0, s2, a.draws, a.n_draws, a.views,
+                         a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, nullptr, nullptr, nullptr);
     }
     join(s, a, 1);
   }
@@ -1419,8 +1429,9 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
       // 8-bit pass over keys[1] + cap_tri / vals[1], results in keys[1] +
       // 2 cap_tri / vals[1] + cap_tri), then the bin offsets in that order
       uint32_t* const dkeys[2] = {a.keys[1] + a.cap_tri, a.keys[1] + 2ull * a.cap_tri};
-      uint32_t* const dvals[2] = {a.vals[1], a.vals[1] + a.cap_tri};
-      // (the sorted keys are not needed: their buffer takes the scan below)
+      // (the values start as the triangle ids themselves: no input array; the
+      // sorted keys are not needed: their buffer takes the scan below)
+      uint32_t* const dvals[2] = {nullptr, a.vals[1] + a.cap_tri};
       const int dc = enqueue_radix_sort(dkeys, dvals, &a.status->n_out, a.cap_tri, kDepthKeyBits, a.hist,
                                         a.partials, s, false, nullptr, 0u);
       join(s, a, 3);
