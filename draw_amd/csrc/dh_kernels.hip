@@ -1390,15 +1390,17 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
     // beyond that.
     const uint32_t g_simple = std::min<uint32_t>(n_blk, 2048u), g_clip = std::min<uint32_t>(n_blk, 1024u);
     const hipStream_t s2 = fork(s, a, 0);
+    // (the clipped emit, the longer one per thread, starts first on the main
+    // stream; the unclipped one waits for the fork on the side stream)
     if (a.ordered_bins) {
-      hipLaunchKernelGGL((k_geom_emit<true, false>), dim3(g_simple), dim3(256), 0, s, a.draws, a.n_draws, a.views,
+      hipLaunchKernelGGL((k_geom_emit<true, true>), dim3(g_clip), dim3(256), 0, s, a.draws, a.n_draws, a.views,
                          a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, a.bincnt, wf, dk);
-      hipLaunchKernelGGL((k_geom_emit<true, true>), dim3(g_clip), dim3(256), 0, s2, a.draws, a.n_draws, a.views,
+      hipLaunchKernelGGL((k_geom_emit<true, false>), dim3(g_simple), dim3(256), 0, s2, a.draws, a.n_draws, a.views,
                          a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, a.bincnt, wf, dk);
     } else {
-      hipLaunchKernelGGL((k_geom_emit<false, false>), dim3(g_simple), dim3(256), 0, s, a.draws, a.n_draws, a.views,
+      hipLaunchKernelGGL((k_geom_emit<false, true>), dim3(g_clip), dim3(256), 0, s, a.draws, a.n_draws, a.views,
                          a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, nullptr, nullptr, nullptr);
-      hipLaunchKernelGGL((k_geom_emit<false, true>), dim3(g_clip), dim3(256), 0, s2, a.draws, a.n_draws, a.views,
+      hipLaunchKernelGGL((k_geom_emit<false, false>), dim3(g_simple), dim3(256), 0, s2, a.draws, a.n_draws, a.views,
                          a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, nullptr, nullptr, nullptr);
     }
     join(s, a, 1);
