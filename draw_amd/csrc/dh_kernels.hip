@@ -285,8 +285,8 @@ __device__ inline void rect_axis(uint64_t a0, uint64_t a1, uint64_t smax, uint32
 // tie-break products (660-666) and the scan rectangle (640-658).  Writes the
 // SetupRec and ShadeRec of one clipped triangle with 16-byte stores.
 __device__ inline void make_setup(const FrameParamsPod& fr, const FV& A, const FV& B, const FV& C, uint32_t material,
-                                  const DevMaterial& mat, uint32_t view, uint32_t group, SetupRec* __restrict__ rout,
-                                  ShadeRec* __restrict__ sout, SetupRec& rec) {
+                                  const DevMaterial& mat, uint32_t view, uint32_t group, uint4 (&ro)[6],
+                                  uint4 (&so)[9], SetupRec& rec) {
   const float* M = fr.M;
   float X[3], Y[3];
   const V3 P[3] = {A.p, B.p, C.p};
@@ -392,21 +392,50 @@ __device__ inline void make_setup(const FrameParamsPod& fr, const FV& A, const F
   rec.ry = y0 | (y1 << 16);
   rec.flags = flags;
   rec.view = view;
-  float4* ro = reinterpret_cast<float4*>(rout);
-  const float4* ri = reinterpret_cast<const float4*>(&rec);
+  const uint4* ri = reinterpret_cast<const uint4*>(&rec);
 #pragma unroll
   for (int k = 0; k < 6; k++) ro[k] = ri[k];
   // shading attributes: uv[6], material, pad, n[9], l[9], h[9], pad
-  float4* so = reinterpret_cast<float4*>(sout);
-  so[0] = make_float4(A.u, A.v, B.u, B.v);
-  so[1] = make_float4(C.u, C.v, __uint_as_float(material), __uint_as_float(group));
-  so[2] = make_float4(A.n.x, A.n.y, A.n.z, B.n.x);
-  so[3] = make_float4(B.n.y, B.n.z, C.n.x, C.n.y);
-  so[4] = make_float4(C.n.z, A.l.x, A.l.y, A.l.z);
-  so[5] = make_float4(B.l.x, B.l.y, B.l.z, C.l.x);
-  so[6] = make_float4(C.l.y, C.l.z, A.h.x, A.h.y);
-  so[7] = make_float4(A.h.z, B.h.x, B.h.y, B.h.z);
-  so[8] = make_float4(C.h.x, C.h.y, C.h.z, 0.0f);
+  auto u4 = [](float a, float b, float c, float d) {
+    return make_uint4(__float_as_uint(a), __float_as_uint(b), __float_as_uint(c), __float_as_uint(d));
+  };
+  so[0] = u4(A.u, A.v, B.u, B.v);
+  so[1] = u4(C.u, C.v, __uint_as_float(material), __uint_as_float(group));
+  so[2] = u4(A.n.x, A.n.y, A.n.z, B.n.x);
+  so[3] = u4(B.n.y, B.n.z, C.n.x, C.n.y);
+  so[4] = u4(C.n.z, A.l.x, A.l.y, A.l.z);
+  so[5] = u4(B.l.x, B.l.y, B.l.z, C.l.x);
+  so[6] = u4(C.l.y, C.l.z, A.h.x, A.h.y);
+  so[7] = u4(A.h.z, B.h.x, B.h.y, B.h.z);
+  so[8] = u4(C.h.x, C.h.y, C.h.z, 0.0f);
+}
+
+// Stores one N-uint4 record per lane at dst[idx] (record units) for the lanes
+// with ok.  When those lanes are a prefix of the wave with consecutive idx (a
+// dense run of output ids), the records go through the wave's LDS area
+// stage[64 * N] and out in coalesced 16-byte lanes; else each lane stores its
+// own.  Every lane of the wave must call it.
+template <int N>
+__device__ inline void wave_store(uint4* stage, const uint4 (&v)[N], bool ok, uint32_t idx, uint4* dst) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t m = __ballot(ok);
+  if (!m) return;
+  const uint32_t cnt = (uint32_t)__popcll(m);
+  const uint32_t first = (uint32_t)__builtin_amdgcn_readfirstlane((int)idx);
+  const bool prefix = m == (cnt == 64u ? ~0ull : ((1ull << cnt) - 1ull));
+  if (prefix && __ballot(ok && idx != first + lane) == 0ull) {
+    if (ok) {
+#pragma unroll
+      for (int q = 0; q < N; q++) stage[lane * N + q] = v[q];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    uint4* out = dst + (uint64_t)first * N;
+    for (uint32_t k = lane; k < cnt * N; k += 64u) out[k] = stage[k];
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  } else if (ok) {
+#pragma unroll
+    for (int q = 0; q < N; q++) dst[(uint64_t)idx * N + q] = v[q];
+  }
 }
 
 // =================================================================== binning
@@ -680,14 +709,14 @@ __global__ __launch_bounds__(256) void k_geom_compact(uint32_t n_in, const uint8
 // One thread per emitting input triangle of one list (dense: culled
 // triangles take no lane).  The unclipped list (CLIP false) skips the
 // clipper, whose attribute registers would otherwise size the whole kernel.
-template <bool BIN, bool CLIP>
-__device__ inline void geom_emit_one(uint32_t j, const DevDraw* __restrict__ draws, uint32_t n_draws,
-                                     const DevView* __restrict__ views, const uint2* __restrict__ surv,
-                                     const DevMaterial* __restrict__ mats, SetupRec* __restrict__ setup,
-                                     ShadeRec* __restrict__ shade, uint32_t cap_tri, Status* __restrict__ status,
-                                     int TS, uint32_t* __restrict__ bincnt, uint8_t* __restrict__ wflag,
-                                     uint32_t* __restrict__ dkey) {
-  const uint2 sv = surv[CLIP ? status->n_simple + j : j];
+template <bool BIN>
+__device__ inline void geom_emit_clipped(uint32_t j, const DevDraw* __restrict__ draws, uint32_t n_draws,
+                                         const DevView* __restrict__ views, const uint2* __restrict__ surv,
+                                         const DevMaterial* __restrict__ mats, SetupRec* __restrict__ setup,
+                                         ShadeRec* __restrict__ shade, uint32_t cap_tri, Status* __restrict__ status,
+                                         int TS, uint32_t* __restrict__ bincnt, uint8_t* __restrict__ wflag,
+                                         uint32_t* __restrict__ dkey) {
+  const uint2 sv = surv[status->n_simple + j];
   const uint32_t t = sv.x, base = sv.y;
   const DevDraw& dr = draws[find_draw(draws, n_draws, t)];
   const FrameParamsPod& fr = views[dr.view].fr;
@@ -722,18 +751,78 @@ __device__ inline void geom_emit_one(uint32_t j, const DevDraw* __restrict__ dra
       return;
     }
     SetupRec rec;
-    make_setup(fr, A, B, C, material, mats[material], view, group, setup + idx, shade + idx, rec);
+    uint4 ro[6], so[9];
+    make_setup(fr, A, B, C, material, mats[material], view, group, ro, so, rec);
+#pragma unroll
+    for (int q = 0; q < 6; q++) reinterpret_cast<uint4*>(setup + idx)[q] = ro[q];
+#pragma unroll
+    for (int q = 0; q < 9; q++) reinterpret_cast<uint4*>(shade + idx)[q] = so[q];
     if (BIN) bin_count_one(rec, TS, idx, bincnt, wflag);
     if (dkey) dkey[idx] = depth_order_key(rec, views[view].depth_base);
   };
-  if constexpr (CLIP)
-    cull_and_clip(fr, v[0], v[1], v[2], !(dr.flags & kDrawNoCull), out);
-  else
-    out(0, v[0], v[1], v[2]);  // passed the back-face and lateral tests in k_geom_count
+  cull_and_clip(fr, v[0], v[1], v[2], !(dr.flags & kDrawNoCull), out);
+}
+
+// One unclipped list entry per lane (every lane of the wave calls it; lanes
+// past the list end with active false): one record each at the entry's output
+// id, stored through the wave's LDS area stage[64 * 9].
+template <bool BIN>
+__device__ inline void geom_emit_simple(uint32_t j, bool active, uint4* stage, const DevDraw* __restrict__ draws,
+                                        uint32_t n_draws, const DevView* __restrict__ views,
+                                        const uint2* __restrict__ surv, const DevMaterial* __restrict__ mats,
+                                        SetupRec* __restrict__ setup, ShadeRec* __restrict__ shade, uint32_t cap_tri,
+                                        Status* __restrict__ status, int TS, uint32_t* __restrict__ bincnt,
+                                        uint8_t* __restrict__ wflag, uint32_t* __restrict__ dkey) {
+  uint4 ro[6], so[9];
+  uint32_t idx = 0;
+  bool ok = false;
+  if (active) {
+    const uint2 sv = surv[j];
+    const uint32_t t = sv.x;
+    idx = sv.y;
+    const DevDraw& dr = draws[find_draw(draws, n_draws, t)];
+    const FrameParamsPod& fr = views[dr.view].fr;
+    const uint32_t lt = t - dr.tri_begin;
+    const V3 cam = v3(fr.cam[0], fr.cam[1], fr.cam[2]);
+    const V3 L = v3(fr.light[0], fr.light[1], fr.light[2]);
+    FV v[3];
+    V3 P[3], N[3];
+    float U[3], W[3];
+    tri_positions(dr, lt, P[0], P[1], P[2]);
+    tri_attributes(dr, lt, N, U, W);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      const V3 p = P[k];
+      v[k].p = p;
+      // per-vertex visual info, scene/mod.rs:920-925
+      const V3 eye_dir = sub(p, cam);
+      const V3 light = normalized(sub(p, L));
+      const V3 eye = normalized(eye_dir);
+      v[k].d = norm(eye_dir);
+      v[k].l = light;
+      v[k].h = normalized(add(light, eye));
+      v[k].n = N[k];
+      v[k].u = U[k];
+      v[k].v = W[k];
+    }
+    if (idx >= cap_tri) {
+      atomicOr(&status->overflow, 1u);
+    } else {
+      // passed the back-face and lateral tests in k_geom_count
+      SetupRec rec;
+      make_setup(fr, v[0], v[1], v[2], dr.material, mats[dr.material], dr.view, dr.group, ro, so, rec);
+      ok = true;
+      if (BIN) bin_count_one(rec, TS, idx, bincnt, wflag);
+      if (dkey) dkey[idx] = depth_order_key(rec, views[dr.view].depth_base);
+    }
+  }
+  wave_store<6>(stage, ro, ok, idx, reinterpret_cast<uint4*>(setup));
+  wave_store<9>(stage, so, ok, idx, reinterpret_cast<uint4*>(shade));
 }
 
 // A grid-stride loop over one list (its length is on the device), so that the
-// grid can be capped well below one thread per input triangle.
+// grid can be capped well below one thread per input triangle.  The unclipped
+// list goes a wave at a time (its records leave through LDS).
 template <bool BIN, bool CLIP>
 __global__ __launch_bounds__(256) void k_geom_emit(const DevDraw* __restrict__ draws, uint32_t n_draws,
                                                    const DevView* __restrict__ views,
@@ -742,10 +831,19 @@ __global__ __launch_bounds__(256) void k_geom_emit(const DevDraw* __restrict__ d
                                                    ShadeRec* __restrict__ shade, uint32_t cap_tri,
                                                    Status* __restrict__ status, int TS, uint32_t* __restrict__ bincnt,
                                                    uint8_t* __restrict__ wflag, uint32_t* __restrict__ dkey) {
-  const uint32_t n = CLIP ? status->n_clip : status->n_simple;
-  for (uint32_t j = blockIdx.x * 256u + threadIdx.x; j < n; j += gridDim.x * 256u)
-    geom_emit_one<BIN, CLIP>(j, draws, n_draws, views, surv, mats, setup, shade, cap_tri, status, TS, bincnt, wflag,
+  if constexpr (CLIP) {
+    const uint32_t n = status->n_clip;
+    for (uint32_t j = blockIdx.x * 256u + threadIdx.x; j < n; j += gridDim.x * 256u)
+      geom_emit_clipped<BIN>(j, draws, n_draws, views, surv, mats, setup, shade, cap_tri, status, TS, bincnt, wflag,
                              dkey);
+  } else {
+    __shared__ uint4 s_stage[4][64 * 9];
+    const uint32_t n = status->n_simple;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    for (uint32_t j0 = blockIdx.x * 256u + wave * 64u; j0 < n; j0 += gridDim.x * 256u)
+      geom_emit_simple<BIN>(j0 + lane, j0 + lane < n, s_stage[wave], draws, n_draws, views, surv, mats, setup, shade,
+                            cap_tri, status, TS, bincnt, wflag, dkey);
+  }
 }
 
 // First clipped-triangle id of each view: the exclusive prefix of the clip
