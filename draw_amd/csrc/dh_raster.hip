@@ -623,18 +623,61 @@ __device__ inline uint32_t shade_pixel(const ShadeRec& s, const DevMaterial& m, 
   return f2u8(sc.z) | (f2u8(sc.y) << 8) | (f2u8(sc.x) << 16) | (255u << 24);
 }
 
+// Records of the runs of equal winners along a wave's 64 pixels are loaded
+// once per run into LDS (up to kShadeRuns runs; more take per-lane loads):
+// the texture address units handle bytes per lane, and a run of pixels
+// shares one triangle's 224 record bytes.  The material is read with scalar
+// loads when the whole wave has one.
+constexpr int kShadeRuns = 16;
+
 __global__ __launch_bounds__(256) void k_shade(const DevView* __restrict__ views, uint32_t n_views, uint32_t n_pixels,
                                                const uint32_t* __restrict__ vis, const SetupRec* __restrict__ setup,
                                                const ShadeRec* __restrict__ shade,
                                                const DevMaterial* __restrict__ mats, Status* __restrict__ status) {
+  __shared__ uint4 s_run[4][kShadeRuns][14];  // setup bytes 0-79, then the 144 shade bytes
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const uint32_t g = xcd_remap(blockIdx.x, gridDim.x) * 256u + threadIdx.x;
-  if (g >= n_pixels) return;
+  const bool in = g < n_pixels;
+  const uint32_t id = in ? vis[g] : kSkip;
+  const bool covered = id != kSkip && id != kNone;
+  // runs: a covered lane starts one unless the lane before holds the same id
+  const uint32_t prev = (uint32_t)__shfl_up((int)id, 1, 64);
+  const bool start = covered && (lane == 0u || prev != id);
+  const uint64_t sm = __ballot(start);
+  SetupRec r;
+  ShadeRec sh;
+  if (__popcll(sm) <= kShadeRuns) {
+    const uint32_t slot = (uint32_t)__popcll(sm & (~0ull >> (63u - lane))) - 1u;  // the last start at or before
+    uint4* const run = s_run[wave][covered ? slot : 0u];
+    if (start) {
+      const uint4* rs = reinterpret_cast<const uint4*>(setup + id);
+      const uint4* ss = reinterpret_cast<const uint4*>(shade + id);
+#pragma unroll
+      for (int q = 0; q < 5; q++) run[q] = rs[q];
+#pragma unroll
+      for (int q = 0; q < 9; q++) run[5 + q] = ss[q];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (covered) {
+      uint4* rd = reinterpret_cast<uint4*>(&r);
+      uint4* sd = reinterpret_cast<uint4*>(&sh);
+#pragma unroll
+      for (int q = 0; q < 5; q++) rd[q] = run[q];
+#pragma unroll
+      for (int q = 0; q < 9; q++) sd[q] = run[5 + q];
+    }
+  } else if (covered) {
+    const uint4* rs = reinterpret_cast<const uint4*>(setup + id);
+    uint4* rd = reinterpret_cast<uint4*>(&r);
+#pragma unroll
+    for (int q = 0; q < 5; q++) rd[q] = rs[q];
+    sh = shade[id];
+  }
+  if (!in || id == kSkip) return;  // (kSkip: shaded by another shard)
   const DevView& vw = views[find_view_px(views, n_views, g)];
   const uint32_t p = g - vw.px_base;
   const uint32_t W = (uint32_t)vw.fr.W, H = (uint32_t)vw.fr.H;
   const uint32_t y = p / W, x = p - y * W;
-  const uint32_t id = vis[g];
-  if (id == kSkip) return;  // shaded by another shard
   uint32_t bgra;
   float depth;
   if (id == kNone) {
@@ -642,10 +685,13 @@ __global__ __launch_bounds__(256) void k_shade(const DevView* __restrict__ views
     bgra = (uint32_t)c[0] | ((uint32_t)c[1] << 8) | ((uint32_t)c[2] << 16) | ((uint32_t)c[3] << 24);
     depth = vw.fr.clear_depth;
   } else {
-    const SetupRec r = setup[id];
     const FragOut o = bary(r, (float)x, (float)y);
-    const ShadeRec& s = shade[id];
-    bgra = shade_pixel(s, mats[s.material], o, status);
+    // one material for the wave's covered pixels: scalar loads
+    const uint32_t m0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)sh.material);
+    if (__ballot(sh.material != m0) == 0ull)
+      bgra = shade_pixel(sh, mats[m0], o, status);
+    else
+      bgra = shade_pixel(sh, mats[sh.material], o, status);
     depth = o.d;
   }
   vw.out_bgra[(H - 1u - y) * W + x] = bgra;  // draw_pixel_coord row flip, canvas.rs:955
