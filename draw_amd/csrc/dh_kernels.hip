@@ -612,19 +612,31 @@ __device__ inline uint32_t wave_tiles(const Spans& g, int TS, int lane, int* s_l
 }
 
 // Ordered bins, one clipped triangle: its wide flag and, when it is not wide,
-// its tile count (the wide ones are counted a wave at a time later).
+// its tile count and its tiles (bit 4 * (ty - cy0) + (tx - cx0) of tmask[i],
+// within its at most 4 x 4 tile box; the wide ones are counted a wave at a
+// time later).
 __device__ inline void bin_count_one(const SetupRec& r, int TS, uint32_t i, uint32_t* __restrict__ bincnt,
-                                     uint8_t* __restrict__ wflag) {
-  uint32_t c = 0;
+                                     uint8_t* __restrict__ wflag, uint16_t* __restrict__ tmask) {
+  uint32_t c = 0, mask = 0;
   bool is_wide = false;
   if (!(r.flags & kDrop)) {
     Spans g;
     spans_init(r, TS, g);
     is_wide = bin_is_wide(g);
-    if (!is_wide) c = thread_tiles(g, TS, [](uint32_t, int, int) {});
+    if (!is_wide)
+      c = thread_tiles(g, TS, [&](uint32_t, int ty, int tx) { mask |= 1u << (4 * (ty - g.cy0) + (tx - g.cx0)); });
   }
   wflag[i] = is_wide ? 1u : 0u;
-  if (!is_wide) bincnt[i] = c;
+  if (!is_wide) {
+    bincnt[i] = c;
+    tmask[i] = (uint16_t)mask;
+  }
+}
+
+// The tile masks of ordered bins live in the wide-flag buffer after the flags
+// (the buffer holds 4 bytes per clipped triangle: a flag byte, a 2-byte mask).
+__device__ __host__ inline uint16_t* tile_masks(void* big, uint32_t cap_tri) {
+  return reinterpret_cast<uint16_t*>(reinterpret_cast<uint8_t*>(big) + ((cap_tri + 15u) & ~15u));
 }
 
 // ================================================================== geometry
@@ -757,7 +769,7 @@ __device__ inline void geom_emit_clipped(uint32_t j, const DevDraw* __restrict__
     for (int q = 0; q < 6; q++) reinterpret_cast<uint4*>(setup + idx)[q] = ro[q];
 #pragma unroll
     for (int q = 0; q < 9; q++) reinterpret_cast<uint4*>(shade + idx)[q] = so[q];
-    if (BIN) bin_count_one(rec, TS, idx, bincnt, wflag);
+    if (BIN) bin_count_one(rec, TS, idx, bincnt, wflag, tile_masks(wflag, cap_tri));
     if (dkey) dkey[idx] = depth_order_key(rec, views[view].depth_base);
   };
   cull_and_clip(fr, v[0], v[1], v[2], !(dr.flags & kDrawNoCull), out);
@@ -812,7 +824,7 @@ __device__ inline void geom_emit_simple(uint32_t j, bool active, uint4* stage, c
       SetupRec rec;
       make_setup(fr, v[0], v[1], v[2], dr.material, mats[dr.material], dr.view, dr.group, ro, so, rec);
       ok = true;
-      if (BIN) bin_count_one(rec, TS, idx, bincnt, wflag);
+      if (BIN) bin_count_one(rec, TS, idx, bincnt, wflag, tile_masks(wflag, cap_tri));
       if (dkey) dkey[idx] = depth_order_key(rec, views[dr.view].depth_base);
     }
   }
@@ -1142,6 +1154,18 @@ __global__ __launch_bounds__(256) void k_bin_emit(const SetupRec* __restrict__ s
     const uint32_t cy0 = (h.y & 0xFFFFu) / (uint32_t)TS, cy1 = (h.y >> 16) / (uint32_t)TS;
     if (cx0 == cx1 && cy0 == cy1) {
       put(views[h.w], 0u, (int)cy0, (int)cx0);
+      continue;
+    }
+    if (ORDERED) {
+      // the tiles the count pass found (bit 4 * row + column of the tile
+      // box), in its order
+      const DevView& vw = views[h.w];
+      uint32_t mask = tile_masks(const_cast<uint8_t*>(wflag), cap_tri)[i], e = 0;
+      while (mask) {
+        const uint32_t b = (uint32_t)__builtin_ctz(mask);
+        mask &= mask - 1u;
+        put(vw, e++, (int)(cy0 + (b >> 2)), (int)(cx0 + (b & 3u)));
+      }
       continue;
     }
     const SetupRec r = setup[i];

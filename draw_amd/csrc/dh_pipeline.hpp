@@ -32,7 +32,8 @@ struct PipeArgs {
   uint32_t cap_tri;
   uint32_t* bincnt;    // [cap_tri]
   uint32_t* binoff;    // [cap_tri]
-  uint32_t* big;       // [cap_tri] used as one byte per triangle: 1 = wide, binned by a wave
+  uint32_t* big;       // [cap_tri] words: a byte per triangle (1 = wide, binned by a wave), then a u16
+                       // tile mask per triangle (ordered bins)
   bool ordered_bins;   // stable radix sort (per-tile lists in submission order)
   bool depth_order;    // ordered bins: per-tile lists near first (triangles sorted by depth key before binning)
   uint32_t* tile_cnt;  // [2 * n_tiles] per-tile counts, then per-tile cursors
