@@ -1119,60 +1119,109 @@ __global__ __launch_bounds__(256) void k_bin_count_wide(const SetupRec* __restri
 // Pass 2.  Ordered: (tile key, triangle) pairs at the triangle's scanned
 // offset, for the stable radix sort.  Unordered: the triangle id goes straight
 // into its tile's list at tile_start + atomicAdd(cursor).
-// SORTED (ordered bins, near first): position j of the depth order takes
-// triangle order[j] at the scanned offset sorted_off[j], and records the
-// offsets of wide triangles in binoff for k_bin_emit_wide.
-template <bool ORDERED, bool SORTED>
+//
+// The pairs of one triangle that is not wide, its e-th tile at position
+// o + e (ORDERED), or at tile_start + atomicAdd(cursor).
+template <bool ORDERED>
+__device__ inline void bin_emit_tri(const SetupRec* __restrict__ setup, const DevView* __restrict__ views, int TS,
+                                    const uint8_t* __restrict__ wflag, uint32_t cap_tri, uint32_t i, uint32_t o,
+                                    const uint32_t* __restrict__ tile_start, uint32_t* __restrict__ tile_cur,
+                                    uint32_t cap_pairs, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  auto put = [&](const DevView& vw, uint32_t e, int ty, int tx) {
+    const uint32_t key = vw.tile_base + (uint32_t)ty * vw.ntx + (uint32_t)tx;
+    const uint32_t pos = ORDERED ? o + e : tile_start[key] + atomicAdd(&tile_cur[key], 1u);
+    if (pos < cap_pairs) {
+      if (ORDERED) keys[pos] = key;
+      vals[pos] = i;
+    }
+  };
+  // the record's last 16 bytes first: a triangle within one tile needs no edges
+  const uint4 h = reinterpret_cast<const uint4*>(setup + i)[5];
+  if (h.z & kDrop) return;
+  const uint32_t cx0 = (h.x & 0xFFFFu) / (uint32_t)TS, cx1 = (h.x >> 16) / (uint32_t)TS;
+  const uint32_t cy0 = (h.y & 0xFFFFu) / (uint32_t)TS, cy1 = (h.y >> 16) / (uint32_t)TS;
+  if (cx0 == cx1 && cy0 == cy1) {
+    put(views[h.w], 0u, (int)cy0, (int)cx0);
+    return;
+  }
+  if (ORDERED) {
+    // the tiles the count pass found (bit 4 * row + column of the tile box),
+    // in its order
+    const DevView& vw = views[h.w];
+    uint32_t mask = tile_masks(const_cast<uint8_t*>(wflag), cap_tri)[i], e = 0;
+    while (mask) {
+      const uint32_t b = (uint32_t)__builtin_ctz(mask);
+      mask &= mask - 1u;
+      put(vw, e++, (int)(cy0 + (b >> 2)), (int)(cx0 + (b & 3u)));
+    }
+    return;
+  }
+  const SetupRec r = setup[i];
+  Spans g;
+  spans_init(r, TS, g);
+  const DevView& vw = views[r.view];
+  thread_tiles(g, TS, [&](uint32_t e, int ty, int tx) { put(vw, e, ty, tx); });
+}
+
+template <bool ORDERED>
 __global__ __launch_bounds__(256) void k_bin_emit(const SetupRec* __restrict__ setup, const DevView* __restrict__ views,
                                                   const Status* status, uint32_t cap_tri, int TS,
-                                                  const uint8_t* __restrict__ wflag, uint32_t* __restrict__ binoff,
-                                                  const uint32_t* __restrict__ order,
-                                                  const uint32_t* __restrict__ sorted_off,
+                                                  const uint8_t* __restrict__ wflag,
+                                                  const uint32_t* __restrict__ binoff,
                                                   const uint32_t* __restrict__ tile_start,
                                                   uint32_t* __restrict__ tile_cur, uint32_t cap_pairs,
                                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   const uint32_t n = min(status->n_out, cap_tri);
-  for (uint32_t j = blockIdx.x * 256u + threadIdx.x; j < n; j += gridDim.x * 256u) {
-    const uint32_t i = SORTED ? order[j] : j;
-    if (wflag[i]) {
-      if (SORTED) binoff[i] = sorted_off[j];
-      continue;
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u)
+    if (!wflag[i])
+      bin_emit_tri<ORDERED>(setup, views, TS, wflag, cap_tri, i, ORDERED ? binoff[i] : 0u, tile_start, tile_cur,
+                            cap_pairs, keys, vals);
+}
+
+// Near-first pairs with the scan of their counts fused in: block b takes
+// chunk b of the depth order (the chunks of k_scan_reduce, whose sums are in
+// partials), scans the counts bincnt[order[j]] from the sums before it, and
+// emits each triangle's pairs at its offset (a wide triangle's offset goes to
+// binoff for k_bin_emit_wide).  Block 0 publishes the pair total and the
+// overflow flag.
+__global__ __launch_bounds__(256) void k_bin_emit_near(const SetupRec* __restrict__ setup,
+                                                       const DevView* __restrict__ views, Status* status,
+                                                       uint32_t cap_tri, int TS, const uint8_t* __restrict__ wflag,
+                                                       uint32_t* __restrict__ binoff, const uint32_t* __restrict__ order,
+                                                       const uint32_t* __restrict__ bincnt,
+                                                       const uint32_t* __restrict__ partials, uint32_t cap_pairs,
+                                                       uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  __shared__ uint32_t s_tmp[4];
+  const uint32_t n = min(status->n_out, cap_tri);
+  const uint32_t C = scan_chunk(n);
+  const uint32_t b0 = blockIdx.x * C, b1 = min(b0 + C, n);
+  const uint32_t upto = blockIdx.x == 0 ? (uint32_t)kScanG : blockIdx.x;
+  uint32_t p = 0;
+  for (uint32_t k = threadIdx.x; k < upto; k += 256) p += partials[k];
+  uint32_t sum;
+  block_exclusive_scan256(p, s_tmp, &sum);
+  uint32_t running = 0;
+  if (blockIdx.x == 0) {
+    if (threadIdx.x == 0) {
+      status->n_pairs = sum;
+      if (sum > cap_pairs) atomicOr(&status->overflow, 2u);
     }
-    const uint32_t o = SORTED ? sorted_off[j] : (ORDERED ? binoff[i] : 0u);
-    auto put = [&](const DevView& vw, uint32_t e, int ty, int tx) {
-      const uint32_t key = vw.tile_base + (uint32_t)ty * vw.ntx + (uint32_t)tx;
-      const uint32_t pos = ORDERED ? o + e : tile_start[key] + atomicAdd(&tile_cur[key], 1u);
-      if (pos < cap_pairs) {
-        if (ORDERED) keys[pos] = key;
-        vals[pos] = i;
-      }
-    };
-    // the record's last 16 bytes first: a triangle within one tile needs no edges
-    const uint4 h = reinterpret_cast<const uint4*>(setup + i)[5];
-    if (h.z & kDrop) continue;
-    const uint32_t cx0 = (h.x & 0xFFFFu) / (uint32_t)TS, cx1 = (h.x >> 16) / (uint32_t)TS;
-    const uint32_t cy0 = (h.y & 0xFFFFu) / (uint32_t)TS, cy1 = (h.y >> 16) / (uint32_t)TS;
-    if (cx0 == cx1 && cy0 == cy1) {
-      put(views[h.w], 0u, (int)cy0, (int)cx0);
-      continue;
-    }
-    if (ORDERED) {
-      // the tiles the count pass found (bit 4 * row + column of the tile
-      // box), in its order
-      const DevView& vw = views[h.w];
-      uint32_t mask = tile_masks(const_cast<uint8_t*>(wflag), cap_tri)[i], e = 0;
-      while (mask) {
-        const uint32_t b = (uint32_t)__builtin_ctz(mask);
-        mask &= mask - 1u;
-        put(vw, e++, (int)(cy0 + (b >> 2)), (int)(cx0 + (b & 3u)));
-      }
-      continue;
-    }
-    const SetupRec r = setup[i];
-    Spans g;
-    spans_init(r, TS, g);
-    const DevView& vw = views[r.view];
-    thread_tiles(g, TS, [&](uint32_t e, int ty, int tx) { put(vw, e, ty, tx); });
+  } else {
+    running = sum;
+  }
+  for (uint32_t tb = b0; tb < b1; tb += 256u) {
+    const uint32_t j = tb + threadIdx.x;
+    const bool act = j < b1;
+    const uint32_t i = act ? order[j] : 0u;
+    const uint32_t c = act ? bincnt[i] : 0u;
+    uint32_t tot;
+    const uint32_t o = block_exclusive_scan256(c, s_tmp, &tot) + running;
+    running += tot;
+    if (!act) continue;
+    if (wflag[i])
+      binoff[i] = o;
+    else
+      bin_emit_tri<true>(setup, views, TS, wflag, cap_tri, i, o, nullptr, nullptr, cap_pairs, keys, vals);
   }
 }
 
@@ -1550,28 +1599,26 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
                        wide_list, a.bincnt, a.tile_cnt);
     if (depth_sorted) {
       // near first: the clipped triangles in depth-key order (one stable
-      // 8-bit pass over keys[1] + cap_tri / vals[1], results in keys[1] +
-      // 2 cap_tri / vals[1] + cap_tri), then the bin offsets in that order
+      // 8-bit pass over keys[1] + cap_tri, the values starting as the
+      // triangle ids themselves; results in vals[1] + cap_tri, the sorted
+      // keys not written), then the bin offsets in that order
       uint32_t* const dkeys[2] = {a.keys[1] + a.cap_tri, a.keys[1] + 2ull * a.cap_tri};
-      // (the values start as the triangle ids themselves: no input array; the
-      // sorted keys are not needed: their buffer takes the scan below)
       uint32_t* const dvals[2] = {nullptr, a.vals[1] + a.cap_tri};
       const int dc = enqueue_radix_sort(dkeys, dvals, &a.status->n_out, a.cap_tri, kDepthKeyBits, a.hist,
                                         a.partials, s, false, nullptr, 0u);
       join(s, a, 3);
       uint32_t* order = dvals[dc];
-      uint32_t* sorted_off = dkeys[dc];
-      scan_u32(a.bincnt, sorted_off, &a.status->n_out, 0, a.cap_tri, a.partials, &a.status->n_pairs, a.cap_pairs,
-               &a.status->overflow, 2u, s, order);
-      hipLaunchKernelGGL((k_bin_emit<true, true>), dim3(gb), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri,
-                         a.TS, wflag, a.binoff, order, sorted_off, a.tile_start, a.tile_cur, a.cap_pairs, a.keys[0],
-                         a.vals[0]);
+      // the scan of the counts in that order: its chunk sums here, the rest
+      // inside the pair emit
+      hipLaunchKernelGGL(k_scan_reduce, dim3(kScanG), dim3(256), 0, s, a.bincnt, &a.status->n_out, 0u, a.cap_tri,
+                         a.partials, order);
+      hipLaunchKernelGGL(k_bin_emit_near, dim3(kScanG), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri,
+                         a.TS, wflag, a.binoff, order, a.bincnt, a.partials, a.cap_pairs, a.keys[0], a.vals[0]);
     } else {
       scan_u32(a.bincnt, a.binoff, &a.status->n_out, 0, a.cap_tri, a.partials, &a.status->n_pairs, a.cap_pairs,
                &a.status->overflow, 2u, s);
-      hipLaunchKernelGGL((k_bin_emit<true, false>), dim3(gb), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri,
-                         a.TS, wflag, a.binoff, nullptr, nullptr, a.tile_start, a.tile_cur, a.cap_pairs, a.keys[0],
-                         a.vals[0]);
+      hipLaunchKernelGGL(k_bin_emit<true>, dim3(gb), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri, a.TS,
+                         wflag, a.binoff, a.tile_start, a.tile_cur, a.cap_pairs, a.keys[0], a.vals[0]);
     }
     hipLaunchKernelGGL(k_bin_emit_wide<true>, dim3(gw), dim3(256), 0, s, a.setup, a.views, a.status, wide_list,
                        a.TS, a.binoff, a.tile_start, a.tile_cur, a.cap_pairs, a.keys[0], a.vals[0]);
@@ -1595,9 +1642,8 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
                        wide_list, a.bincnt, a.tile_cnt);
     scan_u32(a.tile_cnt, a.tile_start, nullptr, a.n_tiles, a.n_tiles, a.partials, &a.status->n_pairs, a.cap_pairs,
              &a.status->overflow, 2u, s);
-    hipLaunchKernelGGL((k_bin_emit<false, false>), dim3(gb), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri,
-                       a.TS, wflag, a.binoff, nullptr, nullptr, a.tile_start, a.tile_cur, a.cap_pairs, a.keys[0],
-                       a.vals[0]);
+    hipLaunchKernelGGL(k_bin_emit<false>, dim3(gb), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri, a.TS,
+                       wflag, a.binoff, a.tile_start, a.tile_cur, a.cap_pairs, a.keys[0], a.vals[0]);
     hipLaunchKernelGGL(k_bin_emit_wide<false>, dim3(gw), dim3(256), 0, s, a.setup, a.views, a.status, wide_list,
                        a.TS, a.binoff, a.tile_start, a.tile_cur, a.cap_pairs, a.keys[0], a.vals[0]);
     hipLaunchKernelGGL(k_ranges_from_counts, dim3(std::min<uint32_t>(cdiv(a.n_tiles, 256), 2048u)), dim3(256), 0, s,
