@@ -199,3 +199,37 @@ def test_graph_replay_matches_eager_and_oracle(renderer):
                 assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), f"replay {rep} view {k}"
     ref = oracle(scenes[1])
     assert_same(*eager[1], ref, label="moved camera")
+
+
+def test_bin_capacity_regrowth_large_triangles():
+    """A fresh context whose first frame has far more bin entries than its
+    initial capacity (40 overlapping screen-filling quads: 80 wide
+    triangles): the frame reports the overflow, regrows and is retried, and
+    the result is still the reference's."""
+    import math
+    from draw_amd.render import Renderer
+    from draw_amd.scene import Camera, Scene
+    rng = np.random.default_rng(17)
+    tex = workloads._phong(workloads.noise_texture(64, rng))
+    objs = []
+    for k in range(40):
+        z = -0.5 - 0.75 * k                      # distance 3.5 .. 33 from the camera
+        half = (3.0 - z) * math.tan(math.radians(30.0)) * float(rng.uniform(0.9, 1.4))
+        objs.append(workloads.window_object(1, half, z, tex, x0=float(rng.uniform(-0.3, 0.3)),
+                                            y0=float(rng.uniform(-0.3, 0.3))))
+    s = Scene(320, 240, Camera(position=(0.0, 0.0, 3.0), fov_x=60.0, near=-0.1, far=-100.0), (0.0, 6.0, 6.0),
+              objs)
+    ref = oracle(s)
+    r = Renderer(0)
+    try:
+        for mode in (2, 1):
+            r.set_bin_order(mode)
+            r.set_raster(0, 16, 4)
+            canvas, winner = r.render(s)
+            assert_same(canvas.frame, canvas.depth, winner, ref, label=f"large quads, bin order {mode}")
+            if mode == 2:
+                st = r.stats()
+                assert st["retries"] >= 1, st
+                assert st["bin_entries"] > 4 * (st["tris_out"] + 1024), st
+    finally:
+        r.close()
