@@ -8,7 +8,8 @@ complete on the device before the next (one-frame-per-launch mode).  Inputs
 
 Prints ONE JSON line: shaded Mpix/s (pixels with a depth-test winner per
 second) as ``value``, plus triangles/s, p50/p99 frame latency, per-stage device
-times, the rooflines of the raster (dominant) and shade kernels, the batched mode
+times, the roofline of the raster (the dominant kernel, which shades its own
+tiles in the default raster kind), the batched mode
 against the plain path on 256 views, and the CPU oracle
 timed on a bounded prefix of the same workload on this host.
 """
@@ -61,6 +62,17 @@ def raster_bytes(P: int, entries: int) -> dict:
     return {"algorithmic": 8 * P + 52 * entries, "record": 4 * P + 100 * entries}
 
 
+def raster_shade_bytes(P: int, entries: int, p_cov: int, t_vis: int) -> dict:
+    """Algorithmic HBM bytes of one kind-0 k_raster launch, which shades its own
+    tiles: per pixel a 4 B BGRA8 and an f32 depth written; per bin entry a 4 B
+    id and a 48 B setup record read; per visible triangle 3 vertices x 14 f32
+    and per covered pixel the 4 B diffuse texel.  ``record``: this
+    implementation also writes the u32 winner per pixel, reads a 96 B raster
+    record per entry and 224 B of records per visible triangle."""
+    return {"algorithmic": 8 * P + 52 * entries + 168 * t_vis + 4 * p_cov,
+            "record": 12 * P + 100 * entries + 224 * t_vis + 4 * p_cov}
+
+
 def measured_traffic(kernel: str):
     """HBM bytes per launch of ``kernel`` on this workload from the committed
     rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE per dispatch, each
@@ -80,9 +92,22 @@ def measured_traffic(kernel: str):
 
 
 def raster_kernel_name(kind: int, tile: int, waves: int) -> str:
-    """The instantiated raster kernel: k_raster<tile, quadrants per wave> or
-    k_raster_tp<tile, waves>."""
-    return f"k_raster_tp<{tile}, {waves}>" if kind == 1 else f"k_raster<{tile}, {(tile // 8) ** 2 // waves}>"
+    """The instantiated raster kernel: k_raster<tile, quadrants per wave, true>
+    (kind 0, shading its own tiles) or k_raster_tp<tile, waves> (kind 1, which
+    k_shade follows)."""
+    return f"k_raster_tp<{tile}, {waves}>" if kind == 1 else f"k_raster<{tile}, {(tile // 8) ** 2 // waves}, true>"
+
+
+def frame_rooflines(kind: int, kernel: str, P: int, entries: int, p_cov: int, t_vis: int, st: dict,
+                    traffic: bool) -> dict:
+    """The dominant kernel's roofline (the raster; for kind 0 it includes the
+    shading), and for kind 1 the separate k_shade's."""
+    tr = (lambda k: measured_traffic(k)) if traffic else (lambda k: None)
+    if kind == 0:
+        return {"roofline": roofline(kernel, raster_shade_bytes(P, entries, p_cov, t_vis), st["ms_raster"],
+                                     tr(kernel))}
+    return {"roofline": roofline(kernel, raster_bytes(P, entries), st["ms_raster"], tr(kernel)),
+            "roofline_shade": roofline("k_shade", shade_bytes(P, p_cov, t_vis), st["ms_shade"], tr("k_shade"))}
 
 
 def roofline(kernel: str, nbytes: dict, ms: float, traffic=None) -> dict:
@@ -182,8 +207,9 @@ def bench_batched(reps: int) -> dict:
         "batched_over_plain": round(batch_mpix / plain_mpix, 3),
         "stage_ms_eager": {k: round(st[k], 4) for k in ("ms_geometry", "ms_binning", "ms_raster", "ms_shade",
                                                         "ms_frame")},
-        "roofline_shade": roofline("k_shade", shade_bytes(px, covered, t_vis), st["ms_shade"]),
-        "roofline_raster": roofline(rast_kernel, raster_bytes(px, int(st["bin_entries"])), st["ms_raster"]),
+        **{("roofline_raster" if k == "roofline" else k): v
+           for k, v in frame_rooflines(kind, rast_kernel, px, int(st["bin_entries"]), covered, t_vis, st,
+                                       False).items()},
     }
 
 
@@ -406,9 +432,7 @@ def main():
     frame_ms = total / args.steps * 1e3
     value = p_cov * args.steps / total / 1e6
     rast_kernel = raster_kernel_name(args.raster_kind, args.tile, args.waves)
-    roof_raster = roofline(rast_kernel, raster_bytes(W * H, int(st["bin_entries"])), st["ms_raster"],
-                           measured_traffic(rast_kernel))
-    roof_shade = roofline("k_shade", shade_bytes(W * H, p_cov, t_vis), st["ms_shade"], measured_traffic("k_shade"))
+    roofs = frame_rooflines(args.raster_kind, rast_kernel, W * H, int(st["bin_entries"]), p_cov, t_vis, st, True)
     out = {
         "metric": "shaded Mpix/s", "value": round(value, 2), "unit": "Mpix/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(frame_ms, 4),
@@ -428,8 +452,7 @@ def main():
         "frame": {"covered_px": p_cov, "visible_tris": t_vis, "clipped_tris": int(st["tris_out"]),
                   "bin_entries": int(st["bin_entries"]), "fragments": int(st["frags"]),
                   "frags_per_px": round(st["frags"] / (W * H), 2)},
-        "roofline": roof_raster,
-        "roofline_shade": roof_shade,
+        **roofs,
     }
     r.close()
     if args.shard_steps > 0:

@@ -1698,11 +1698,12 @@ void enqueue_visibility(const PipeArgs& a, uint32_t shard, uint64_t* out, hipStr
 }
 
 void enqueue_frame(const PipeArgs& a, hipStream_t s) {
-  const PipeArgs b = enqueue_front(a, s);
-  // raster, shade (dh_raster.hip)
+  PipeArgs b = enqueue_front(a, s);
+  // raster, shade (dh_raster.hip): the kind-0 raster shades its own tiles
+  b.shade_in_raster = a.raster_kind == 0;
   enqueue_raster(b, s);
   if (a.ev[3]) (void)hipEventRecord(a.ev[3], s);
-  enqueue_shade(a, s);
+  if (!b.shade_in_raster) enqueue_shade(a, s);
   if (a.ev[4]) (void)hipEventRecord(a.ev[4], s);
 }
 

@@ -35,6 +35,7 @@ struct PipeArgs {
   uint32_t* big;       // [cap_tri] words: a byte per triangle (1 = wide, binned by a wave), then a u16
                        // tile mask per triangle (ordered bins)
   bool ordered_bins;   // stable radix sort (per-tile lists in submission order)
+  bool shade_in_raster;  // kind 0 rasters shade their own tiles (no visibility buffer, no k_shade)
   bool depth_order;    // ordered bins: per-tile lists near first (triangles sorted by depth key before binning)
   uint32_t* tile_cnt;  // [2 * n_tiles] per-tile counts, then per-tile cursors
   uint32_t* tile_cur;  // = tile_cnt + n_tiles

@@ -70,6 +70,126 @@ __device__ inline uint32_t wave_max_u32(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 
+// ===================================================================== shade
+__device__ inline void fetch_rgb(const uint8_t* img, int32_t comp, uint64_t off, uint32_t c[3]) {
+  if (comp == 4) {
+    const uint32_t w = *reinterpret_cast<const uint32_t*>(img + off);
+    c[0] = w & 255u; c[1] = (w >> 8) & 255u; c[2] = (w >> 16) & 255u;
+  } else {
+    c[0] = img[off]; c[1] = img[off + 1]; c[2] = img[off + 2];
+  }
+}
+
+// Shading of a covered pixel, canvas.rs:685-741; returns B | G<<8 | R<<16 | A<<24.
+__device__ inline uint32_t shade_pixel(const ShadeRec& s, const DevMaterial& m, const FragOut& o, Status* status) {
+  const float u = (s.uv[0] * o.al + s.uv[2] * o.be) + s.uv[4] * o.ga;
+  const float v = (s.uv[1] * o.al + s.uv[3] * o.be) + s.uv[5] * o.ga;
+  uint64_t okd, oka;
+  const bool ok1 = rgb_offset(m.kd_w, m.kd_h, m.kd_comp, u, v, &okd);
+  const bool ok2 = rgb_offset(m.ka_w, m.ka_h, m.ka_comp, u, v, &oka);
+  if (!(ok1 && ok2)) {
+    atomicOr(&status->error, 1u);
+    okd = 0;
+    oka = 0;
+  }
+  uint32_t dk[3], ak[3];
+  fetch_rgb(m.kd_img, m.kd_comp, okd, dk);
+  fetch_rgb(m.ka_img, m.ka_comp, oka, ak);
+  const V3 diffuse = v3((float)dk[0] / 255.0f, (float)dk[1] / 255.0f, (float)dk[2] / 255.0f);
+  const V3 ambient = v3((float)ak[0] / 255.0f, (float)ak[1] / 255.0f, (float)ak[2] / 255.0f);
+  const V3 pn = add(add(muls(v3(s.n[0], s.n[1], s.n[2]), o.al), muls(v3(s.n[3], s.n[4], s.n[5]), o.be)),
+                    muls(v3(s.n[6], s.n[7], s.n[8]), o.ga));
+  const V3 pl = add(add(muls(v3(s.l[0], s.l[1], s.l[2]), o.al), muls(v3(s.l[3], s.l[4], s.l[5]), o.be)),
+                    muls(v3(s.l[6], s.l[7], s.l[8]), o.ga));
+  const V3 ph = add(add(muls(v3(s.h[0], s.h[1], s.h[2]), o.al), muls(v3(s.h[3], s.h[4], s.h[5]), o.be)),
+                    muls(v3(s.h[6], s.h[7], s.h[8]), o.ga));
+  const V3 c_l = v3(m.ks[0], m.ks[1], m.ks[2]);
+  const V3 c_r = v3(diffuse.x * m.kd[0], diffuse.y * m.kd[1], diffuse.z * m.kd[2]);
+  const V3 c_a = v3(ambient.x * m.ka[0], ambient.y * m.ka[1], ambient.z * m.ka[2]);
+  const float dln = dot(pl, pn);
+  const float mx = (dln > 0.0f) ? dln : 0.0f;  // 0.0_f32.max(x); the sign of a zero is irrelevant below
+  const float sfac = 1.0f - mx;
+  const V3 inner = add(c_a, muls(c_l, sfac));
+  const V3 t1 = v3(c_r.x * inner.x, c_r.y * inner.y, c_r.z * inner.z);
+  const float h2 = powi_f(dot(ph, pn), m.power);
+  const V3 col = add(t1, muls(c_l, h2));
+  const V3 sc = muls(col, 255.0f);
+  return f2u8(sc.z) | (f2u8(sc.y) << 8) | (f2u8(sc.x) << 16) | (255u << 24);
+}
+
+// Shading of up to 64 pixels by one wave; every lane calls it.  A lane's
+// pixel (x, y) of view vw has winner id, or kNone (the clear colour); lanes
+// with active false have no pixel.  Runs of equal winners along the wave load
+// their records once into run[slot] (up to R runs; more take per-lane loads):
+// the texture address units cost per requested byte per lane, and a run of
+// pixels shares one triangle's 224 record bytes.  A material shared by the
+// whole wave is read with scalar loads.
+template <int R>
+__device__ inline void shade_wave(uint32_t id, bool active, uint32_t x, uint32_t y, const DevView& vw,
+                                  const SetupRec* __restrict__ setup, const ShadeRec* __restrict__ shade,
+                                  const DevMaterial* __restrict__ mats, Status* __restrict__ status,
+                                  uint4 (*run)[14]) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const bool covered = active && id != kNone;
+  const uint32_t key = covered ? id : kSkip;
+  // runs: a covered lane starts one unless the lane before holds the same id
+  const uint32_t prev = (uint32_t)__shfl_up((int)key, 1, 64);
+  const bool start = covered && (lane == 0u || prev != key);
+  const uint64_t sm = __ballot(start);
+  SetupRec r;
+  ShadeRec sh;
+  if (__popcll(sm) <= R) {
+    const uint32_t slot = (uint32_t)__popcll(sm & (~0ull >> (63u - lane))) - 1u;  // the last start at or before
+    uint4* const rs_run = run[covered ? slot : 0u];
+    if (start) {
+      const uint4* rs = reinterpret_cast<const uint4*>(setup + id);
+      const uint4* ss = reinterpret_cast<const uint4*>(shade + id);
+#pragma unroll
+      for (int q = 0; q < 5; q++) rs_run[q] = rs[q];
+#pragma unroll
+      for (int q = 0; q < 9; q++) rs_run[5 + q] = ss[q];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (covered) {
+      uint4* rd = reinterpret_cast<uint4*>(&r);
+      uint4* sd = reinterpret_cast<uint4*>(&sh);
+#pragma unroll
+      for (int q = 0; q < 5; q++) rd[q] = rs_run[q];
+#pragma unroll
+      for (int q = 0; q < 9; q++) sd[q] = rs_run[5 + q];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  } else if (covered) {
+    const uint4* rs = reinterpret_cast<const uint4*>(setup + id);
+    uint4* rd = reinterpret_cast<uint4*>(&r);
+#pragma unroll
+    for (int q = 0; q < 5; q++) rd[q] = rs[q];
+    sh = shade[id];
+  }
+  if (!active) return;
+  const uint32_t W = (uint32_t)vw.fr.W, H = (uint32_t)vw.fr.H;
+  uint32_t bgra;
+  float depth;
+  if (!covered) {
+    const uint8_t* c = vw.fr.clear_bgra;
+    bgra = (uint32_t)c[0] | ((uint32_t)c[1] << 8) | ((uint32_t)c[2] << 16) | ((uint32_t)c[3] << 24);
+    depth = vw.fr.clear_depth;
+  } else {
+    const FragOut o = bary(r, (float)x, (float)y);
+    // one material for the wave's covered pixels: scalar loads
+    const uint32_t m0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)sh.material);
+    if (__ballot(sh.material != m0) == 0ull)
+      bgra = shade_pixel(sh, mats[m0], o, status);
+    else
+      bgra = shade_pixel(sh, mats[sh.material], o, status);
+    depth = o.d;
+  }
+  const uint32_t p = y * W + x;
+  vw.out_bgra[(H - 1u - y) * W + x] = bgra;  // draw_pixel_coord row flip, canvas.rs:955
+  vw.out_depth[p] = depth;
+  if (vw.out_winner) vw.out_winner[p] = covered ? id - vw.id_base : kNone;
+}
+
 // Per-tile visibility.  The tile's bin list is near first (or in submission
 // order, or unordered: the result is the same); every
 // covered fragment with depth < clear_depth contributes its key
@@ -85,7 +205,10 @@ __device__ inline uint32_t wave_max_u32(uint32_t v) {
 //  4. the queue is drained densely: divisions, depth, key, atomicMin.
 // Early-Z only drops a fragment whose depth lower bound is strictly above a
 // depth already recorded at its pixel, so it cannot be the minimum nor tie it.
-template <int TS, int QPW>
+// SHADE: the workgroup shades its tile's pixels itself once the tile's
+// visibility is final (the tiles that finish first shade while others still
+// rasterize), else it writes the winners to vis_out for k_shade.
+template <int TS, int QPW, bool SHADE>
 __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
     const DevView* __restrict__ views, uint32_t n_views, const uint2* __restrict__ ranges,
     const uint32_t* __restrict__ vals, const SetupRec* __restrict__ setup, const ShadeRec* __restrict__ shade,
@@ -95,10 +218,15 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
   // fragment queue per wave: drained once it holds more than kQFlush entries,
   // checked before each triangle, which appends at most 64 per quadrant
   constexpr int kQFlush = 256, QCAP = kQFlush + 64 * QPW;
-  __shared__ SetupRec s_rec[CH];
-  __shared__ uint4 s_hdr[CH];  // scan rectangle x, y; flags; depth-bound key
-  __shared__ float4 s_pre[CH][3];  // per edge: s*A, s*B, s*(C-D), |C|+|D| with s = sign(f)
-  __shared__ uint32_t s_id[CH];
+  // the staged chunk: records, headers, pre-filter constants, ids (one LDS
+  // area, which the shading epilogue reuses for its record runs)
+  constexpr int kChunkU4 = CH * (sizeof(SetupRec) + 16 + 48 + 4) / 16;
+  __shared__ uint4 s_chunk[kChunkU4];
+  SetupRec* const s_rec = reinterpret_cast<SetupRec*>(s_chunk);
+  uint4* const s_hdr = s_chunk + CH * sizeof(SetupRec) / 16;  // scan rectangle x, y; flags; depth-bound key
+  // per edge: s*A, s*B, s*(C-D), |C|+|D| with s = sign(f)
+  float4 (*const s_pre)[3] = reinterpret_cast<float4 (*)[3]>(s_hdr + CH);
+  uint32_t* const s_id = reinterpret_cast<uint32_t*>(s_hdr + CH + 3 * CH);
   __shared__ unsigned long long s_vis[TS * TS];
   __shared__ uint32_t s_q[NW][QCAP];
 
@@ -400,11 +528,26 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
     }
   }
   __syncthreads();
-  for (int p = tid; p < TS * TS; p += NT) {
-    const int px = tx0 + p % TS, py = ty0 + p / TS;
-    if (px < W && py < H) {
+  if constexpr (SHADE) {
+    // each wave shades 64 consecutive pixels of the tile at a time, its
+    // record runs in its share of the chunk area
+    constexpr int R = (int)(kChunkU4 / (NW * 14)) < 16 ? (int)(kChunkU4 / (NW * 14)) : 16;
+    uint4 (*const run)[14] = reinterpret_cast<uint4 (*)[14]>(s_chunk) + wave * R;
+    for (int p0 = wave * 64; p0 < TS * TS; p0 += NT) {
+      const int p = p0 + lane;
+      const int px = tx0 + p % TS, py = ty0 + p / TS;
+      const bool on = px < W && py < H;
       const unsigned long long v = s_vis[p];
-      vis_out[vw.px_base + (uint32_t)py * (uint32_t)W + (uint32_t)px] = (v == ~0ull) ? kNone : (uint32_t)v;
+      shade_wave<R>((v == ~0ull) ? kNone : (uint32_t)v, on, (uint32_t)px, (uint32_t)py, vw, setup, shade, mats,
+                    status, run);
+    }
+  } else {
+    for (int p = tid; p < TS * TS; p += NT) {
+      const int px = tx0 + p % TS, py = ty0 + p / TS;
+      if (px < W && py < H) {
+        const unsigned long long v = s_vis[p];
+        vis_out[vw.px_base + (uint32_t)py * (uint32_t)W + (uint32_t)px] = (v == ~0ull) ? kNone : (uint32_t)v;
+      }
     }
   }
   if (lane == 0) atomicAdd(&status->frags, nfrag);  // nfrag is wave-uniform
@@ -576,58 +719,7 @@ __global__ __launch_bounds__(64 * NW) void k_raster_tp(
   if (lane == 0) atomicAdd(&status->frags, nfrag);
 }
 
-// ===================================================================== shade
-__device__ inline void fetch_rgb(const uint8_t* img, int32_t comp, uint64_t off, uint32_t c[3]) {
-  if (comp == 4) {
-    const uint32_t w = *reinterpret_cast<const uint32_t*>(img + off);
-    c[0] = w & 255u; c[1] = (w >> 8) & 255u; c[2] = (w >> 16) & 255u;
-  } else {
-    c[0] = img[off]; c[1] = img[off + 1]; c[2] = img[off + 2];
-  }
-}
-
-// Shading of a covered pixel, canvas.rs:685-741; returns B | G<<8 | R<<16 | A<<24.
-__device__ inline uint32_t shade_pixel(const ShadeRec& s, const DevMaterial& m, const FragOut& o, Status* status) {
-  const float u = (s.uv[0] * o.al + s.uv[2] * o.be) + s.uv[4] * o.ga;
-  const float v = (s.uv[1] * o.al + s.uv[3] * o.be) + s.uv[5] * o.ga;
-  uint64_t okd, oka;
-  const bool ok1 = rgb_offset(m.kd_w, m.kd_h, m.kd_comp, u, v, &okd);
-  const bool ok2 = rgb_offset(m.ka_w, m.ka_h, m.ka_comp, u, v, &oka);
-  if (!(ok1 && ok2)) {
-    atomicOr(&status->error, 1u);
-    okd = 0;
-    oka = 0;
-  }
-  uint32_t dk[3], ak[3];
-  fetch_rgb(m.kd_img, m.kd_comp, okd, dk);
-  fetch_rgb(m.ka_img, m.ka_comp, oka, ak);
-  const V3 diffuse = v3((float)dk[0] / 255.0f, (float)dk[1] / 255.0f, (float)dk[2] / 255.0f);
-  const V3 ambient = v3((float)ak[0] / 255.0f, (float)ak[1] / 255.0f, (float)ak[2] / 255.0f);
-  const V3 pn = add(add(muls(v3(s.n[0], s.n[1], s.n[2]), o.al), muls(v3(s.n[3], s.n[4], s.n[5]), o.be)),
-                    muls(v3(s.n[6], s.n[7], s.n[8]), o.ga));
-  const V3 pl = add(add(muls(v3(s.l[0], s.l[1], s.l[2]), o.al), muls(v3(s.l[3], s.l[4], s.l[5]), o.be)),
-                    muls(v3(s.l[6], s.l[7], s.l[8]), o.ga));
-  const V3 ph = add(add(muls(v3(s.h[0], s.h[1], s.h[2]), o.al), muls(v3(s.h[3], s.h[4], s.h[5]), o.be)),
-                    muls(v3(s.h[6], s.h[7], s.h[8]), o.ga));
-  const V3 c_l = v3(m.ks[0], m.ks[1], m.ks[2]);
-  const V3 c_r = v3(diffuse.x * m.kd[0], diffuse.y * m.kd[1], diffuse.z * m.kd[2]);
-  const V3 c_a = v3(ambient.x * m.ka[0], ambient.y * m.ka[1], ambient.z * m.ka[2]);
-  const float dln = dot(pl, pn);
-  const float mx = (dln > 0.0f) ? dln : 0.0f;  // 0.0_f32.max(x); the sign of a zero is irrelevant below
-  const float sfac = 1.0f - mx;
-  const V3 inner = add(c_a, muls(c_l, sfac));
-  const V3 t1 = v3(c_r.x * inner.x, c_r.y * inner.y, c_r.z * inner.z);
-  const float h2 = powi_f(dot(ph, pn), m.power);
-  const V3 col = add(t1, muls(c_l, h2));
-  const V3 sc = muls(col, 255.0f);
-  return f2u8(sc.z) | (f2u8(sc.y) << 8) | (f2u8(sc.x) << 16) | (255u << 24);
-}
-
-// Records of the runs of equal winners along a wave's 64 pixels are loaded
-// once per run into LDS (up to kShadeRuns runs; more take per-lane loads):
-// the texture address units handle bytes per lane, and a run of pixels
-// shares one triangle's 224 record bytes.  The material is read with scalar
-// loads when the whole wave has one.
+// One thread per pixel of the visibility buffer (see shade_wave).
 constexpr int kShadeRuns = 16;
 
 __global__ __launch_bounds__(256) void k_shade(const DevView* __restrict__ views, uint32_t n_views, uint32_t n_pixels,
@@ -635,68 +727,15 @@ __global__ __launch_bounds__(256) void k_shade(const DevView* __restrict__ views
                                                const ShadeRec* __restrict__ shade,
                                                const DevMaterial* __restrict__ mats, Status* __restrict__ status) {
   __shared__ uint4 s_run[4][kShadeRuns][14];  // setup bytes 0-79, then the 144 shade bytes
-  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t wave = threadIdx.x >> 6;
   const uint32_t g = xcd_remap(blockIdx.x, gridDim.x) * 256u + threadIdx.x;
-  const bool in = g < n_pixels;
-  const uint32_t id = in ? vis[g] : kSkip;
-  const bool covered = id != kSkip && id != kNone;
-  // runs: a covered lane starts one unless the lane before holds the same id
-  const uint32_t prev = (uint32_t)__shfl_up((int)id, 1, 64);
-  const bool start = covered && (lane == 0u || prev != id);
-  const uint64_t sm = __ballot(start);
-  SetupRec r;
-  ShadeRec sh;
-  if (__popcll(sm) <= kShadeRuns) {
-    const uint32_t slot = (uint32_t)__popcll(sm & (~0ull >> (63u - lane))) - 1u;  // the last start at or before
-    uint4* const run = s_run[wave][covered ? slot : 0u];
-    if (start) {
-      const uint4* rs = reinterpret_cast<const uint4*>(setup + id);
-      const uint4* ss = reinterpret_cast<const uint4*>(shade + id);
-#pragma unroll
-      for (int q = 0; q < 5; q++) run[q] = rs[q];
-#pragma unroll
-      for (int q = 0; q < 9; q++) run[5 + q] = ss[q];
-    }
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    if (covered) {
-      uint4* rd = reinterpret_cast<uint4*>(&r);
-      uint4* sd = reinterpret_cast<uint4*>(&sh);
-#pragma unroll
-      for (int q = 0; q < 5; q++) rd[q] = run[q];
-#pragma unroll
-      for (int q = 0; q < 9; q++) sd[q] = run[5 + q];
-    }
-  } else if (covered) {
-    const uint4* rs = reinterpret_cast<const uint4*>(setup + id);
-    uint4* rd = reinterpret_cast<uint4*>(&r);
-#pragma unroll
-    for (int q = 0; q < 5; q++) rd[q] = rs[q];
-    sh = shade[id];
-  }
-  if (!in || id == kSkip) return;  // (kSkip: shaded by another shard)
-  const DevView& vw = views[find_view_px(views, n_views, g)];
-  const uint32_t p = g - vw.px_base;
-  const uint32_t W = (uint32_t)vw.fr.W, H = (uint32_t)vw.fr.H;
+  const uint32_t id = g < n_pixels ? vis[g] : kSkip;
+  const bool active = id != kSkip;  // (kSkip: shaded by another shard)
+  const DevView& vw = views[active ? find_view_px(views, n_views, g) : 0u];
+  const uint32_t p = active ? g - vw.px_base : 0u;
+  const uint32_t W = (uint32_t)vw.fr.W;
   const uint32_t y = p / W, x = p - y * W;
-  uint32_t bgra;
-  float depth;
-  if (id == kNone) {
-    const uint8_t* c = vw.fr.clear_bgra;
-    bgra = (uint32_t)c[0] | ((uint32_t)c[1] << 8) | ((uint32_t)c[2] << 16) | ((uint32_t)c[3] << 24);
-    depth = vw.fr.clear_depth;
-  } else {
-    const FragOut o = bary(r, (float)x, (float)y);
-    // one material for the wave's covered pixels: scalar loads
-    const uint32_t m0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)sh.material);
-    if (__ballot(sh.material != m0) == 0ull)
-      bgra = shade_pixel(sh, mats[m0], o, status);
-    else
-      bgra = shade_pixel(sh, mats[sh.material], o, status);
-    depth = o.d;
-  }
-  vw.out_bgra[(H - 1u - y) * W + x] = bgra;  // draw_pixel_coord row flip, canvas.rs:955
-  vw.out_depth[p] = depth;
-  if (vw.out_winner) vw.out_winner[p] = (id == kNone) ? kNone : id - vw.id_base;
+  shade_wave<kShadeRuns>(id, active, x, y, vw, setup, shade, mats, status, s_run[wave]);
 }
 
 
@@ -823,8 +862,12 @@ static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b -
 template <int TS, int QPW>
 static void launch_raster(const PipeArgs& a, hipStream_t s) {
   constexpr int NT = 64 * (TS / 8) * (TS / 8) / QPW;
-  hipLaunchKernelGGL((k_raster<TS, QPW>), dim3(a.n_tiles), dim3(NT), 0, s, a.views, a.n_views, a.ranges, a.vals[0],
-                     a.setup, a.shade, a.mats, a.vis, a.status);
+  if (a.shade_in_raster)
+    hipLaunchKernelGGL((k_raster<TS, QPW, true>), dim3(a.n_tiles), dim3(NT), 0, s, a.views, a.n_views, a.ranges,
+                       a.vals[0], a.setup, a.shade, a.mats, a.vis, a.status);
+  else
+    hipLaunchKernelGGL((k_raster<TS, QPW, false>), dim3(a.n_tiles), dim3(NT), 0, s, a.views, a.n_views, a.ranges,
+                       a.vals[0], a.setup, a.shade, a.mats, a.vis, a.status);
 }
 
 template <int TS, int NW>
