@@ -110,6 +110,20 @@ def frame_rooflines(kind: int, kernel: str, P: int, entries: int, p_cov: int, t_
             "roofline_shade": roofline("k_shade", shade_bytes(P, p_cov, t_vis), st["ms_shade"], tr("k_shade"))}
 
 
+def separate_shade_roofline(r, render, restore: tuple, P: int, p_cov: int, t_vis: int, traffic=None) -> dict:
+    """k_shade on its own: the same frame (or batch) through the kind-1
+    pipeline, whose raster writes the winner buffer and whose separate shading
+    pass then reads it; the median HIP-event time of three eager launches.
+    ``render`` must launch without the graph (a replay records no stages)."""
+    r.set_raster(1, 32, 4)
+    ms = []
+    for _ in range(3):
+        render()
+        ms.append(r.stats()["ms_shade"])
+    r.set_raster(*restore)
+    return roofline("k_shade", shade_bytes(P, p_cov, t_vis), float(np.median(ms)), traffic)
+
+
 def roofline(kernel: str, nbytes: dict, ms: float, traffic=None) -> dict:
     gbs = nbytes["algorithmic"] / (ms * 1e-3) / 1e9 if ms > 0 else None
     return {"kernel": kernel, "bound": "hbm", "achieved": round(gbs, 1) if gbs else None, "peak": HBM_PEAK_GBS,
@@ -187,6 +201,9 @@ def bench_batched(reps: int) -> dict:
         t_vis += tv
     tris = sum(s.triangle_count for s in views)
     px = sum(s.width * s.height for s in views)
+    log("batched: separate shading pass ...")
+    shade_sep = separate_shade_roofline(r, lambda: r.render_batch(items, graph=False, prepared=prepared),
+                                        (kind, tile, waves), px, covered, t_vis)
     rast_kernel = raster_kernel_name(kind, tile, waves)
     r.close()
     plain_mpix = covered / plain_s / 1e6
@@ -210,6 +227,7 @@ def bench_batched(reps: int) -> dict:
         **{("roofline_raster" if k == "roofline" else k): v
            for k, v in frame_rooflines(kind, rast_kernel, px, int(st["bin_entries"]), covered, t_vis, st,
                                        False).items()},
+        "roofline_shade_separate": shade_sep,
     }
 
 
@@ -427,8 +445,13 @@ def main():
     for _ in range(max(3, min(args.steps, 10))):
         r.render_device(scene, fr, meshes)
         st_acc.append(r.stats())
-    r.set_graph(True)
     st = {k: float(np.mean([s[k] for s in st_acc])) for k in st_acc[0]}
+    shade_sep = None
+    if args.raster_kind == 0:
+        shade_sep = separate_shade_roofline(r, lambda: r.render_device(scene, fr, meshes),
+                                            (args.raster_kind, args.tile, args.waves), W * H, p_cov, t_vis,
+                                            measured_traffic("k_shade"))
+    r.set_graph(True)
     frame_ms = total / args.steps * 1e3
     value = p_cov * args.steps / total / 1e6
     rast_kernel = raster_kernel_name(args.raster_kind, args.tile, args.waves)
@@ -454,6 +477,8 @@ def main():
                   "frags_per_px": round(st["frags"] / (W * H), 2)},
         **roofs,
     }
+    if shade_sep is not None:
+        out["roofline_shade"] = shade_sep
     r.close()
     if args.shard_steps > 0:
         log("sort-last shard (1 of 8) ...")

@@ -229,6 +229,7 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
   uint32_t* const s_id = reinterpret_cast<uint32_t*>(s_hdr + CH + 3 * CH);
   __shared__ unsigned long long s_vis[TS * TS];
   __shared__ uint32_t s_q[NW][QCAP];
+  __shared__ uint32_t s_nfrag;  // the tile's fragment count (one global atomic per tile)
 
   const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
   const DevView& vw = views[find_view_tile(views, n_views, tile)];
@@ -246,6 +247,7 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
   // early-Z test alone keeps them uncovered.
   for (int p = tid; p < TS * TS; p += NT)
     s_vis[p] = (tx0 + p % TS < W && ty0 + p / TS < H) ? ~0ull : 0ull;
+  if (tid == 0) s_nfrag = 0;
 
   const int lx = lane & 7, ly = lane >> 3;
   const uint64_t lanemask_lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -527,7 +529,12 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
       if (qn) flush();
     }
   }
+  // A wave with fragments ran the chunk loop, whose barriers order this
+  // after the zeroing above.  Same-address atomics from every wave of a large
+  // batch serialize in memory; one per tile keeps them off the critical path.
+  if (nfrag && lane == 0) atomicAdd(&s_nfrag, nfrag);  // nfrag is wave-uniform
   __syncthreads();
+  if (tid == 0 && s_nfrag) atomicAdd(&status->frags, s_nfrag);
   if constexpr (SHADE) {
     // each wave shades 64 consecutive pixels of the tile at a time, its
     // record runs in its share of the chunk area
@@ -550,7 +557,6 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
       }
     }
   }
-  if (lane == 0) atomicAdd(&status->frags, nfrag);  // nfrag is wave-uniform
 }
 
 // Triangle-parallel variant of the per-tile visibility pass.  The waves of a
@@ -577,6 +583,7 @@ __global__ __launch_bounds__(64 * NW) void k_raster_tp(
   __shared__ unsigned long long s_vis[TS * TS];
   __shared__ uint32_t s_q[NW][QCAP];
   __shared__ uint32_t s_next;
+  __shared__ uint32_t s_nfrag;  // the tile's fragment count (one global atomic per tile)
 
   const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
   const DevView& vw = views[find_view_tile(views, n_views, tile)];
@@ -590,6 +597,7 @@ __global__ __launch_bounds__(64 * NW) void k_raster_tp(
   const uint2 rg = ranges[tile];
   const uint64_t lanemask_lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   for (int p = tid; p < TS * TS; p += NT) s_vis[p] = ~0ull;
+  if (tid == 0) s_nfrag = 0;
   uint32_t qn = 0, nfrag = 0;
 
   auto submit = [&](const SetupRec& r, uint32_t id, int px, int py) {
@@ -708,7 +716,9 @@ __global__ __launch_bounds__(64 * NW) void k_raster_tp(
     }
     if (qn) flush();
   }
+  if (nfrag && lane == 0) atomicAdd(&s_nfrag, nfrag);  // after the loop's barriers, as in k_raster
   __syncthreads();
+  if (tid == 0 && s_nfrag) atomicAdd(&status->frags, s_nfrag);
   for (int p = tid; p < TS * TS; p += NT) {
     const int px = tx0 + p % TS, py = ty0 + p / TS;
     if (px < W && py < H) {
@@ -716,7 +726,6 @@ __global__ __launch_bounds__(64 * NW) void k_raster_tp(
       vis_out[vw.px_base + (uint32_t)py * (uint32_t)W + (uint32_t)px] = (v == ~0ull) ? kNone : (uint32_t)v;
     }
   }
-  if (lane == 0) atomicAdd(&status->frags, nfrag);
 }
 
 // One thread per pixel of the visibility buffer (see shade_wave).
