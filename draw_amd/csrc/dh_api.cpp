@@ -941,7 +941,8 @@ int render_views(dh_ctx c, const dh_view* views, uint32_t n_views, bool use_grap
     const dh::Status st2 = transparent ? c->h_status[1] : dh::Status{};
     c->stats.tris_out = (uint64_t)st.n_out + st2.n_out;
     c->stats.bin_entries = (uint64_t)st.n_pairs + st2.n_pairs;
-    c->stats.frags = st.frags;
+    c->stats.frags = 0;
+    for (uint32_t i = 0; i < dh::kFragWords; i++) c->stats.frags += st.frags[i];
     if (st.overflow || st2.overflow) {
       c->stats.retries++;
       const uint64_t n_out = std::max<uint32_t>(st.n_out, st2.n_out);

@@ -30,15 +30,20 @@ enum : uint32_t {
 };
 
 // Status words written by the kernels and read back once per frame.
+constexpr uint32_t kFragWords = 32;
+
 struct Status {
   uint32_t n_out;     // clipped triangles emitted (may exceed capacity)
   uint32_t n_pairs;   // (tile, triangle) bin entries (may exceed capacity)
   uint32_t overflow;  // bit0: triangle records, bit1: bin entries
   uint32_t error;     // bit0: texture slice out of range (scene/mod.rs:167 panic)
-  uint32_t frags;     // covered fragments submitted to the visibility test
   uint32_t n_simple;  // input triangles emitted unclipped (inside the near and far planes)
   uint32_t n_clip;    // input triangles that emit after near/far clipping
   uint32_t n_wide;    // triangles on the wide-triangle bin list
+  // covered fragments submitted to the visibility test: the sum of these
+  // words, tile t adding to word t % kFragWords (one address for every tile
+  // of a large batch serializes their atomics in memory)
+  uint32_t frags[kFragWords];
 };
 
 // Material = the reference's `Texture` (scene/mod.rs:206-216) with device maps.
@@ -172,22 +177,35 @@ __device__ inline uint32_t depth_key(float d) {
 }
 
 // Views are stored in increasing tile_base / px_base order: the view owning a
-// global tile or pixel index is the last one whose base is <= it.
-__device__ inline uint32_t find_view_tile(const DevView* views, uint32_t n, uint32_t t) {
-  uint32_t lo = 0, hi = n;
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (views[mid].tile_base <= t) lo = mid; else hi = mid;
+// global tile or pixel index is the last one whose base is <= it.  A whole
+// wave searches at once, for a t (or p0) that every lane holds: each round
+// the 64 lanes load evenly spaced candidates and a ballot keeps the span from
+// the last one at or below t to the next, so two rounds of loads cover up to
+// 4096 views where a binary search takes log2(n) dependent loads.  Every lane
+// of the wave must be active.
+template <typename Key>
+__device__ inline uint32_t wave_find_view(const DevView* views, uint32_t n, uint32_t t, Key key) {
+  const uint32_t lane = threadIdx.x & 63u;
+  uint32_t lo = 0, len = n;  // views[lo] is at or below t (views[0] starts at 0)
+  while (len > 1) {
+    const uint32_t step = (len + 63u) >> 6;
+    const uint32_t j = lane * step;
+    const bool le = j < len && key(views[lo + j]) <= t;
+    const uint32_t k = (uint32_t)__popcll(__ballot(le)) - 1u;  // lane 0 is always set
+    lo += k * step;
+    len = min(step, len - k * step);
   }
   return lo;
 }
-__device__ inline uint32_t find_view_px(const DevView* views, uint32_t n, uint32_t p) {
-  uint32_t lo = 0, hi = n;
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (views[mid].px_base <= p) lo = mid; else hi = mid;
-  }
-  return lo;
+__device__ inline uint32_t wave_find_view_tile(const DevView* views, uint32_t n, uint32_t t) {
+  return wave_find_view(views, n, t, [](const DevView& v) { return v.tile_base; });
+}
+// View of pixel p >= p0, where p0 (the same in every lane) is the wave's
+// first pixel: the view of p0, then the (rare) view starts between p0 and p.
+__device__ inline uint32_t wave_find_view_px(const DevView* views, uint32_t n, uint32_t p0, uint32_t p) {
+  uint32_t v = wave_find_view(views, n, p0, [](const DevView& w) { return w.px_base; });
+  while (v + 1u < n && views[v + 1u].px_base <= p) v++;
+  return v;
 }
 
 // XCD-aware block order: workgroups are dispatched round-robin over the 8

@@ -232,7 +232,7 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
   __shared__ uint32_t s_nfrag;  // the tile's fragment count (one global atomic per tile)
 
   const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
-  const DevView& vw = views[find_view_tile(views, n_views, tile)];
+  const DevView& vw = views[wave_find_view_tile(views, n_views, tile)];
   const uint32_t lt = tile - vw.tile_base;
   const int tx = (int)(lt % vw.ntx), ty = (int)(lt / vw.ntx);
   const int W = vw.fr.W, H = vw.fr.H;
@@ -531,10 +531,10 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
   }
   // A wave with fragments ran the chunk loop, whose barriers order this
   // after the zeroing above.  Same-address atomics from every wave of a large
-  // batch serialize in memory; one per tile keeps them off the critical path.
+  // batch serialize in memory: one per tile, spread over kFragWords words.
   if (nfrag && lane == 0) atomicAdd(&s_nfrag, nfrag);  // nfrag is wave-uniform
   __syncthreads();
-  if (tid == 0 && s_nfrag) atomicAdd(&status->frags, s_nfrag);
+  if (tid == 0 && s_nfrag) atomicAdd(&status->frags[tile % kFragWords], s_nfrag);
   if constexpr (SHADE) {
     // each wave shades 64 consecutive pixels of the tile at a time, its
     // record runs in its share of the chunk area
@@ -586,7 +586,7 @@ __global__ __launch_bounds__(64 * NW) void k_raster_tp(
   __shared__ uint32_t s_nfrag;  // the tile's fragment count (one global atomic per tile)
 
   const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
-  const DevView& vw = views[find_view_tile(views, n_views, tile)];
+  const DevView& vw = views[wave_find_view_tile(views, n_views, tile)];
   const uint32_t lt = tile - vw.tile_base;
   const int tx0 = (int)(lt % vw.ntx) * TS, ty0 = (int)(lt / vw.ntx) * TS;
   const int W = vw.fr.W, H = vw.fr.H;
@@ -718,7 +718,7 @@ __global__ __launch_bounds__(64 * NW) void k_raster_tp(
   }
   if (nfrag && lane == 0) atomicAdd(&s_nfrag, nfrag);  // after the loop's barriers, as in k_raster
   __syncthreads();
-  if (tid == 0 && s_nfrag) atomicAdd(&status->frags, s_nfrag);
+  if (tid == 0 && s_nfrag) atomicAdd(&status->frags[tile % kFragWords], s_nfrag);
   for (int p = tid; p < TS * TS; p += NT) {
     const int px = tx0 + p % TS, py = ty0 + p / TS;
     if (px < W && py < H) {
@@ -736,11 +736,13 @@ __global__ __launch_bounds__(256) void k_shade(const DevView* __restrict__ views
                                                const ShadeRec* __restrict__ shade,
                                                const DevMaterial* __restrict__ mats, Status* __restrict__ status) {
   __shared__ uint4 s_run[4][kShadeRuns][14];  // setup bytes 0-79, then the 144 shade bytes
-  const uint32_t wave = threadIdx.x >> 6;
-  const uint32_t g = xcd_remap(blockIdx.x, gridDim.x) * 256u + threadIdx.x;
+  const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const uint32_t g0 = xcd_remap(blockIdx.x, gridDim.x) * 256u + wave * 64u;  // the wave's first pixel
+  const uint32_t g = g0 + (threadIdx.x & 63u);
   const uint32_t id = g < n_pixels ? vis[g] : kSkip;
   const bool active = id != kSkip;  // (kSkip: shaded by another shard)
-  const DevView& vw = views[active ? find_view_px(views, n_views, g) : 0u];
+  const uint32_t v = wave_find_view_px(views, n_views, g0, g);
+  const DevView& vw = views[active ? v : 0u];
   const uint32_t p = active ? g - vw.px_base : 0u;
   const uint32_t W = (uint32_t)vw.fr.W;
   const uint32_t y = p / W, x = p - y * W;
