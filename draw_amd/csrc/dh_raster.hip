@@ -80,8 +80,18 @@ __device__ inline void fetch_rgb(const uint8_t* img, int32_t comp, uint64_t off,
   }
 }
 
+// k / 255 for every byte k (Pixel::normalized_as_vec3, canvas.rs:81-87),
+// divided once per workgroup into an LDS table that the shading reads, the
+// block's nt threads filling it together.  The caller places a barrier
+// between the fill and the first shade_pixel.
+__device__ inline void fill_unit_table(float* unit, int tid, int nt) {
+  for (int k = tid; k < 256; k += nt) unit[k] = (float)k / 255.0f;
+}
+
 // Shading of a covered pixel, canvas.rs:685-741; returns B | G<<8 | R<<16 | A<<24.
-__device__ inline uint32_t shade_pixel(const ShadeRec& s, const DevMaterial& m, const FragOut& o, Status* status) {
+// unit: the k / 255 table (fill_unit_table).
+__device__ inline uint32_t shade_pixel(const ShadeRec& s, const DevMaterial& m, const FragOut& o, Status* status,
+                                       const float* unit) {
   const float u = (s.uv[0] * o.al + s.uv[2] * o.be) + s.uv[4] * o.ga;
   const float v = (s.uv[1] * o.al + s.uv[3] * o.be) + s.uv[5] * o.ga;
   uint64_t okd, oka;
@@ -95,8 +105,8 @@ __device__ inline uint32_t shade_pixel(const ShadeRec& s, const DevMaterial& m, 
   uint32_t dk[3], ak[3];
   fetch_rgb(m.kd_img, m.kd_comp, okd, dk);
   fetch_rgb(m.ka_img, m.ka_comp, oka, ak);
-  const V3 diffuse = v3((float)dk[0] / 255.0f, (float)dk[1] / 255.0f, (float)dk[2] / 255.0f);
-  const V3 ambient = v3((float)ak[0] / 255.0f, (float)ak[1] / 255.0f, (float)ak[2] / 255.0f);
+  const V3 diffuse = v3(unit[dk[0]], unit[dk[1]], unit[dk[2]]);
+  const V3 ambient = v3(unit[ak[0]], unit[ak[1]], unit[ak[2]]);
   const V3 pn = add(add(muls(v3(s.n[0], s.n[1], s.n[2]), o.al), muls(v3(s.n[3], s.n[4], s.n[5]), o.be)),
                     muls(v3(s.n[6], s.n[7], s.n[8]), o.ga));
   const V3 pl = add(add(muls(v3(s.l[0], s.l[1], s.l[2]), o.al), muls(v3(s.l[3], s.l[4], s.l[5]), o.be)),
@@ -128,7 +138,7 @@ template <int R>
 __device__ inline void shade_wave(uint32_t id, bool active, uint32_t x, uint32_t y, const DevView& vw,
                                   const SetupRec* __restrict__ setup, const ShadeRec* __restrict__ shade,
                                   const DevMaterial* __restrict__ mats, Status* __restrict__ status,
-                                  uint4 (*run)[14]) {
+                                  uint4 (*run)[14], const float* unit) {
   const uint32_t lane = threadIdx.x & 63u;
   const bool covered = active && id != kNone;
   const uint32_t key = covered ? id : kSkip;
@@ -179,9 +189,9 @@ __device__ inline void shade_wave(uint32_t id, bool active, uint32_t x, uint32_t
     // one material for the wave's covered pixels: scalar loads
     const uint32_t m0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)sh.material);
     if (__ballot(sh.material != m0) == 0ull)
-      bgra = shade_pixel(sh, mats[m0], o, status);
+      bgra = shade_pixel(sh, mats[m0], o, status, unit);
     else
-      bgra = shade_pixel(sh, mats[sh.material], o, status);
+      bgra = shade_pixel(sh, mats[sh.material], o, status, unit);
     depth = o.d;
   }
   const uint32_t p = y * W + x;
@@ -230,6 +240,7 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
   __shared__ unsigned long long s_vis[TS * TS];
   __shared__ uint32_t s_q[NW][QCAP];
   __shared__ uint32_t s_nfrag;  // the tile's fragment count (one global atomic per tile)
+  __shared__ float s_unit[SHADE ? 256 : 1];
 
   const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
   const DevView& vw = views[wave_find_view_tile(views, n_views, tile)];
@@ -248,6 +259,7 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
   for (int p = tid; p < TS * TS; p += NT)
     s_vis[p] = (tx0 + p % TS < W && ty0 + p / TS < H) ? ~0ull : 0ull;
   if (tid == 0) s_nfrag = 0;
+  if constexpr (SHADE) fill_unit_table(s_unit, tid, NT);  // read after the barrier before the shading
 
   const int lx = lane & 7, ly = lane >> 3;
   const uint64_t lanemask_lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -546,7 +558,7 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
       const bool on = px < W && py < H;
       const unsigned long long v = s_vis[p];
       shade_wave<R>((v == ~0ull) ? kNone : (uint32_t)v, on, (uint32_t)px, (uint32_t)py, vw, setup, shade, mats,
-                    status, run);
+                    status, run, s_unit);
     }
   } else {
     for (int p = tid; p < TS * TS; p += NT) {
@@ -736,6 +748,9 @@ __global__ __launch_bounds__(256) void k_shade(const DevView* __restrict__ views
                                                const ShadeRec* __restrict__ shade,
                                                const DevMaterial* __restrict__ mats, Status* __restrict__ status) {
   __shared__ uint4 s_run[4][kShadeRuns][14];  // setup bytes 0-79, then the 144 shade bytes
+  __shared__ float s_unit[256];
+  fill_unit_table(s_unit, (int)threadIdx.x, 256);
+  __syncthreads();
   const uint32_t wave = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const uint32_t g0 = xcd_remap(blockIdx.x, gridDim.x) * 256u + wave * 64u;  // the wave's first pixel
   const uint32_t g = g0 + (threadIdx.x & 63u);
@@ -746,7 +761,7 @@ __global__ __launch_bounds__(256) void k_shade(const DevView* __restrict__ views
   const uint32_t p = active ? g - vw.px_base : 0u;
   const uint32_t W = (uint32_t)vw.fr.W;
   const uint32_t y = p / W, x = p - y * W;
-  shade_wave<kShadeRuns>(id, active, x, y, vw, setup, shade, mats, status, s_run[wave]);
+  shade_wave<kShadeRuns>(id, active, x, y, vw, setup, shade, mats, status, s_run[wave], s_unit);
 }
 
 
@@ -784,11 +799,13 @@ __global__ __launch_bounds__(256) void k_blend(const DevView* __restrict__ views
   constexpr int CH = 64;
   __shared__ SetupRec s_rec[CH];
   __shared__ uint32_t s_id[CH];
+  __shared__ float s_unit[256];
   const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
   const DevView& vw = views[0];
   const int W = vw.fr.W, H = vw.fr.H;
   const int tx0 = (int)(tile % vw.ntx) * TS, ty0 = (int)(tile / vw.ntx) * TS;
   const int tid = threadIdx.x;
+  fill_unit_table(s_unit, tid, 256);  // read after the chunk loop's first barrier
   uint32_t col[PPT];
   float D[PPT];
   int gw[PPT];
@@ -854,7 +871,7 @@ __global__ __launch_bounds__(256) void k_blend(const DevView* __restrict__ views
         // both maps are sampled before the depth test (canvas.rs:689-695)
         const ShadeRec& sr = shade[s_id[k]];
         const DevMaterial& m = mats[sr.material];
-        const uint32_t c = shade_pixel(sr, m, o, status);
+        const uint32_t c = shade_pixel(sr, m, o, status, s_unit);
         if (o.d < D[q] && (int)sr.group >= gw[q]) col[q] = blend_px(col[q], c, m.alpha);
       }
     }
