@@ -80,3 +80,23 @@ def test_batch_of_256_views_matches_oracle_on_samples(renderer):
         m = np.zeros((s.height, s.width), bool)
         m[grid[3]::grid[1], grid[2]::grid[0]] = True
         assert_same(f, d, w, ref, mask=m, label=f"view {k}")
+
+
+@pytest.mark.parametrize("kind,tile,waves", [(0, 16, 4), (1, 16, 4)])
+def test_batch_of_tiny_views_matches_oracle(renderer, kind, tile, waves):
+    """150 views of 3-24 pixels a side, most of them smaller than a wave's 64
+    pixels: a shading wave's pixels then span several views, and the raster
+    workgroups search 150 view bases.  Both raster kinds (kind 1 shades in a
+    separate pass over the visibility buffer)."""
+    rng = np.random.default_rng(12)
+    views = [workloads.small_random_scene(100 + k, n_tri=40, width=int(rng.integers(3, 25)),
+                                          height=int(rng.integers(3, 25)), tex_size=8) for k in range(150)]
+    renderer.set_raster(kind, tile, waves)
+    try:
+        cvs = renderer.render_batch([(s, None) for s in views])
+        outs = [_read(renderer, cv, s) for cv, s in zip(cvs, views)]
+    finally:
+        renderer.set_raster(0, 16, 4)
+    assert sum(s.width * s.height < 64 for s in views) > 10
+    for k, (s, (f, d, w)) in enumerate(zip(views, outs)):
+        assert_same(f, d, w, oracle(s), label=f"tiny view {k} ({s.width}x{s.height}, kind {kind})")
