@@ -7,11 +7,12 @@ complete on the device before the next (one-frame-per-launch mode).  Inputs
 (meshes, texture, canvas) are resident in HBM before the timed region.
 
 Prints ONE JSON line: shaded Mpix/s (pixels with a depth-test winner per
-second) as ``value``, plus triangles/s, p50/p99 frame latency, per-stage device
-times, the roofline of the raster (the dominant kernel, which shades its own
-tiles in the default raster kind), the batched mode
-against the plain path on 256 views, and the CPU oracle
-timed on a bounded prefix of the same workload on this host.
+second) as ``value``, plus triangles/s, p50/p99 frame latency, the frame time
+with the canvas read back to the host, per-stage device times, the roofline of
+the raster (the dominant kernel, which shades its own tiles in the default
+raster kind), the 640x480 torus and 3840x2160 20M-triangle workloads, the
+batched mode against the plain path on 256 views, the serving workload, and
+the CPU oracle timed on whole frames of the same workloads on this host.
 """
 from __future__ import annotations
 
@@ -131,26 +132,91 @@ def roofline(kernel: str, nbytes: dict, ms: float, traffic=None) -> dict:
             "algorithmic_bytes": nbytes["algorithmic"], "record_bytes": nbytes["record"], "kernel_ms": round(ms, 4)}
 
 
-def cpu_baseline(scene, n_sample: int, p_cov_full: int) -> dict:
-    """Oracle (C restatement of the reference, single thread) on the first
-    n_sample input triangles of the same frame; the frame time is scaled by
-    n_total / n_sample (the soup's submission order is a random permutation)."""
+def cpu_model() -> str:
+    """The host CPU's model name (from /proc/cpuinfo) and the usable core count."""
+    name = platform.processor() or platform.machine()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                name = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return f"{name}, {len(os.sched_getaffinity(0))} usable cores"
+
+
+def cpu_frames(scene, runs: int, warmup: int) -> dict:
+    """The oracle (C restatement of the reference, one thread) rendering the
+    whole frame: ``warmup`` untimed frames, then the median of ``runs`` timed
+    ones (host wall clock around the full render)."""
     from oracle import oracle as O
-    n_total = scene.triangle_count
     fr = O.frame_uniforms(scene)
-    t0 = time.perf_counter()
-    r = O.render(scene, frame=fr, tri_range=(0, n_sample))
-    dt = time.perf_counter() - t0
-    assert r.rc == 0
-    est = dt * n_total / n_sample
+    for _ in range(warmup):
+        O.render(scene, frame=fr)
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        r = O.render(scene, frame=fr)
+        times.append(time.perf_counter() - t0)
+        assert r.rc == 0
+    covered = int(np.count_nonzero(r.winner != 0xFFFFFFFF))
+    t = float(np.median(times))
+    return {"frame_s": round(t, 4), "runs": runs, "warmup": warmup, "covered_px": covered,
+            "shaded_mpix_per_s": round(covered / t / 1e6, 4), "tris_per_s": round(scene.triangle_count / t, 1)}
+
+
+def cpu_baseline(scene, p_cov_full: int) -> dict:
+    """The oracle (one thread) on one whole frame of the headline workload:
+    the 1920x1080 1M-triangle soup, timed once (a frame takes seconds)."""
+    c = cpu_frames(scene, runs=1, warmup=0)
+    assert c["covered_px"] == p_cov_full, "the oracle's frame must cover what the GPU frame covers"
     return {
-        "value": round(p_cov_full / est / 1e6, 4), "unit": "shaded Mpix/s", "cores": 1, "kind": "port",
-        "sample": f"first {n_sample} of {n_total} input triangles of the same 1920x1080 frame, "
-                  f"{dt:.2f} s on {platform.processor() or platform.machine()} (nproc {os.cpu_count()}); "
-                  f"frame time scaled x{n_total / n_sample:.2f} = {est:.2f} s",
-        "tris_per_s": round(n_total / est, 1),
-        "frame_s_est": round(est, 3),
+        "value": c["shaded_mpix_per_s"], "unit": "shaded Mpix/s", "cores": 1, "kind": "port",
+        "sample": f"one whole 1920x1080 frame of the 1M-triangle soup (all {scene.triangle_count} input "
+                  f"triangles), {c['frame_s']:.2f} s on {cpu_model()}",
+        "tris_per_s": c["tris_per_s"], "frame_s": c["frame_s"],
     }
+
+
+def bench_config(name: str, scene, steps: int, warmup: int, cpu_runs: int, cpu_warmup: int) -> dict:
+    """One more plain-path workload at its configured size: autotuned raster,
+    ``steps`` timed frames (one dh_render each, graph-replayed), p50/p99 frame
+    latency, and the oracle on whole frames of the same scene on this host."""
+    from draw_amd.autotune import autotune
+    from draw_amd.render import Renderer, frame_uniforms
+    fr = frame_uniforms(scene)
+    r = Renderer(0, profiling=True)
+    meshes = r.mesh_handles(scene)
+    (kind, tile, waves), table = autotune(r, lambda: r.render_device(scene, fr, meshes))
+    for _ in range(warmup):
+        r.render_device(scene, fr, meshes)
+    lat = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        a = time.perf_counter()
+        cv = r.render_device(scene, fr, meshes)
+        lat.append(time.perf_counter() - a)
+    total = time.perf_counter() - t0
+    _, _, winner = r.read(cv, fr.W, fr.H)
+    p_cov, t_vis = visible_triangles(winner)
+    st = r.stats()
+    r.close()
+    out = {"workload": name, "width": fr.W, "height": fr.H, "triangles": scene.triangle_count,
+           "raster_kind": kind, "tile": tile, "waves_per_tile": waves, "steps": steps,
+           "shaded_mpix_per_s": round(p_cov * steps / total / 1e6, 2),
+           "tris_per_s": round(scene.triangle_count * steps / total, 1),
+           "ms_per_frame": round(total / steps * 1e3, 4),
+           "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 4),
+                          "p99": round(float(np.percentile(lat, 99)) * 1e3, 4)},
+           "frame": {"covered_px": p_cov, "visible_tris": t_vis, "clipped_tris": int(st["tris_out"])}}
+    if cpu_runs > 0:
+        c = cpu_frames(scene, cpu_runs, cpu_warmup)
+        assert c["covered_px"] == p_cov
+        c["cores"] = 1
+        c["host"] = cpu_model()
+        out["cpu_oracle"] = c
+        out["gpu_over_cpu"] = round(out["shaded_mpix_per_s"] / c["shaded_mpix_per_s"], 1)
+    return out
 
 
 def bench_batched(reps: int) -> dict:
@@ -388,12 +454,14 @@ def main():
     ap.add_argument("--tile", type=int, default=16)
     ap.add_argument("--waves", type=int, default=4)
     ap.add_argument("--n-tri", type=int, default=1_000_000)
-    ap.add_argument("--cpu-sample", type=int, default=400_000, help="0 disables the CPU baseline")
+    ap.add_argument("--cpu", type=int, default=1, help="0 disables the CPU oracle timings")
+    ap.add_argument("--configs", type=int, default=1,
+                    help="0 skips the 640x480 torus and the 3840x2160 20M-triangle legs")
     ap.add_argument("--batched-reps", type=int, default=5, help="timed batched launches; 0 skips the batched run")
     ap.add_argument("--shard-steps", type=int, default=5,
                     help="timed calls of the 1-of-8 sort-last shard leg; 0 skips it")
-    ap.add_argument("--serve-seconds", type=float, default=20.0,
-                    help="length of the Poisson serving stream (the serving config names 60 s); 0 skips serving")
+    ap.add_argument("--serve-seconds", type=float, default=60.0,
+                    help="length of the Poisson serving stream (60 s, as the serving config names); 0 skips serving")
     ap.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate of the serving stream")
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="use --tile/--waves instead of the autotuner's choice")
@@ -438,6 +506,17 @@ def main():
     total = time.perf_counter() - t0
     _, _, winner = r.read(cv, W, H)
     p_cov, t_vis = visible_triangles(winner)
+    # the drop-in's whole Scene::render: the frame plus dh_canvas_read of the
+    # BGRA8 frame and f32 depth into caller-owned (pageable) host buffers
+    host_bgra = np.empty((H, W, 4), np.uint8)
+    host_depth = np.empty((H, W), np.float32)
+    rb = []
+    for _ in range(max(5, args.steps // 2)):
+        a = time.perf_counter()
+        cv = r.render_device(scene, fr, meshes)
+        r.read_into(cv, host_bgra, host_depth)
+        rb.append(time.perf_counter() - a)
+    rb_ms = float(np.median(rb)) * 1e3
     # per-stage HIP event times need eager launches (a replayed graph records
     # only the whole frame): the same frame again, untimed, without the graph
     r.set_graph(False)
@@ -461,15 +540,21 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(frame_ms, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "plain GPU path, 1920x1080, seeded 1M-triangle soup (20% near-plane straddlers, "
-                               "log-normal area median 40 px), 2048x2048 RGBA8 texture, one frame per dh_render "
-                               "(launch sequence replayed from a hipGraph)",
-                   "width": W, "height": H, "triangles": scene.triangle_count, "raster_kind": args.raster_kind,
+                               "log-normal area median 40 px, random winding: half the triangles face away and "
+                               "are culled), 2048x2048 RGBA8 texture, one frame per dh_render into a canvas "
+                               "resident in HBM (launch sequence replayed from a hipGraph)",
+                   "width": W, "height": H, "triangles": scene.triangle_count, "back_facing_frac": 0.5,
+                   "raster_kind": args.raster_kind,
                    "tile": args.tile,
                    "waves_per_tile": args.waves, "autotune_ms": tune_table},
         "tris_per_s": round(scene.triangle_count * args.steps / total, 1),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)) * 1e3, 4),
                        "p99": round(float(np.percentile(lat, 99)) * 1e3, 4)},
         "device_ms_per_frame": round(float(np.mean(graph_ms)), 4),
+        "readback_inclusive": {"what": "dh_render + dh_canvas_read of BGRA8 and f32 depth (16.6 MB) into "
+                                       "pageable host memory, median per frame",
+                               "ms_per_frame": round(rb_ms, 4),
+                               "shaded_mpix_per_s": round(p_cov / rb_ms / 1e3, 2)},
         "stage_ms_eager": {k: round(st[k], 4) for k in ("ms_geometry", "ms_binning", "ms_raster", "ms_shade",
                                                         "ms_frame")},
         "frame": {"covered_px": p_cov, "visible_tris": t_vis, "clipped_tris": int(st["tris_out"]),
@@ -487,9 +572,19 @@ def main():
         out["batched"] = bench_batched(args.batched_reps)
     if args.serve_seconds > 0:
         out["serving"] = bench_serving(args.serve_seconds, args.serve_rate)
-    if args.cpu_sample > 0:
-        log(f"CPU baseline on {args.cpu_sample} triangles ...")
-        out["cpu_baseline"] = cpu_baseline(scene, min(args.cpu_sample, scene.triangle_count), p_cov)
+    if args.configs > 0:
+        log("torus 640x480 ...")
+        out["torus_640x480"] = bench_config(
+            "2304-triangle torus (R 1.0, r 0.4, 48x24) at 640x480, 256x256 checker, one dh_render per frame",
+            workloads.torus_scene(1), steps=200, warmup=20, cpu_runs=5 if args.cpu else 0, cpu_warmup=1)
+        log("shells 3840x2160, 20M triangles ...")
+        out["shells_20m_4k"] = bench_config(
+            "20M micro-triangles on 8 nested jittered shells at 3840x2160 (5% flipped, 5% outside the frustum), "
+            "8192x8192 texture, one dh_render per frame", workloads.shells_scene(3), steps=20, warmup=3,
+            cpu_runs=1 if args.cpu else 0, cpu_warmup=0)
+    if args.cpu > 0:
+        log("CPU baseline: one whole frame of the soup ...")
+        out["cpu_baseline"] = cpu_baseline(scene, p_cov)
     print(json.dumps(out), flush=True)
 
 

@@ -107,6 +107,8 @@ _P = C.c_void_p
 _H = C.c_uint64
 SIGNATURES = {
     "dh_frame_uniforms": [C.POINTER(DhCamera), C.c_int32, C.c_int32, C.POINTER(C.c_float), C.POINTER(DhFrameParams)],
+    "dh_frame_uniforms_canvas": [C.POINTER(DhCamera)] + [C.c_int32] * 6 + [C.POINTER(C.c_float),
+                                                                          C.POINTER(DhFrameParams)],
     "dh_ctx_create": [C.c_int32, C.POINTER(_P)],
     "dh_ctx_create_shared": [_P, C.POINTER(_P)],
     "dh_ctx_destroy": [_P],

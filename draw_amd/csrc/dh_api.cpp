@@ -79,6 +79,11 @@ struct DevBuf {
 // on the same device (dh_ctx_create_shared); every access holds mu.
 struct ResPool {
   std::mutex mu;
+  // held by a frame with a transparent pass from its launch until it has
+  // completed on the device: that pass sorts each transparent mesh's index
+  // buffer in place (scene/mod.rs:1100), so two contexts sharing the pool
+  // must not sort one mesh at the same time
+  std::mutex trans_mu;
   int device = 0;
   uint64_t next_id = 1;
   std::unordered_map<uint64_t, TexRes> tex;
@@ -846,7 +851,11 @@ int render_views(dh_ctx c, const dh_view* views, uint32_t n_views, bool use_grap
     mats = c->res->mat_dev.p;
   }
   const bool transparent = !c->h_tdraws.empty();
-  if (transparent) use_graph = false;
+  std::unique_lock<std::mutex> trans_lock(c->res->trans_mu, std::defer_lock);
+  if (transparent) {
+    use_graph = false;
+    trans_lock.lock();  // until this function returns, after the stream has synchronised
+  }
   c->shard_frame = false;
   const uint64_t n_max = std::max<uint64_t>(n_in, c->n_tin);
   key.push_back((uint64_t)(uintptr_t)mats);

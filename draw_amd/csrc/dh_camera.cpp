@@ -109,11 +109,15 @@ int view_plane(V3 a, V3 b, V3 c, V3 visible, float out[4]) {
   return DH_OK;
 }
 
-}  // namespace
-
-extern "C" int dh_frame_uniforms(const dh_camera* cam, int32_t W, int32_t H, const float light[3],
-                                 dh_frame_params* out) {
-  if (!cam || !out || W <= 0 || H <= 0) return DH_E_ARG;
+// The uniforms of a Scene of screen size W x H (Scene::new, scene/mod.rs:760-786:
+// the camera's aspect ratio and the viewport matrix use the screen size)
+// drawn into a canvas of cw x ch pixels whose origin sits at (ox, oy) on that
+// screen (Canvas::new + Canvas::apply_offset, canvas.rs:366, 382-385): the
+// raster's screen rectangle is the canvas (canvas.rs:647) and every snapped
+// vertex is screen_coord - offset (canvas.rs:585-587).
+int uniforms(const dh_camera* cam, int32_t W, int32_t H, int32_t cw, int32_t ch, int32_t ox, int32_t oy,
+             const float light[3], dh_frame_params* out) {
+  if (!cam || !out || W <= 0 || H <= 0 || cw <= 0 || ch <= 0) return DH_E_ARG;
   memset(out, 0, sizeof(*out));
   // Camera::new: the window half-extents from |near| and fov_x.
   const float ratio = (float)W / (float)H;  // Scene::new, scene/mod.rs:768
@@ -171,14 +175,27 @@ extern "C" int dh_frame_uniforms(const dh_camera* cam, int32_t W, int32_t H, con
   out->light[0] = L[0];
   out->light[1] = L[1];
   out->light[2] = L[2];
-  out->W = W;
-  out->H = H;
-  out->off_x = 0.0f;
-  out->off_y = 0.0f;
+  out->W = cw;
+  out->H = ch;
+  out->off_x = (float)ox;  // Canvas::apply_offset: x as f32 (canvas.rs:382-385)
+  out->off_y = (float)oy;
   out->clear_depth = 100000.0f;  // app/mod.rs:80
   out->clear_bgra[0] = 255;      // Pixel::azul_bb = (r 155, g 186, b 255), canvas.rs:131-133
   out->clear_bgra[1] = 186;
   out->clear_bgra[2] = 155;
   out->clear_bgra[3] = 255;
   return DH_OK;
+}
+
+}  // namespace
+
+extern "C" int dh_frame_uniforms(const dh_camera* cam, int32_t W, int32_t H, const float light[3],
+                                 dh_frame_params* out) {
+  return uniforms(cam, W, H, W, H, 0, 0, light, out);
+}
+
+extern "C" int dh_frame_uniforms_canvas(const dh_camera* cam, int32_t screen_w, int32_t screen_h, int32_t canvas_w,
+                                        int32_t canvas_h, int32_t off_x, int32_t off_y, const float light[3],
+                                        dh_frame_params* out) {
+  return uniforms(cam, screen_w, screen_h, canvas_w, canvas_h, off_x, off_y, light, out);
 }

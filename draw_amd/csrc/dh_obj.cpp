@@ -27,6 +27,8 @@
 #include <zlib.h>
 
 #include <algorithm>
+#include <exception>
+#include <new>
 #include <fstream>
 #include <sstream>
 #include <string>
@@ -42,6 +44,10 @@ using dh::sub;
 using dh::v3;
 
 namespace {
+
+// Largest decoded texture accepted from a file: 1 GiB (a 16384 x 16384 RGBA8
+// image).
+constexpr uint64_t kMaxImageBytes = 1ull << 30;
 
 struct ObjTexture {
   std::string name;
@@ -139,6 +145,16 @@ int png_decode(const std::vector<uint8_t>& f, std::vector<uint8_t>* img, int32_t
   }
   const size_t bpp = (size_t)std::max(1, chans * depth / 8);           // filter unit
   const size_t stride = ((size_t)w * chans * depth + 7) / 8;           // bytes per row
+  const int out_c = (ctype == 3) ? (trns.empty() ? 3 : 4) : chans;
+  // The header's size is untrusted: refuse one that the compressed stream
+  // cannot hold (deflate expands at most 1032:1) or whose decoded image
+  // exceeds kMaxImageBytes, before allocating for it.
+  const uint64_t raw_bytes = (uint64_t)(stride + 1) * h;
+  if (raw_bytes > (uint64_t)idat.size() * 1032u + 1024u ||
+      (uint64_t)w * h * (uint64_t)out_c > kMaxImageBytes) {
+    *err = "PNG size does not fit its data or the image limit";
+    return DH_E_ARG;
+  }
   std::vector<uint8_t> raw((stride + 1) * h);
   uLongf rn = (uLongf)raw.size();
   if (uncompress(raw.data(), &rn, idat.data(), (uLong)idat.size()) != Z_OK || rn != raw.size()) {
@@ -146,7 +162,6 @@ int png_decode(const std::vector<uint8_t>& f, std::vector<uint8_t>* img, int32_t
     return DH_E_ARG;
   }
   std::vector<uint8_t> cur(stride), prev(stride, 0);
-  const int out_c = (ctype == 3) ? (trns.empty() ? 3 : 4) : chans;
   img->assign((size_t)w * h * out_c, 0);
   for (uint32_t y = 0; y < h; y++) {
     const uint8_t ft = raw[(stride + 1) * y];
@@ -487,10 +502,20 @@ int load_obj(dh_obj o, const std::string& path) {
 extern "C" int dh_obj_load(const char* path, dh_obj* out, char* err, uint32_t err_len) {
   if (!path || !out) return DH_E_ARG;
   *out = nullptr;
-  dh_obj o = new dh_obj_s();
-  const int rc = load_obj(o, path);
+  dh_obj o = nullptr;
+  int rc;
+  try {
+    o = new dh_obj_s();
+    rc = load_obj(o, path);
+  } catch (const std::bad_alloc&) {
+    rc = DH_E_NOMEM;
+    if (o) o->err = "out of host memory";
+  } catch (const std::exception& e) {
+    rc = DH_E_ARG;
+    if (o) o->err = e.what();
+  }
   if (rc) {
-    if (err && err_len) snprintf(err, err_len, "%s", o->err.c_str());
+    if (err && err_len) snprintf(err, err_len, "%s", o ? o->err.c_str() : "out of host memory");
     delete o;
     return rc;
   }
@@ -536,8 +561,23 @@ extern "C" int dh_obj_texture(dh_obj o, uint32_t i, dh_obj_texture_desc* out) {
   return DH_OK;
 }
 
+namespace {
+int obj_upload(dh_ctx ctx, dh_obj o, dh_handle* object, dh_handle* meshes);
+}
+
 extern "C" int dh_obj_upload(dh_ctx ctx, dh_obj o, dh_handle* object, dh_handle* meshes, uint32_t cap) {
   if (!ctx || !o || !object || (o->meshes.size() && !meshes) || cap < o->meshes.size()) return DH_E_ARG;
+  try {
+    return obj_upload(ctx, o, object, meshes);
+  } catch (const std::bad_alloc&) {
+    return DH_E_NOMEM;
+  } catch (const std::exception&) {
+    return DH_E_ARG;
+  }
+}
+
+namespace {
+int obj_upload(dh_ctx ctx, dh_obj o, dh_handle* object, dh_handle* meshes) {
   int rc = dh_object_create(ctx, o->pos.data(), (uint32_t)(o->pos.size() / 3), o->nrm.data(),
                             (uint32_t)(o->nrm.size() / 3), o->uv.data(), (uint32_t)(o->uv.size() / 2), object);
   if (rc) return rc;
@@ -564,6 +604,7 @@ extern "C" int dh_obj_upload(dh_ctx ctx, dh_obj o, dh_handle* object, dh_handle*
   }
   return DH_OK;
 }
+}  // namespace
 
 extern "C" int dh_obj_free(dh_obj o) {
   if (!o) return DH_E_ARG;

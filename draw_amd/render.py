@@ -23,12 +23,20 @@ def _f3(v):
 
 
 def frame_uniforms(scene: Scene) -> DhFrameParams:
-    """a1 on the host: Camera::new + gen_transformation_matrix + gen_view_planes."""
+    """a1 on the host: Camera::new + gen_transformation_matrix + gen_view_planes
+    (dh_frame_uniforms), or with the scene's own canvas size and offset
+    (dh_frame_uniforms_canvas) when it has one."""
     cam = scene.camera
     cd = DhCamera(_f3(cam.position), _f3(cam.direction), float(cam.fov_x), float(cam.near), float(cam.far))
     fr = DhFrameParams()
-    rc = _lib.lib().dh_frame_uniforms(C.byref(cd), int(scene.width), int(scene.height), _f3(scene.light_source),
-                                      C.byref(fr))
+    if scene.canvas is None:
+        rc = _lib.lib().dh_frame_uniforms(C.byref(cd), int(scene.width), int(scene.height),
+                                          _f3(scene.light_source), C.byref(fr))
+    else:
+        cw, ch = scene.canvas_size
+        ox, oy = scene.canvas_offset
+        rc = _lib.lib().dh_frame_uniforms_canvas(C.byref(cd), int(scene.width), int(scene.height), cw, ch,
+                                                 int(ox), int(oy), _f3(scene.light_source), C.byref(fr))
     if rc != 0:
         raise DrawHipError(rc, "dh_frame_uniforms")
     return fr
@@ -255,6 +263,18 @@ class Renderer:
         self._check(self.L.dh_canvas_read(self.ctx, canvas_handle, bgra.ctypes.data, depth.ctypes.data,
                                           winner.ctypes.data), "dh_canvas_read")
         return bgra, depth, winner
+
+    def read_into(self, canvas_handle: int, bgra: np.ndarray, depth: np.ndarray | None = None,
+                  winner: np.ndarray | None = None) -> None:
+        """dh_canvas_read into caller-owned contiguous arrays (any may be None),
+        as Scene::render's caller reads the frame and depth into its Canvas."""
+        def ptr(a, dtype):
+            if a is None:
+                return None
+            assert a.dtype == dtype and a.flags["C_CONTIGUOUS"]
+            return a.ctypes.data
+        self._check(self.L.dh_canvas_read(self.ctx, canvas_handle, ptr(bgra, np.uint8), ptr(depth, np.float32),
+                                          ptr(winner, np.uint32)), "dh_canvas_read")
 
     def draw_gui(self, canvas_handle: int, vtx: np.ndarray, idx: np.ndarray, cmds: np.ndarray, font) -> None:
         """Gui::render onto a device canvas (dh_draw_gui): vtx/cmds in the

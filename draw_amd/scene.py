@@ -22,7 +22,7 @@ on the GPU).
 from __future__ import annotations
 
 from dataclasses import dataclass, field
-from typing import List, Optional, Sequence
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -156,12 +156,23 @@ class Camera:
 @dataclass
 class Scene:
     """scene/mod.rs:749-786: the reference's defaults are camera (0,0,150) and
-    light (0,300,300)."""
+    light (0,300,300).  ``width`` x ``height`` is the scene (screen) size.
+    ``canvas`` = (W, H) draws into a canvas of that size moved by
+    ``canvas_offset`` (Canvas::new + Canvas::apply_offset, as the reference
+    app does: an 800x600 canvas centred on the screen, app/mod.rs:59-63,
+    81-84); None draws into a canvas of the scene's size at offset 0."""
     width: int
     height: int
     camera: Camera = field(default_factory=Camera)
     light_source: Sequence[float] = (0.0, 300.0, 300.0)
     objects: List[Object] = field(default_factory=list)
+    canvas: Optional[Tuple[int, int]] = None
+    canvas_offset: Tuple[int, int] = (0, 0)
+
+    @property
+    def canvas_size(self) -> Tuple[int, int]:
+        return (int(self.width), int(self.height)) if self.canvas is None else (int(self.canvas[0]),
+                                                                                int(self.canvas[1]))
 
     def add_obj(self, obj: Object) -> int:
         """Scene::add_obj (scene/mod.rs:788-801); returns the object id."""

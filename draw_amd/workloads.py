@@ -298,6 +298,29 @@ def shells_scene(seed: int = 3, n_tri: int = 20_000_000, width: int = 3840, heig
     return Scene(width, height, cam, (0.0, 2.0 * cam_d, 2.0 * cam_d), objs)
 
 
+def app_scene(seed: int = 6, screen=(1920, 1080), canvas=(800, 600), n_tri: int = 60_000,
+              cross_frac: float = 0.1, tex_size: int = 512, median_px: float = 40.0, sigma: float = 1.5) -> Scene:
+    """The reference application's own configuration (app/mod.rs:57-84,
+    scene/mod.rs:760-786, 308-310): scene at screen size, camera at
+    (0, 0, 150) looking at the origin with near -10, far -510 and fov_x 135,
+    light (0, 300, 300), drawn into an 800x600 canvas centred on the screen
+    (offset ((screen - canvas) / 2)).  Objects: a torus rescaled to a largest
+    vertex norm of 100 (as Object::load_from_file rescales, object.rs:159-170)
+    and a seeded soup filling the screen's view frustum, so many triangles
+    cross the canvas border or lie wholly outside it."""
+    rng = _rng(seed)
+    tex = _phong(noise_texture(tex_size, rng))
+    torus = torus_object(1.0, 0.4, 48, 24, _phong(checker_texture(256, 8, rng)), _rotation(rng))
+    v = torus.vertices.astype(np.float64)
+    torus.vertices = (v * (100.0 / np.linalg.norm(v, axis=1).max())).astype(F32)
+    soup = soup_object(n_tri, screen[0], screen[1], 135.0, 10.0, 510.0, 150.0, tex, rng, cross_frac=cross_frac,
+                       median_px=median_px, sigma=sigma)
+    cam = Camera(position=(0.0, 0.0, 150.0))
+    ox, oy = (screen[0] - canvas[0]) // 2, (screen[1] - canvas[1]) // 2
+    return Scene(screen[0], screen[1], cam, (0.0, 300.0, 300.0), [torus, soup], canvas=tuple(canvas),
+                 canvas_offset=(ox, oy))
+
+
 def small_random_scene(seed: int, n_tri: int = 300, width: int = 96, height: int = 64,
                        cross_frac: float = 0.2, tex_size: int = 16, power: int = 2) -> Scene:
     """Tiny soup for exhaustive parity checks."""

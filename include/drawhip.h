@@ -109,6 +109,19 @@ typedef struct {
  * (app/mod.rs:80), clear colour azul_bb and offset 0. */
 int dh_frame_uniforms(const dh_camera* cam, int32_t W, int32_t H, const float light[3],
                       dh_frame_params* out);
+/* The reference application's configuration: Scene::new(screen_w, screen_h)
+ * (scene/mod.rs:760-786) sets the camera's aspect ratio and the viewport
+ * matrix from the screen size, and the frame is drawn into a separate
+ * Canvas::new(canvas_w, canvas_h) (canvas.rs:366) moved by
+ * Canvas::apply_offset(off_x, off_y) (canvas.rs:382-385); the app uses an
+ * 800x600 canvas at ((screen_w-800)/2, (screen_h-600)/2) (app/mod.rs:59-63,
+ * 81-84).  W/H of the result are the canvas size, and every snapped vertex is
+ * screen_coord - offset (canvas.rs:585-587); the raster's screen rectangle is
+ * the canvas (canvas.rs:647).  dh_frame_uniforms is this with canvas = screen
+ * and offset 0. */
+int dh_frame_uniforms_canvas(const dh_camera* cam, int32_t screen_w, int32_t screen_h, int32_t canvas_w,
+                             int32_t canvas_h, int32_t off_x, int32_t off_y, const float light[3],
+                             dh_frame_params* out);
 
 /* ---- context --------------------------------------------------------- */
 int dh_ctx_create(int32_t device, dh_ctx* out);

@@ -155,9 +155,13 @@ static int view_plane(v3 a, v3 b, v3 c, v3 visible, float out[4]) {
     return OR_OK;
 }
 
-int or_frame_uniforms(const or_camera_desc* cd, int32_t W, int32_t H, const float light[3],
-                      or_frame* out) {
-    if (!cd || !out || W <= 0 || H <= 0) return OR_E_ARG;
+/* Scene::new(W, H) (scene/mod.rs:760-786) drawn into Canvas::new(cw, ch)
+ * (canvas.rs:366) with Canvas::apply_offset(ox, oy) (canvas.rs:382-385): the
+ * aspect ratio and viewport use the scene (screen) size, the raster the
+ * canvas size and offset (canvas.rs:585-587, 647). */
+static int frame_uniforms(const or_camera_desc* cd, int32_t W, int32_t H, int32_t cw, int32_t ch,
+                          int32_t ox, int32_t oy, const float light[3], or_frame* out) {
+    if (!cd || !out || W <= 0 || H <= 0 || cw <= 0 || ch <= 0) return OR_E_ARG;
     memset(out, 0, sizeof(*out));
     camera c;
     memset(&c, 0, sizeof(c));
@@ -217,12 +221,23 @@ int or_frame_uniforms(const or_camera_desc* cd, int32_t W, int32_t H, const floa
 
     out->cam[0] = cp.x; out->cam[1] = cp.y; out->cam[2] = cp.z;
     if (light) { out->light[0] = light[0]; out->light[1] = light[1]; out->light[2] = light[2]; }
-    out->W = W; out->H = H;
-    out->off_x = 0.0f; out->off_y = 0.0f;
+    out->W = cw; out->H = ch;
+    out->off_x = (float)ox; out->off_y = (float)oy;   /* apply_offset: x as f32 */
     out->clear_depth = 100000.0f;                /* app/mod.rs:80            */
     out->clear_bgra[0] = 255; out->clear_bgra[1] = 186; /* Pixel::azul_bb canvas.rs:131-133 */
     out->clear_bgra[2] = 155; out->clear_bgra[3] = 255;
     return OR_OK;
+}
+
+int or_frame_uniforms(const or_camera_desc* cd, int32_t W, int32_t H, const float light[3],
+                      or_frame* out) {
+    return frame_uniforms(cd, W, H, W, H, 0, 0, light, out);
+}
+
+int or_frame_uniforms_canvas(const or_camera_desc* cd, int32_t screen_w, int32_t screen_h, int32_t canvas_w,
+                             int32_t canvas_h, int32_t off_x, int32_t off_y, const float light[3],
+                             or_frame* out) {
+    return frame_uniforms(cd, screen_w, screen_h, canvas_w, canvas_h, off_x, off_y, light, out);
 }
 
 /* ---------------------------------------------------------- per-vertex (a2) */

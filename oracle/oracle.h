@@ -86,6 +86,12 @@ typedef struct {
 /* a1: Camera::new + gen_transformation_matrix + gen_view_planes. */
 int or_frame_uniforms(const or_camera_desc* cam, int32_t W, int32_t H,
                       const float light[3], or_frame* out);
+/* a1 in the reference app's configuration: scene (screen) size W x H for the
+ * camera and viewport, a canvas_w x canvas_h canvas at offset (off_x, off_y)
+ * (app/mod.rs:59-63, 81-84). */
+int or_frame_uniforms_canvas(const or_camera_desc* cam, int32_t screen_w, int32_t screen_h, int32_t canvas_w,
+                             int32_t canvas_h, int32_t off_x, int32_t off_y, const float light[3],
+                             or_frame* out);
 
 /* a2: per-vertex visual info (scene/mod.rs:920-925): out = light[3], eye[3],
  * halfway[3], depth. */

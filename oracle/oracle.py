@@ -69,6 +69,8 @@ def lib():
         L = C.CDLL(path)
         L.or_frame_uniforms.argtypes = [C.POINTER(OrCameraDesc), C.c_int32, C.c_int32, C.POINTER(C.c_float),
                                         C.POINTER(OrFrame)]
+        L.or_frame_uniforms_canvas.argtypes = [C.POINTER(OrCameraDesc)] + [C.c_int32] * 6 + [
+            C.POINTER(C.c_float), C.POINTER(OrFrame)]
         L.or_render.argtypes = [C.POINTER(OrFrame), C.POINTER(OrObject), C.c_uint32, C.POINTER(OrMaterial),
                                 C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(OrStats)]
         L.or_render_range.argtypes = [C.POINTER(OrFrame), C.POINTER(OrObject), C.c_uint32, C.POINTER(OrMaterial),
@@ -100,8 +102,13 @@ def frame_uniforms(scene) -> OrFrame:
     cam = scene.camera
     cd = OrCameraDesc(_f3(cam.position), _f3(cam.direction), float(cam.fov_x), float(cam.near), float(cam.far))
     fr = OrFrame()
-    rc = lib().or_frame_uniforms(C.byref(cd), int(scene.width), int(scene.height), _f3(scene.light_source),
-                                 C.byref(fr))
+    if getattr(scene, "canvas", None) is None:
+        rc = lib().or_frame_uniforms(C.byref(cd), int(scene.width), int(scene.height), _f3(scene.light_source),
+                                     C.byref(fr))
+    else:
+        (cw, ch), (ox, oy) = scene.canvas, scene.canvas_offset
+        rc = lib().or_frame_uniforms_canvas(C.byref(cd), int(scene.width), int(scene.height), int(cw), int(ch),
+                                            int(ox), int(oy), _f3(scene.light_source), C.byref(fr))
     if rc != 0:
         raise RuntimeError(f"or_frame_uniforms failed: {ERRORS.get(rc, rc)}")
     return fr

@@ -121,3 +121,23 @@ def test_renderer_fails_loudly_without_device():
         pytest.skip("a GPU is present")
     with pytest.raises(DrawHipError):
         Renderer(0)
+
+
+def test_frame_uniforms_canvas_bit_exact_vs_oracle():
+    """dh_frame_uniforms_canvas (screen size for the camera, a canvas of its
+    own size and offset) equals the oracle's restatement bit for bit."""
+    import copy
+    rng = np.random.default_rng(12)
+    for _ in range(30):
+        sw, sh = int(rng.integers(64, 4000)), int(rng.integers(64, 2200))
+        s = workloads.torus_scene(1, width=sw, height=sh)
+        s.camera = workloads.random_camera(rng, dist=(1.0, 400.0))
+        s.canvas = (int(rng.integers(1, sw + 1)), int(rng.integers(1, sh + 1)))
+        s.canvas_offset = (int(rng.integers(-100, sw)), int(rng.integers(-100, sh)))
+        assert bytes(frame_uniforms(s)) == bytes(O.frame_uniforms(s))
+    app = workloads.app_scene(6, n_tri=10)
+    fr = frame_uniforms(app)
+    assert (fr.W, fr.H, fr.off_x, fr.off_y) == (800, 600, 560.0, 240.0)
+    plain = copy.copy(app)
+    plain.canvas = None
+    assert bytes(frame_uniforms(plain))[:184] == bytes(fr)[:184]

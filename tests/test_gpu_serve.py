@@ -84,6 +84,53 @@ def test_poisson_stream_completes(renderer):
         srv.close()
 
 
+def _digest(bufs) -> bytes:
+    import hashlib
+    h = hashlib.blake2b(digest_size=16)
+    for b in bufs:
+        h.update(np.ascontiguousarray(b).view(np.uint8).tobytes())
+    return h.digest()
+
+
+def test_serving_configured_pool_mix_equals_alone_and_oracle(renderer):
+    """The serving workload at its configured size: the 64-mesh resident pool
+    (1k-2M triangles log-uniform, 14.3M in total, 1024x1024 textures), random
+    cameras, resolutions 854x480 / 1280x720 / 1920x1080 / 3840x2160 with
+    p = .4/.3/.25/.05 and the 200 GB budget.  192 requests (every 4K one among
+    the first 1024 included) go in as one burst over two workers; each
+    result equals the same job rendered alone by dh_render byte for byte, and
+    every 12th job and every 4K job equal the oracle on every pixel."""
+    pool, scenes = workloads.serving_requests(seed=5)
+    assert len(pool) == 64 and sum(o.triangle_count for o in pool) > 10_000_000
+    pick = list(range(160)) + [k for k in range(160, 1024) if scenes[k].width == 3840][:32]
+    jobs = [scenes[k] for k in pick]
+    reqs = [(frame_uniforms(sc), renderer.mesh_handles(sc)) for sc in jobs]
+    n4k = sum(sc.width == 3840 for sc in jobs)
+    assert n4k >= 8 and max(sc.triangle_count for sc in jobs) > 1_000_000
+    srv = Server(renderer, budget_bytes=200 << 30, workers=2, max_views=64)
+    try:
+        tickets = [srv.submit(fr, hs) for fr, hs in reqs]
+        mixed, batch_views = [], []
+        for t, sc in zip(tickets, jobs):
+            res = srv.poll(t, wait=True)
+            assert res.status == 0
+            batch_views.append(res.batch_views)
+            mixed.append(_digest(srv.read(res, sc.width, sc.height)))
+            srv.release(t)
+        st = srv.stats()
+        assert st["completed"] == len(jobs) and st["failed"] == 0
+        assert st["launches"] < len(jobs) and max(batch_views) > 1, "jobs were not batched"
+    finally:
+        srv.close()
+    for k, ((fr, hs), sc) in enumerate(zip(reqs, jobs)):
+        cv = renderer.render_device(sc, fr, hs)
+        f, d, w = renderer.read(cv, sc.width, sc.height)
+        assert _digest((f, d, w)) == mixed[k], f"job {k} ({sc.width}x{sc.height}): mix differs from alone"
+        if k % 12 == 0 or sc.width == 3840:
+            assert_same(f, d, w, oracle(sc), label=f"served job {k} ({sc.width}x{sc.height}, "
+                                                   f"{sc.triangle_count} tris)")
+
+
 def _stacked_quads(n=40, width=1280, height=720):
     """n full-screen quads at slightly different depths: every triangle lands
     in every tile, far beyond the 4 bin entries per triangle a launch is

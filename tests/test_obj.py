@@ -182,6 +182,20 @@ def test_loader_errors(tmp_path):
         load_obj(str(tmp_path / "d.obj"))
 
 
+def test_png_header_larger_than_its_data_is_an_error(tmp_path):
+    """An IHDR claiming 65535 x 65535 16-bit RGBA (34 GB decoded) over a few
+    bytes of IDAT is refused before anything is allocated for it."""
+    def chunk(t, d):
+        return struct.pack(">I", len(d)) + t + d + struct.pack(">I", zlib.crc32(t + d) & 0xFFFFFFFF)
+    png = (b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", 65535, 65535, 16, 6, 0, 0, 0)) +
+           chunk(b"IDAT", zlib.compress(bytes(64))) + chunk(b"IEND", b""))
+    (tmp_path / "big.png").write_bytes(png)
+    (tmp_path / "big.mtl").write_text("newmtl M\nKa 1 1 1\nKd 1 1 1\nKs 0 0 0\nmap_Kd big.png\n")
+    (tmp_path / "big.obj").write_text("mtllib big.mtl\nv 0 0 0\nv 1 0 0\nv 0 1 0\nusemtl M\nf 1 2 3\n")
+    with pytest.raises(DrawHipError, match="DH_E_ARG"):
+        load_obj(str(tmp_path / "big.obj"))
+
+
 @pytest.mark.skipif(not os.path.isdir(REF_MODELS), reason="the reference's model files are not on this machine")
 def test_reference_donut_model():
     """models/donut/donut.obj: 1024 vertices, 2048 triangles, no normals and

@@ -346,3 +346,180 @@ def test_transparent_blend_formula():
     want = [(int(np.trunc(F(F(bg) * F(F(1) - a)))) + int(np.trunc(F(F(255) * a)))) & 255 for bg in CLEAR[:3]]
     assert list(bgra[7 - 1, 1]) == want + [0]
     assert depth[1, 1] == F(100000.0)
+
+
+# ------------------------------------------ hand-computed pins of sub-terms
+def _dot(a, b):
+    """Vec3::dot, linalg.rs:182-184: (x0*y0 + x1*y1) + x2*y2 in binary32."""
+    a = [F(v) for v in a]
+    b = [F(v) for v in b]
+    return F(F(F(a[0] * b[0]) + F(a[1] * b[1])) + F(a[2] * b[2]))
+
+
+def _unit(v):
+    v = np.asarray(v, np.float64)
+    return tuple(F(x) for x in v / np.linalg.norm(v))
+
+
+@pytest.mark.parametrize("power", [2, 32])
+def test_specular_term_hand_computed(power):
+    """At vertex pixel A (alpha = 1, beta = gama = 0 exactly) the lerped N, L, H
+    are A's own, so the colour follows canvas.rs:713-741 step by step:
+    col = c_r*(c_a + c_l*(1 - max(0, L.N))) + c_l*(H.N)^power, with H.N far
+    from 0 so that the specular term moves every channel.  (H.N)^32 is
+    repeated squaring in binary32, the form integer powers take."""
+    t = 180
+    img = np.full((1, 1, 3), t, np.uint8)
+    tex = Texture(ka=(0.1, 0.15, 0.05), kd=(0.7, 0.5, 0.6), ks=(0.2, 0.3, 0.25), power=power, map_kd=TextureMap(img))
+    n = _unit((0.1, 0.2, 1.0))
+    l = _unit((0.3, -0.4, 0.8))
+    h = _unit((0.05, 0.1, 1.0))
+    other = dict(n=_unit((1, 0, 0)), l=_unit((0, 1, 0)), h=_unit((0, 0, -1)))
+    fr, bgra, depth, winner = _canvas(32, 32)
+    a = _attr(20, 20, 1.0, n=n, l=l, h=h)
+    b, c = _attr(2, 20, 1.0, **other), _attr(20, 3, 1.0, **other)
+    rc, _ = O.draw_triangle(fr, a, b, c, tex, bgra, depth, winner)
+    assert rc == 0 and winner[20, 20] == 0
+    diff = F(F(t) / F(255))
+    dln = _dot(l, n)
+    s = F(F(1) - max(dln, F(0)))
+    hn = _dot(h, n)
+    if power == 2:
+        h2 = F(hn * hn)
+    else:
+        h2 = hn
+        for _ in range(5):
+            h2 = F(h2 * h2)
+    assert F(0.5) < hn and F(0) < dln < F(1)
+    want, no_spec = [], []
+    for i in range(3):
+        c_r = F(diff * F(tex.kd[i]))
+        t1 = F(c_r * F(F(tex.ka[i]) + F(F(tex.ks[i]) * s)))
+        want.append(int(np.trunc(F(F(t1 + F(F(tex.ks[i]) * h2)) * F(255)))))
+        no_spec.append(int(np.trunc(F(t1 * F(255)))))
+    got = list(bgra[31 - 20, 20])
+    assert got == [want[2], want[1], want[0], 255]
+    assert all(w != x for w, x in zip(want, no_spec)), "the specular term must change every channel"
+
+
+def _near_plane():
+    fr = _clip_setup()
+    return fr, tuple(F(fr.planes[0][k]) for k in range(3)), F(fr.planes[0][3])
+
+
+def _clip_case():
+    """A triangle with vertex O behind the near plane (z = 2.9) and P, Q in
+    front; attributes deliberately unlike any recomputed value (depths 1, 2,
+    3; light and halfway unit vectors far apart; uv distinct)."""
+    O_ = (F(0.0137), F(-0.0211), F(2.9531))
+    P = (F(-0.0419), F(0.0087), F(2.1173))
+    Q = (F(0.0311), F(0.0243), F(1.7351))
+    attrs = [O.make_attr(0, 0, d, _unit(nv), _unit(lv), _unit(hv), uv) for d, nv, lv, hv, uv in [
+        (1.0, (0, 0, 1), (1, 0, 0), (0, 1, 0), (0.125, 0.25)),
+        (2.0, (0, 1, 0), (0, 0, 1), (1, 0, 0), (0.75, 0.5)),
+        (3.0, (1, 0, 0), (0, 1, 0), (0, 0, 1), (0.375, 0.875))]]
+    return O_, P, Q, attrs
+
+
+def _lerp_edge(p, o, fp, nrm, eps):
+    """scene/mod.rs:715-717: t = f(p) / n.(p - o) - EPS; p + (o - p)*t."""
+    d = tuple(F(p[i] - o[i]) for i in range(3))
+    t = F(F(fp / _dot(nrm, d)) - eps)
+    return t, tuple(F(p[i] + F(F(o[i] - p[i]) * t)) for i in range(3))
+
+
+def test_clip_eps_nudge_hand_computed():
+    """Near-plane split of a triangle whose first vertex is behind the plane:
+    the odd vertex becomes c (rotation, scene/mod.rs:691-711), and the new
+    vertices are P + (O - P)*t_P and Q + (O - Q)*t_Q with t = f/n.(v - O) - EPS
+    (EPS = 1e-7, linalg.rs:6).  The case is chosen so that t and t - EPS give
+    different binary32 vertices: dropping EPS fails."""
+    fr, nrm, k = _near_plane()
+    O_, P, Q, attrs = _clip_case()
+    f = {name: F(_dot(nrm, v) + k) for name, v in (("O", O_), ("P", P), ("Q", Q))}
+    assert f["O"] <= 0 < f["P"] and f["Q"] > 0
+    eps = F(1e-7)
+    tP, Ap = _lerp_edge(P, O_, f["P"], nrm, eps)
+    tQ, Bp = _lerp_edge(Q, O_, f["Q"], nrm, eps)
+    _, Ap0 = _lerp_edge(P, O_, f["P"], nrm, F(0))
+    _, Bp0 = _lerp_edge(Q, O_, f["Q"], nrm, F(0))
+    assert Ap != Ap0 and Bp != Bp0, "case must distinguish t from t - EPS"
+    n, pos, _ = O.clip(fr, [*O_, *P, *Q], attrs)
+    assert n == 2
+    want = np.array([[P, Ap, Bp], [P, Q, Bp]], F)
+    assert np.array_equal(pos.view(np.uint32), want.view(np.uint32))
+
+
+def test_clipped_attributes_are_lerped_not_recomputed():
+    """The new vertices' attributes are a_attr + (c_attr - a_attr)*t
+    component-wise (canvas.rs:242-291, scene/mod.rs:716-720): depth, light and
+    halfway are lerped, never recomputed from the new position nor
+    renormalised."""
+    fr, nrm, k = _near_plane()
+    O_, P, Q, attrs = _clip_case()
+    fP = F(_dot(nrm, P) + k)
+    tP, Ap = _lerp_edge(P, O_, fP, nrm, F(1e-7))
+    aO, aP = attrs[0], attrs[1]
+    n, _, out = O.clip(fr, [*O_, *P, *Q], attrs)
+    new_a = out[1]                      # second vertex of [P, A', B']
+
+    def lerp(x, y):
+        return F(F(x) + F(F(F(y) - F(x)) * tP))
+    assert new_a.depth == lerp(aP.depth, aO.depth)
+    for field in ("normal", "light", "halfway"):
+        got = [F(v) for v in getattr(new_a, field)]
+        want = [lerp(getattr(aP, field)[i], getattr(aO, field)[i]) for i in range(3)]
+        assert got == want, field
+    for i in range(2):
+        assert F(new_a.uv[i]) == lerp(aP.uv[i], aO.uv[i])
+    light = np.array([F(v) for v in new_a.light], np.float64)
+    assert abs(np.linalg.norm(light) - 1.0) > 1e-3, "lerped light is not renormalised"
+    cam = np.array(fr.cam[:], np.float64)
+    assert abs(new_a.depth - np.linalg.norm(np.array(Ap, np.float64) - cam)) > 0.5, "depth is not recomputed"
+
+
+# -------------------------------------- canvas offset (the reference app)
+def test_frame_uniforms_canvas_closed_form():
+    """Scene::new(screen) + Canvas::new(canvas) + apply_offset: M and the
+    planes are the screen-size uniforms bit for bit; W, H are the canvas size
+    and the offset is the integer offset as f32.  Canvas = screen with offset
+    0 is or_frame_uniforms itself."""
+    import copy
+    rng = np.random.default_rng(21)
+    for _ in range(20):
+        sw, sh = int(rng.integers(640, 4000)), int(rng.integers(480, 2200))
+        cw, ch = int(rng.integers(64, sw + 1)), int(rng.integers(64, sh + 1))
+        ox, oy = int(rng.integers(-50, sw)), int(rng.integers(-50, sh))
+        s = workloads.torus_scene(1, width=sw, height=sh)
+        s.camera = workloads.random_camera(rng)
+        full = O.frame_uniforms(s)
+        s2 = copy.copy(s)
+        s2.canvas, s2.canvas_offset = (cw, ch), (ox, oy)
+        fr = O.frame_uniforms(s2)
+        assert bytes(fr)[:64 + 24 + 96] == bytes(full)[:64 + 24 + 96]
+        assert (fr.W, fr.H, fr.off_x, fr.off_y) == (cw, ch, float(ox), float(oy))
+        assert fr.clear_depth == full.clear_depth and bytes(fr.clear_bgra) == bytes(full.clear_bgra)
+        s2.canvas, s2.canvas_offset = (sw, sh), (0, 0)
+        assert bytes(O.frame_uniforms(s2)) == bytes(full)
+
+
+def test_canvas_offset_moves_the_triangle_and_its_tie_point():
+    """pos_map_center(screen_coord - offset) (canvas.rs:585-587): a quad drawn
+    at screen (p + offset + 0.25) into a canvas with that offset covers the
+    canvas pixels the quad at (p + 0.25) covers with offset 0 -- including the
+    tie rule, evaluated at canvas (-1, -1): right and top edges in, left and
+    bottom out.  A flipped sign would move the quad off the canvas."""
+    rng = np.random.default_rng(8)
+    for _ in range(20):
+        x0, y0 = (int(v) for v in rng.integers(-5, 20, 2))
+        x1, y1 = x0 + int(rng.integers(1, 30)), y0 + int(rng.integers(1, 30))
+        ox, oy = int(rng.integers(100, 3000)), int(rng.integers(100, 2000))
+        fr, bgra, depth, winner = _canvas(64, 64)
+        fr.off_x, fr.off_y = float(ox), float(oy)
+        q = [(x0, y0), (x1, y0), (x1, y1), (x0, y1)]
+        p = [_attr(x + ox + 0.25, y + oy + 0.25) for x, y in q]
+        O.draw_triangle(fr, p[0], p[1], p[2], Texture(), bgra, depth, winner, tid=1)
+        O.draw_triangle(fr, p[0], p[2], p[3], Texture(), bgra, depth, winner, tid=2)
+        xs, ys = np.meshgrid(np.arange(64), np.arange(64))
+        want = (xs > x0) & (xs <= x1) & (ys > y0) & (ys <= y1)
+        assert np.array_equal(winner != 0xFFFFFFFF, want)
