@@ -139,6 +139,8 @@ SIGNATURES = {
     "dh_canvas_read_rgba": [_P, _H, _P],
     "dh_canvas_export_png": [_P, _H, C.c_char_p],
     "dh_write_png": [C.c_char_p, _P, C.c_int32, C.c_int32],
+    "dh_canvas_export_jpg": [_P, _H, C.c_char_p, C.c_int32],
+    "dh_write_jpg": [C.c_char_p, _P, C.c_int32, C.c_int32, C.c_int32],
     "dh_get_stats": [_P, C.POINTER(DhStats)],
     "dh_draw_gui": [_P, _H, _P, C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, _H],
     "dh_obj_load": [C.c_char_p, C.POINTER(_P), C.c_char_p, C.c_uint32],

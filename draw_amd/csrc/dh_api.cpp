@@ -1152,6 +1152,18 @@ extern "C" int dh_canvas_export_png(dh_ctx c, dh_handle canvas, const char* path
   return rc ? fail(c, rc, std::string("cannot write ") + path) : DH_OK;
 }
 
+extern "C" int dh_canvas_export_jpg(dh_ctx c, dh_handle canvas, const char* path, int32_t quality) {
+  if (!c || !path) return DH_E_ARG;
+  CanvasRes cv;
+  int rc = find_canvas(c, canvas, &cv);
+  if (rc) return rc;
+  std::vector<uint8_t> rgba((size_t)cv.W * (size_t)cv.H * 4);
+  rc = dh_canvas_read_rgba(c, canvas, rgba.data());
+  if (rc) return rc;
+  rc = dh_write_jpg(path, rgba.data(), cv.W, cv.H, quality);
+  return rc ? fail(c, rc, std::string("cannot write ") + path) : DH_OK;
+}
+
 extern "C" int dh_draw_gui(dh_ctx c, dh_handle canvas, const dh_gui_vertex* vtx, uint32_t n_vtx, const uint32_t* idx,
                            uint32_t n_idx, const dh_gui_cmd* cmds, uint32_t n_cmds, dh_handle font) {
   if (!c || (n_vtx && !vtx) || (n_idx && !idx) || (n_cmds && !cmds)) return DH_E_ARG;

@@ -295,6 +295,12 @@ class Renderer:
     def export_png(self, canvas_handle: int, path: str):
         self._check(self.L.dh_canvas_export_png(self.ctx, canvas_handle, str(path).encode()), "dh_canvas_export_png")
 
+    def export_jpg(self, canvas_handle: int, path: str, quality: int = 100):
+        """dh_canvas_export_jpg: the RGBA frame as a baseline JPEG (quality 100
+        is what the reference's export call amounts to)."""
+        self._check(self.L.dh_canvas_export_jpg(self.ctx, canvas_handle, str(path).encode(), int(quality)),
+                    "dh_canvas_export_jpg")
+
     def render(self, scene: Scene, canvas: Canvas | None = None, frame: DhFrameParams | None = None):
         """Scene::render(&mut canvas): returns (canvas, winner ids)."""
         fr = frame if frame is not None else frame_uniforms(scene)

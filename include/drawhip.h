@@ -254,11 +254,22 @@ int dh_canvas_read(dh_ctx ctx, dh_handle canvas, uint8_t* bgra, float* depth, ui
 int dh_canvas_read_rgba(dh_ctx ctx, dh_handle canvas, uint8_t* rgba);
 /* That RGBA frame written as an 8-bit RGBA PNG file (the reference writes it
  * with stb_image_write; decoded pixels are the same, the compressed bytes are
- * not).  JPEG export is not provided. */
+ * not). */
 int dh_canvas_export_png(dh_ctx ctx, dh_handle canvas, const char* path);
 /* Host-only PNG writer behind dh_canvas_export_png: W*H RGBA8 pixels, rows
  * top-down. */
 int dh_write_png(const char* path, const uint8_t* rgba, int32_t W, int32_t H);
+/* The same RGBA frame as a baseline JPEG file (app/mod.rs:316-379 with the
+ * JPEG format).  quality as stbi_write_jpg takes it: 0 means 90, anything
+ * else is clamped to 1..100; the reference passes width*4, i.e. 100.  A
+ * baseline sequential encoder (ITU-T T.81, JFIF): YCbCr without chroma
+ * subsampling, the Annex K quantisation tables scaled by quality (every
+ * step 1 at 100) and Annex K Huffman tables.  The compressed bytes are not
+ * stb's. */
+int dh_canvas_export_jpg(dh_ctx ctx, dh_handle canvas, const char* path, int32_t quality);
+/* Host-only JPEG writer behind dh_canvas_export_jpg: W*H RGBA8 pixels (alpha
+ * ignored), rows top-down, W and H at most 65535. */
+int dh_write_jpg(const char* path, const uint8_t* rgba, int32_t W, int32_t H, int32_t quality);
 /* Device pointers of a canvas (for zero-copy consumers on the same device). */
 int dh_canvas_device_ptrs(dh_ctx ctx, dh_handle canvas, void** bgra, void** depth, void** winner);
 int dh_get_stats(dh_ctx ctx, dh_stats* out);

@@ -58,3 +58,86 @@ def test_export_swaps_channels_of_the_rendered_frame(renderer, tmp_path):
     path = tmp_path / "torus.png"
     renderer.export_png(cv, path)
     assert np.array_equal(decode_png(path.read_bytes()), rgba)
+
+
+def _decode_jpeg(path):
+    """Decode with Pillow (an independent libjpeg decoder), as RGB."""
+    PIL = pytest.importorskip("PIL.Image")
+    with PIL.open(path) as im:
+        assert im.format == "JPEG" and im.mode == "RGB"
+        return np.asarray(im.convert("RGB"))
+
+
+def _markers(data: bytes):
+    """(marker, segment body) of every segment before the scan, T.81 B.1."""
+    assert data[:2] == b"\xff\xd8" and data[-2:] == b"\xff\xd9"
+    pos, out = 2, []
+    while True:
+        assert data[pos] == 0xFF
+        m = data[pos + 1]
+        (n,) = struct.unpack(">H", data[pos + 2:pos + 4])
+        out.append((m, data[pos + 4:pos + 2 + n]))
+        pos += 2 + n
+        if m == 0xDA:
+            return out
+
+
+def _psnr(a, b):
+    mse = np.mean((a.astype(np.float64) - b.astype(np.float64)) ** 2)
+    return 10 * np.log10(255.0 ** 2 / max(mse, 1e-12))
+
+
+@pytest.mark.parametrize("W,H", [(1, 1), (13, 7), (640, 480)])
+def test_jpeg_writer_quality_100_decodes_close(tmp_path, W, H):
+    """Quality 100 (the reference's export call: its quality argument is
+    width*4, clamped to 100): every quantisation step is 1, no chroma
+    subsampling; Pillow's decoder returns the frame within a few levels."""
+    s = workloads.torus_scene(1, width=W, height=H)
+    rgba = np.ascontiguousarray(oracle(s).frame[:, :, [2, 1, 0, 3]])
+    path = tmp_path / "f.jpg"
+    # the app's 800-pixel-wide canvas: quality argument 800 * 4
+    assert _lib.lib().dh_write_jpg(str(path).encode(), rgba.ctypes.data, W, H, 800 * 4) == 0
+    segs = _markers(path.read_bytes())
+    dqt = b"".join(b for m, b in segs if m == 0xDB)
+    assert len(dqt) == 130 and set(dqt[1:65]) == {1} and set(dqt[66:130]) == {1}
+    sof = [b for m, b in segs if m == 0xC0][0]
+    assert struct.unpack(">BHHB", sof[:6]) == (8, H, W, 3)
+    assert [sof[6 + 3 * k + 1] for k in range(3)] == [0x11, 0x11, 0x11]
+    got = _decode_jpeg(path)
+    assert got.shape == (H, W, 3)
+    err = np.abs(got.astype(int) - rgba[:, :, :3].astype(int))
+    assert err.max() <= 4 and err.mean() < 0.8
+
+
+def test_jpeg_writer_lower_quality_and_stb_quality_rule(tmp_path):
+    """Lower quality scales the Annex K tables (IJG rule) and still decodes
+    to the picture; 0 means 90 and values above 100 mean 100, as
+    stbi_write_jpg treats its quality argument."""
+    s = workloads.torus_scene(2)
+    rgba = np.ascontiguousarray(oracle(s).frame[:, :, [2, 1, 0, 3]])
+    L = _lib.lib()
+    sizes = {}
+    for q in (0, 50, 90, 100, 3200):
+        path = tmp_path / f"q{q}.jpg"
+        assert L.dh_write_jpg(str(path).encode(), rgba.ctypes.data, 640, 480, q) == 0
+        sizes[q] = path.stat().st_size
+        dqt = b"".join(b for m, b in _markers(path.read_bytes()) if m == 0xDB)
+        if q == 50:
+            assert dqt[1] == 16 and dqt[66] == 17        # Annex K tables unscaled at 50
+        assert _psnr(_decode_jpeg(path), rgba[:, :, :3]) > (30 if q == 50 else 38)
+    assert (tmp_path / "q0.jpg").read_bytes() == (tmp_path / "q90.jpg").read_bytes()
+    assert (tmp_path / "q3200.jpg").read_bytes() == (tmp_path / "q100.jpg").read_bytes()
+    assert sizes[50] < sizes[90] < sizes[100]
+    assert L.dh_write_jpg(str(tmp_path / "x.jpg").encode(), rgba.ctypes.data, 0, 1, 90) == -1
+
+
+@pytest.mark.gpu
+def test_export_jpg_of_the_rendered_frame(renderer, tmp_path):
+    s = workloads.torus_scene(1)
+    cv = renderer.render_device(s)
+    rgba = renderer.read_rgba(cv, s.width, s.height)
+    path = tmp_path / "torus.jpg"
+    renderer.export_jpg(cv, path)
+    host = tmp_path / "host.jpg"
+    assert _lib.lib().dh_write_jpg(str(host).encode(), rgba.ctypes.data, s.width, s.height, 100) == 0
+    assert path.read_bytes() == host.read_bytes()
