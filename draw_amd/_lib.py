@@ -137,6 +137,7 @@ SIGNATURES = {
     "dh_canvas_read": [_P, _H, _P, _P, _P],
     "dh_canvas_device_ptrs": [_P, _H, C.POINTER(_P), C.POINTER(_P), C.POINTER(_P)],
     "dh_canvas_read_rgba": [_P, _H, _P],
+    "dh_canvas_read_rgba_noalloc": [_P, _H, _P],
     "dh_canvas_export_png": [_P, _H, C.c_char_p],
     "dh_write_png": [C.c_char_p, _P, C.c_int32, C.c_int32],
     "dh_canvas_export_jpg": [_P, _H, C.c_char_p, C.c_int32],

@@ -335,6 +335,9 @@ int ctx_new(int32_t device, std::shared_ptr<ResPool> pool, dh_ctx* out) {
     e = hipHostMalloc(reinterpret_cast<void**>(&c->h_status), 2 * sizeof(dh::Status), hipHostMallocCoherent);
   if (e == hipSuccess) e = hipHostGetDevicePointer(reinterpret_cast<void**>(&c->d_status_host), c->h_status, 0);
   for (int i = 0; i < 5 && e == hipSuccess; i++) e = hipEventCreate(&c->ev[i]);
+  // set-up of the fixed export staging on this device (its address, once per
+  // device); a failure is reported by the entry point that needs it
+  if (e == hipSuccess) (void)dh::rgba_scratch_init(device);
   if (e == hipSuccess && !pool) {
     pool = std::make_shared<ResPool>();
     pool->device = device;
@@ -1136,6 +1139,27 @@ extern "C" int dh_canvas_read_rgba(dh_ctx c, dh_handle canvas, uint8_t* rgba) {
   HIPCHK(c, c->export_buf.ensure(n));
   HIPCHK(c, dh::enqueue_bgra_to_rgba(cv.bgra, c->export_buf.p, (uint32_t)n, c->stream));
   HIPCHK(c, hipMemcpyAsync(rgba, c->export_buf.p, n * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return DH_OK;
+}
+
+extern "C" int dh_canvas_read_rgba_noalloc(dh_ctx c, dh_handle canvas, uint8_t* rgba) {
+  if (!c || !rgba) return DH_E_ARG;
+  CanvasRes cv;
+  const int rc = find_canvas(c, canvas, &cv);
+  if (rc) return rc;
+  const uint64_t n = (uint64_t)cv.W * (uint64_t)cv.H;
+  if (n > dh::kRgbaScratchPixels)
+    return fail(c, DH_E_CAPACITY, "canvas larger than the fixed export staging (3840x2160 pixels)");
+  uint32_t* staging = nullptr;
+  std::mutex* mu = nullptr;
+  const hipError_t e = dh::rgba_scratch(c->device, &staging, &mu);
+  if (e != hipSuccess) return hip_fail(c, e, "hipGetSymbolAddress(export staging)");
+  // every call on this device shares the array: one at a time
+  std::lock_guard<std::mutex> lk(*mu);
+  HIPCHK(c, hipSetDevice(c->device));
+  HIPCHK(c, dh::enqueue_bgra_to_rgba(cv.bgra, staging, (uint32_t)n, c->stream));
+  HIPCHK(c, hipMemcpyAsync(rgba, staging, n * 4, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return DH_OK;
 }

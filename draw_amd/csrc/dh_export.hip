@@ -7,9 +7,11 @@
 #include <string.h>
 #include <zlib.h>
 
+#include <mutex>
 #include <vector>
 
 #include "dh_common.hpp"
+#include "dh_internal.hpp"
 
 namespace dh {
 
@@ -37,6 +39,59 @@ hipError_t enqueue_bgra_to_rgba(const uint32_t* src, uint32_t* dst, uint32_t n, 
   if (threads == 0) return hipSuccess;
   hipLaunchKernelGGL(k_bgra_to_rgba, dim3((threads + 255u) / 256u), dim3(256), 0, s, src, dst, n);
   return hipGetLastError();
+}
+
+}  // namespace dh
+
+// ---- fixed RGBA staging for dh_canvas_read_rgba_noalloc ----------------
+// The staging buffer of the export, as a file-scope device array: every GPU
+// that loads this library holds its own copy (31.6 MiB, one 3840x2160 frame)
+// until the process ends, so the entry point that uses it allocates nothing.
+static __device__ __align__(256) uint32_t scratch[dh::kRgbaScratchPixels];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+
+// Per device: the array's device address (from hipGetSymbolAddress with that
+// device current; the host-side name is not a device address), the status of
+// that lookup, and a lock that serialises the calls sharing the array.
+struct ScratchSlot {
+  std::once_flag once;
+  hipError_t status = hipErrorNotInitialized;
+  uint32_t* ptr = nullptr;
+  std::mutex mu;
+};
+ScratchSlot g_scratch[kMaxDevices];
+
+}  // namespace
+
+namespace dh {
+
+hipError_t rgba_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  ScratchSlot& s = g_scratch[device];
+  std::call_once(s.once, [&] {
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e == hipSuccess && cur != device) e = hipSetDevice(device);
+    void* p = nullptr;
+    if (e == hipSuccess) e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess && p == nullptr) e = hipErrorInvalidSymbol;
+    if (e == hipSuccess) s.ptr = static_cast<uint32_t*>(p);
+    s.status = e;
+    if (cur >= 0 && cur != device) (void)hipSetDevice(cur);
+  });
+  return s.status;
+}
+
+hipError_t rgba_scratch(int device, uint32_t** ptr, std::mutex** mu) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  const hipError_t e = rgba_scratch_init(device);
+  if (e != hipSuccess) return e;
+  *ptr = g_scratch[device].ptr;
+  *mu = &g_scratch[device].mu;
+  return hipSuccess;
 }
 
 }  // namespace dh

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "../../include/drawhip.h"
 
 namespace dh {
@@ -21,4 +23,12 @@ hipError_t enqueue_gui(const dh_gui_vertex* vtx, const uint4* tris, uint32_t n_t
                        const float* depth, uint32_t* error, hipStream_t s);
 constexpr size_t kGuiRecBytes = 144;  // sizeof(GuiRec)
 hipError_t enqueue_bgra_to_rgba(const uint32_t* src, uint32_t* dst, uint32_t n, hipStream_t s);
+// The fixed RGBA staging array of dh_canvas_read_rgba_noalloc: at most
+// kRgbaScratchPixels pixels (one 3840x2160 frame, 31.6 MiB per device).
+constexpr uint32_t kRgbaScratchPixels = 3840u * 2160u;
+// Looks up the array's device address on `device` once (set-up: called by
+// dh_ctx_create); later calls return the stored status.
+hipError_t rgba_scratch_init(int device);
+// The array's device address on `device` and the lock its users share.
+hipError_t rgba_scratch(int device, uint32_t** ptr, std::mutex** mu);
 }  // namespace dh

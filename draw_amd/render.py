@@ -292,6 +292,14 @@ class Renderer:
         self._check(self.L.dh_canvas_read_rgba(self.ctx, canvas_handle, rgba.ctypes.data), "dh_canvas_read_rgba")
         return rgba
 
+    def read_rgba_noalloc(self, canvas_handle: int, W: int, H: int) -> np.ndarray:
+        """read_rgba through dh_canvas_read_rgba_noalloc (fixed device staging,
+        no allocation; canvases up to 3840x2160)."""
+        rgba = np.empty((H, W, 4), np.uint8)
+        self._check(self.L.dh_canvas_read_rgba_noalloc(self.ctx, canvas_handle, rgba.ctypes.data),
+                    "dh_canvas_read_rgba_noalloc")
+        return rgba
+
     def export_png(self, canvas_handle: int, path: str):
         self._check(self.L.dh_canvas_export_png(self.ctx, canvas_handle, str(path).encode()), "dh_canvas_export_png")
 

@@ -252,6 +252,14 @@ int dh_canvas_read(dh_ctx ctx, dh_handle canvas, uint8_t* bgra, float* depth, ui
  * with bytes 0 and 2 of every pixel swapped (BGRA -> RGBA, app/mod.rs:347-351)
  * into rgba, W*H*4 bytes, rows top-down; the swap runs on the device. */
 int dh_canvas_read_rgba(dh_ctx ctx, dh_handle canvas, uint8_t* rgba);
+/* dh_canvas_read_rgba without allocating: no counterpart in the reference.
+ * For callers that cannot allocate device memory (tight serving loops, code
+ * that must not call hipMalloc).  The BGRA -> RGBA staging is a fixed array
+ * of 3840*2160 pixels (31.6 MiB) that every GPU loading the library holds
+ * until the process ends; a larger canvas returns DH_E_CAPACITY.  Calls on
+ * one GPU from one process share that array and take turns on it (a lock);
+ * each process has its own copy.  Same output as dh_canvas_read_rgba. */
+int dh_canvas_read_rgba_noalloc(dh_ctx ctx, dh_handle canvas, uint8_t* rgba);
 /* That RGBA frame written as an 8-bit RGBA PNG file (the reference writes it
  * with stb_image_write; decoded pixels are the same, the compressed bytes are
  * not). */
