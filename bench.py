@@ -74,22 +74,56 @@ def raster_shade_bytes(P: int, entries: int, p_cov: int, t_vis: int) -> dict:
             "record": 12 * P + 100 * entries + 224 * t_vis + 4 * p_cov}
 
 
-def measured_traffic(kernel: str):
-    """HBM bytes per launch of ``kernel`` on this workload from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE per dispatch, each
-    scaled by the same counter's reading for a device copy of known size), or
-    None when that summary has no line for the kernel."""
-    path = os.path.join(ROOT, "profiles", "hbm_traffic_soup1m_1080p.md")
+TRAFFIC_PROFILE = "profiles/traffic_soup1m_1080p.md"
+COUNTERS_PROFILE = "profiles/counters_soup1m_1080p.md"
+
+
+def profile_rows(rel: str) -> dict:
+    """The kernel rows of a committed rocprofv3 summary table: {kernel: cells}."""
+    path = os.path.join(ROOT, rel)
+    rows = {}
     if not os.path.exists(path):
-        return None
+        return rows
     for line in open(path):
         cells = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cells) >= 4 and cells[0].strip("`").replace("dh::", "") == kernel:
-            try:
-                return int(float(cells[3]))
-            except ValueError:
-                return None
-    return None
+        if len(cells) >= 3 and cells[0].startswith("`"):
+            rows[cells[0].strip("`").replace("dh::", "")] = cells[1:]
+    return rows
+
+
+def measured_traffic(kernel: str):
+    """L2-to-memory bytes per launch of ``kernel`` on this workload, from the
+    committed rocprofv3 --pmc summary: read requests by size
+    (TCC_EA0_RDREQ_32B/64B/128B_sum x 32/64/128 B) plus write requests
+    (TCC_EA0_WRREQ_64B_sum x 64 B, the rest x 32 B), per dispatch.  These
+    counters read exactly 1 GiB each way on a 1 GiB device copy, so no scale
+    factor is applied.  Returns (bytes, source) or (None, None)."""
+    row = profile_rows(TRAFFIC_PROFILE).get(kernel)
+    if not row:
+        return None, None
+    try:
+        return int(float(row[-1])), TRAFFIC_PROFILE + " (TCC_EA0 read/write requests by size, per dispatch)"
+    except ValueError:
+        return None, None
+
+
+def issue_roofline(kernel: str, ms: float):
+    """VALU issue of ``kernel`` against the chip's: wave-level VALU
+    instructions per launch (SQ_INSTS_VALU from the committed counter summary)
+    over the launch time, against 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per
+    wave64 VALU instruction = 614.4 G instructions/s."""
+    row = profile_rows(COUNTERS_PROFILE).get(kernel)
+    if not row or not ms:
+        return None
+    try:
+        waves, valu = float(row[0]), float(row[1])
+    except ValueError:
+        return None
+    insts = waves * valu
+    achieved = insts / (ms * 1e-3) / 1e9
+    return {"unit": "G VALU wave-instructions/s", "valu_per_wave": valu, "waves": int(waves),
+            "achieved": round(achieved, 1), "peak": 614.4, "frac": round(achieved / 614.4, 4),
+            "source": COUNTERS_PROFILE + " (SQ_WAVES, SQ_INSTS_VALU per dispatch)"}
 
 
 def raster_kernel_name(kind: int, tile: int, waves: int) -> str:
@@ -103,15 +137,17 @@ def frame_rooflines(kind: int, kernel: str, P: int, entries: int, p_cov: int, t_
                     traffic: bool) -> dict:
     """The dominant kernel's roofline (the raster; for kind 0 it includes the
     shading), and for kind 1 the separate k_shade's."""
-    tr = (lambda k: measured_traffic(k)) if traffic else (lambda k: None)
+    tr = (lambda k: measured_traffic(k)) if traffic else (lambda k: (None, None))
     if kind == 0:
         return {"roofline": roofline(kernel, raster_shade_bytes(P, entries, p_cov, t_vis), st["ms_raster"],
-                                     tr(kernel))}
-    return {"roofline": roofline(kernel, raster_bytes(P, entries), st["ms_raster"], tr(kernel)),
-            "roofline_shade": roofline("k_shade", shade_bytes(P, p_cov, t_vis), st["ms_shade"], tr("k_shade"))}
+                                     *tr(kernel), issue=traffic)}
+    return {"roofline": roofline(kernel, raster_bytes(P, entries), st["ms_raster"], *tr(kernel), issue=traffic),
+            "roofline_shade": roofline("k_shade", shade_bytes(P, p_cov, t_vis), st["ms_shade"], *tr("k_shade"),
+                                       issue=traffic)}
 
 
-def separate_shade_roofline(r, render, restore: tuple, P: int, p_cov: int, t_vis: int, traffic=None) -> dict:
+def separate_shade_roofline(r, render, restore: tuple, P: int, p_cov: int, t_vis: int, traffic=(None, None),
+                            issue: bool = False) -> dict:
     """k_shade on its own: the same frame (or batch) through the kind-1
     pipeline, whose raster writes the winner buffer and whose separate shading
     pass then reads it; the median HIP-event time of three eager launches.
@@ -122,14 +158,32 @@ def separate_shade_roofline(r, render, restore: tuple, P: int, p_cov: int, t_vis
         render()
         ms.append(r.stats()["ms_shade"])
     r.set_raster(*restore)
-    return roofline("k_shade", shade_bytes(P, p_cov, t_vis), float(np.median(ms)), traffic)
+    return roofline("k_shade", shade_bytes(P, p_cov, t_vis), float(np.median(ms)), *traffic, issue=issue)
 
 
-def roofline(kernel: str, nbytes: dict, ms: float, traffic=None) -> dict:
+def roofline(kernel: str, nbytes: dict, ms: float, traffic=None, traffic_source=None, issue: bool = False) -> dict:
+    """The HBM roofline of one kernel: algorithmic bytes over its HIP-event
+    time against 8 TB/s.  ``traffic``: the bytes the kernel actually moves
+    between L2 and memory per launch (committed counter summary, named in
+    traffic_source), with the rate they imply.  ``issue``: add the VALU issue
+    roofline from the committed counters, and name the limiter: the resource
+    with the larger fraction of its peak."""
     gbs = nbytes["algorithmic"] / (ms * 1e-3) / 1e9 if ms > 0 else None
-    return {"kernel": kernel, "bound": "hbm", "achieved": round(gbs, 1) if gbs else None, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4) if gbs else None, "traffic": traffic,
-            "algorithmic_bytes": nbytes["algorithmic"], "record_bytes": nbytes["record"], "kernel_ms": round(ms, 4)}
+    out = {"kernel": kernel, "bound": "hbm", "achieved": round(gbs, 1) if gbs else None, "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4) if gbs else None, "traffic": traffic,
+           "algorithmic_bytes": nbytes["algorithmic"], "record_bytes": nbytes["record"], "kernel_ms": round(ms, 4)}
+    if traffic and ms > 0:
+        tg = traffic / (ms * 1e-3) / 1e9
+        out.update({"traffic_source": traffic_source, "traffic_gbs": round(tg, 1),
+                    "traffic_frac": round(tg / HBM_PEAK_GBS, 4)})
+    if issue:
+        iss = issue_roofline(kernel, ms)
+        if iss:
+            out["issue"] = iss
+            tf = out.get("traffic_frac") or 0.0
+            out["limiter"] = ("VALU issue" if iss["frac"] >= tf else "memory traffic") + \
+                f" ({max(iss['frac'], tf):.0%} of its peak)"
+    return out
 
 
 def cpu_model() -> str:
@@ -529,7 +583,7 @@ def main():
     if args.raster_kind == 0:
         shade_sep = separate_shade_roofline(r, lambda: r.render_device(scene, fr, meshes),
                                             (args.raster_kind, args.tile, args.waves), W * H, p_cov, t_vis,
-                                            measured_traffic("k_shade"))
+                                            measured_traffic("k_shade"), issue=True)
     r.set_graph(True)
     frame_ms = total / args.steps * 1e3
     value = p_cov * args.steps / total / 1e6
