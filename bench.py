@@ -406,7 +406,7 @@ def bench_shard(scene, raster: tuple, steps: int = 5) -> dict:
             "merge_ms_per_buffer": med(t_m), "shade_ms_one_shard": med(t_s), "visibility_bytes": W * H * 8}
 
 
-def bench_serving(seconds: float, rate_hz: float, n_distinct: int = 512) -> dict:
+def bench_serving(seconds: float, rate_hz: float, n_distinct: int = 512, overload_seconds: float = 10.0) -> dict:
     """Config 5: a device-resident pool of 64 seeded meshes (1k-2M triangles,
     log-uniform, 1024^2 textures) and requests (mesh, random camera,
     resolution 854x480 p=.4 / 1280x720 .3 / 1920x1080 .25 / 3840x2160 .05).
@@ -415,7 +415,10 @@ def bench_serving(seconds: float, rate_hz: float, n_distinct: int = 512) -> dict
       capacity  a burst of 2 x n_distinct jobs through the server at once;
       stream    open-loop Poisson arrivals at ``rate_hz`` for ``seconds``
                 (requests cycled), HBM budget 200 GB, p50/p99 latency from
-                submit to launch completion.
+                submit to launch completion;
+      stream_3x_plain_rate  the same for ``overload_seconds`` at three times
+                the request rate the plain path sustains, where arrivals pile
+                up behind running launches and are batched.
     Shaded pixels per request come from one untimed render of each."""
     from draw_amd import workloads
     from draw_amd.render import Renderer, frame_uniforms
@@ -456,11 +459,31 @@ def bench_serving(seconds: float, rate_hz: float, n_distinct: int = 512) -> dict
     log(f"serving: Poisson {rate_hz:g}/s for {seconds:g} s ...")
     out = poisson_load(srv, reqs, rate_hz=rate_hz, seconds=seconds, seed=6)
     st1 = srv.stats()
+    # the same stream at 3x the rate one dh_render per request can sustain:
+    # jobs that arrive while launches run join the next launch
+    plain_rps = len(reqs) / plain_s
+    over_rate = round(3.0 * plain_rps, -2)
+    over = None
+    if overload_seconds > 0:
+        log(f"serving: Poisson {over_rate:g}/s (3x the plain path) for {overload_seconds:g} s ...")
+        st2 = srv.stats()
+        over = poisson_load(srv, reqs, rate_hz=over_rate, seconds=overload_seconds, seed=7)
+        st3 = srv.stats()
     srv.close()
     r.close()
     # served jobs cycle through the distinct requests in submission order
     served_cov = sum(covered[(res.ticket - st0["submitted"] - 1) % len(reqs)] for res, _ in out["results"])
     span = out["span_s"]
+    overload = None
+    if over is not None:
+        o_cov = sum(covered[(res.ticket - st2["submitted"] - 1) % len(reqs)] for res, _ in over["results"])
+        overload = {"rate_hz": over_rate, "seconds": overload_seconds, "requests": over["requests"],
+                    "completed": over["completed"], "failed": over["failed"], "late_submits": over["late_submits"],
+                    "latency_ms": over["latency_ms"], "mean_batch_views": over["mean_batch_views"],
+                    "shaded_mpix_per_s": round(o_cov / over["span_s"] / 1e6, 1) if over["span_s"] > 0 else None,
+                    "over_plain": round(o_cov / over["span_s"] / 1e6 / (cov_total / plain_s / 1e6), 3)
+                    if over["span_s"] > 0 else None,
+                    "launches": st3["launches"] - st2["launches"]}
     return {
         "workload": "64-mesh resident pool (1k-2M triangles log-uniform, 1024^2 RGBA8 textures), random cameras, "
                     "resolutions 854x480/1280x720/1920x1080/3840x2160 with p=.4/.3/.25/.05",
@@ -481,6 +504,7 @@ def bench_serving(seconds: float, rate_hz: float, n_distinct: int = 512) -> dict
                    "shaded_mpix_per_s": round(served_cov / span / 1e6, 1) if span > 0 else None,
                    "launches": st1["launches"] - st0["launches"], "hbm_budget_bytes": 200 << 30,
                    "hbm_peak_bytes": st1["bytes_peak"]},
+        "stream_3x_plain_rate": overload,
     }
 
 
