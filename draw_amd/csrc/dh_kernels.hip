@@ -411,28 +411,33 @@ __device__ inline void make_setup(const FrameParamsPod& fr, const FV& A, const F
 }
 
 // Stores one N-uint4 record per lane at dst[idx] (record units) for the lanes
-// with ok.  When those lanes are a prefix of the wave with consecutive idx (a
+// with ok and not skip (skip: a record nothing will read, whose id is still
+// taken).  When the ok lanes are a prefix of the wave with consecutive idx (a
 // dense run of output ids), the records go through the wave's LDS area
-// stage[64 * N] and out in coalesced 16-byte lanes; else each lane stores its
-// own.  Every lane of the wave must call it.
+// stage[64 * N] and out in coalesced 16-byte lanes, leaving the skipped
+// records' memory untouched; else each lane stores its own.  Every lane of the
+// wave must call it.
 template <int N>
-__device__ inline void wave_store(uint4* stage, const uint4 (&v)[N], bool ok, uint32_t idx, uint4* dst) {
+__device__ inline void wave_store(uint4* stage, const uint4 (&v)[N], bool ok, uint32_t idx, uint4* dst,
+                                  bool skip = false) {
   const uint32_t lane = threadIdx.x & 63u;
   const uint64_t m = __ballot(ok);
-  if (!m) return;
+  const uint64_t sm = __ballot(ok && skip);
+  if (m == sm) return;
   const uint32_t cnt = (uint32_t)__popcll(m);
   const uint32_t first = (uint32_t)__builtin_amdgcn_readfirstlane((int)idx);
   const bool prefix = m == (cnt == 64u ? ~0ull : ((1ull << cnt) - 1ull));
   if (prefix && __ballot(ok && idx != first + lane) == 0ull) {
-    if (ok) {
+    if (ok && !skip) {
 #pragma unroll
       for (int q = 0; q < N; q++) stage[lane * N + q] = v[q];
     }
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     uint4* out = dst + (uint64_t)first * N;
-    for (uint32_t k = lane; k < cnt * N; k += 64u) out[k] = stage[k];
+    for (uint32_t k = lane; k < cnt * N; k += 64u)
+      if (!((sm >> (k / (uint32_t)N)) & 1ull)) out[k] = stage[k];
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-  } else if (ok) {
+  } else if (ok && !skip) {
 #pragma unroll
     for (int q = 0; q < N; q++) dst[(uint64_t)idx * N + q] = v[q];
   }
@@ -611,10 +616,11 @@ __device__ inline uint32_t wave_tiles(const Spans& g, int TS, int lane, int* s_l
   return o;
 }
 
-// Ordered bins, one clipped triangle: its wide flag and, when it is not wide,
-// its tile count and its tiles (bit 4 * (ty - cy0) + (tx - cx0) of tmask[i],
-// within its at most 4 x 4 tile box; the wide ones are counted a wave at a
-// time later).
+// Ordered bins, one clipped triangle: its flag byte (0 binned here, 1 wide,
+// 2 dropped: no tiles, and its records were not stored) and, when it is not
+// wide, its tile count and its tiles (bit 4 * (ty - cy0) + (tx - cx0) of
+// tmask[i], within its at most 4 x 4 tile box; the wide ones are counted a
+// wave at a time later).
 __device__ inline void bin_count_one(const SetupRec& r, int TS, uint32_t i, uint32_t* __restrict__ bincnt,
                                      uint8_t* __restrict__ wflag, uint16_t* __restrict__ tmask) {
   uint32_t c = 0, mask = 0;
@@ -626,7 +632,7 @@ __device__ inline void bin_count_one(const SetupRec& r, int TS, uint32_t i, uint
     if (!is_wide)
       c = thread_tiles(g, TS, [&](uint32_t, int ty, int tx) { mask |= 1u << (4 * (ty - g.cy0) + (tx - g.cx0)); });
   }
-  wflag[i] = is_wide ? 1u : 0u;
+  wflag[i] = is_wide ? 1u : ((r.flags & kDrop) ? 2u : 0u);
   if (!is_wide) {
     bincnt[i] = c;
     tmask[i] = (uint16_t)mask;
@@ -765,10 +771,13 @@ __device__ inline void geom_emit_clipped(uint32_t j, const DevDraw* __restrict__
     SetupRec rec;
     uint4 ro[6], so[9];
     make_setup(fr, A, B, C, material, mats[material], view, group, ro, so, rec);
+    // with ordered bins nothing reads a dropped triangle's records
+    if (!BIN || !(rec.flags & kDrop)) {
 #pragma unroll
-    for (int q = 0; q < 6; q++) reinterpret_cast<uint4*>(setup + idx)[q] = ro[q];
+      for (int q = 0; q < 6; q++) reinterpret_cast<uint4*>(setup + idx)[q] = ro[q];
 #pragma unroll
-    for (int q = 0; q < 9; q++) reinterpret_cast<uint4*>(shade + idx)[q] = so[q];
+      for (int q = 0; q < 9; q++) reinterpret_cast<uint4*>(shade + idx)[q] = so[q];
+    }
     if (BIN) bin_count_one(rec, TS, idx, bincnt, wflag, tile_masks(wflag, cap_tri));
     if (dkey) dkey[idx] = depth_order_key(rec, views[view].depth_base);
   };
@@ -787,7 +796,7 @@ __device__ inline void geom_emit_simple(uint32_t j, bool active, uint4* stage, c
                                         uint8_t* __restrict__ wflag, uint32_t* __restrict__ dkey) {
   uint4 ro[6], so[9];
   uint32_t idx = 0;
-  bool ok = false;
+  bool ok = false, skip = false;
   if (active) {
     const uint2 sv = surv[j];
     const uint32_t t = sv.x;
@@ -824,12 +833,14 @@ __device__ inline void geom_emit_simple(uint32_t j, bool active, uint4* stage, c
       SetupRec rec;
       make_setup(fr, v[0], v[1], v[2], dr.material, mats[dr.material], dr.view, dr.group, ro, so, rec);
       ok = true;
+      // with ordered bins nothing reads a dropped triangle's records
+      skip = BIN && (rec.flags & kDrop);
       if (BIN) bin_count_one(rec, TS, idx, bincnt, wflag, tile_masks(wflag, cap_tri));
       if (dkey) dkey[idx] = depth_order_key(rec, views[dr.view].depth_base);
     }
   }
-  wave_store<6>(stage, ro, ok, idx, reinterpret_cast<uint4*>(setup));
-  wave_store<9>(stage, so, ok, idx, reinterpret_cast<uint4*>(shade));
+  wave_store<6>(stage, ro, ok, idx, reinterpret_cast<uint4*>(setup), skip);
+  wave_store<9>(stage, so, ok, idx, reinterpret_cast<uint4*>(shade), skip);
 }
 
 // A grid-stride loop over one list (its length is on the device), so that the
@@ -1047,7 +1058,7 @@ __global__ __launch_bounds__(256) void k_bin_count(const SetupRec* __restrict__ 
   }
 }
 
-// The wide-triangle list: the flagged triangles of each 4096-triangle chunk,
+// The wide-triangle list: the wide triangles (flag 1) of each 4096-triangle chunk,
 // 16 flags per thread, placed by a block scan; one atomic per chunk reserves
 // the chunk's run of the list.  The list order does not matter: a triangle's
 // bin entries go to its own scanned offset (or through per-tile atomics for
@@ -1068,10 +1079,10 @@ __global__ __launch_bounds__(256) void k_wide_list(const uint8_t* __restrict__ w
       const uint32_t w[4] = {f.x, f.y, f.z, f.w};
 #pragma unroll
       for (int k = 0; k < 16; k++)
-        if ((w[k >> 2] >> (8 * (k & 3))) & 0xFFu) mask |= 1u << k;
+        if (((w[k >> 2] >> (8 * (k & 3))) & 0xFFu) == 1u) mask |= 1u << k;
     } else {
       for (uint32_t k = 0; t0 + k < n; k++)
-        if (wflag[t0 + k]) mask |= 1u << k;
+        if (wflag[t0 + k] == 1u) mask |= 1u << k;
     }
     uint32_t total;
     const uint32_t local = block_exclusive_scan256((uint32_t)__popc(mask), s_tmp, &total);
@@ -1173,7 +1184,7 @@ __global__ __launch_bounds__(256) void k_bin_emit(const SetupRec* __restrict__ s
                                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   const uint32_t n = min(status->n_out, cap_tri);
   for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u)
-    if (!wflag[i])
+    if (wflag[i] == 0u)
       bin_emit_tri<ORDERED>(setup, views, TS, wflag, cap_tri, i, ORDERED ? binoff[i] : 0u, tile_start, tile_cur,
                             cap_pairs, keys, vals);
 }
@@ -1218,9 +1229,10 @@ __global__ __launch_bounds__(256) void k_bin_emit_near(const SetupRec* __restric
     const uint32_t o = block_exclusive_scan256(c, s_tmp, &tot) + running;
     running += tot;
     if (!act) continue;
-    if (wflag[i])
+    const uint8_t f = wflag[i];
+    if (f == 1u)
       binoff[i] = o;
-    else
+    else if (f == 0u)
       bin_emit_tri<true>(setup, views, TS, wflag, cap_tri, i, o, nullptr, nullptr, cap_pairs, keys, vals);
   }
 }
