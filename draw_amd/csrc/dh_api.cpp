@@ -44,7 +44,7 @@ struct ObjRes {
 struct MeshRes {
   dh_handle obj = 0;
   uint32_t* tri = nullptr;
-  float* soa = nullptr;  // per-triangle vertex streams (opaque material), or null
+  float* soa = nullptr;  // per-triangle vertex streams and records (opaque material), or null
   uint32_t n_tri = 0;
   dh_handle mat = 0;
 };
@@ -554,11 +554,11 @@ extern "C" int dh_mesh_create(dh_ctx c, dh_handle object, const uint32_t* tri, u
   HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&m.tri), std::max<size_t>(1, 9ull * n_tri) * 4));
   if (n_tri) HIPCHK(c, hipMemcpy(m.tri, tri, 36ull * n_tri, hipMemcpyHostToDevice));
   // Opaque meshes (their triangle order never changes) get per-triangle vertex
-  // streams for the geometry kernels; transparent ones are re-sorted every
-  // frame and keep reading through their index triplets.
+  // streams and records for the geometry kernels; transparent ones are
+  // re-sorted every frame and keep reading through their index triplets.
   const bool opaque = !material || c->res->mats.at(material & ((1ull << 56) - 1)).alpha >= 1.0f;
   if (opaque && n_tri) {
-    const hipError_t e = hipMalloc(reinterpret_cast<void**>(&m.soa), (size_t)dh::kMeshStreams * 4ull * n_tri);
+    const hipError_t e = hipMalloc(reinterpret_cast<void**>(&m.soa), (size_t)dh::mesh_stream_floats(n_tri) * 4ull);
     if (e != hipSuccess) {
       hipFree(m.tri);
       return hip_fail(c, e, "hipMalloc(mesh streams)");

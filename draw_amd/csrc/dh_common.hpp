@@ -60,19 +60,34 @@ struct DevMaterial {
 
 // One draw = one IndexedMesh of one Object, in submission order.
 constexpr uint32_t kDrawNoCull = 1u;
-// Per-triangle vertex streams of an opaque mesh, built once when the mesh is
-// created: stream s of triangle t at soa[s * n_tri + t]; streams 0-8 hold the
-// positions of vertices a, b, c (x, y, z each), 9-17 their normals, 18-23
-// their texture coordinates (u, v each).  The geometry kernels read each value
-// with one coalesced load instead of gathering through the index triplets.
+// Per-triangle vertex data of an opaque mesh, built once when the mesh is
+// created, in one allocation at soa:
+//  - streams: stream s of triangle t at soa[s * n_tri + t]; streams 0-8 hold
+//    the positions of vertices a, b, c (x, y, z each), 9-17 their normals,
+//    18-23 their texture coordinates (u, v each).  A wave whose triangles are
+//    consecutive reads each value with one coalesced load instead of
+//    gathering through the index triplets;
+//  - records: the same 24 values of triangle t as one 128-byte record (then 32 bytes
+//    of padding), 128-byte aligned, at float offset
+//    mesh_records_offset(n_tri) + 32 t.  A wave whose triangles lie far apart
+//    in the mesh (survivors of culling scattered among culled ones) reads one
+//    cache line per triangle there, where each stream load would fetch a
+//    line per lane for one value.
 constexpr uint32_t kMeshStreams = 24;
+constexpr uint32_t kMeshRecFloats = 32;
+__host__ __device__ inline uint64_t mesh_records_offset(uint32_t n_tri) {
+  return ((uint64_t)kMeshStreams * n_tri + (kMeshRecFloats - 1u)) & ~(uint64_t)(kMeshRecFloats - 1u);
+}
+__host__ __device__ inline uint64_t mesh_stream_floats(uint32_t n_tri) {
+  return mesh_records_offset(n_tri) + (uint64_t)kMeshRecFloats * n_tri;
+}
 
 struct DevDraw {
   const float* pos;       // xyz
   const float* nrm;       // xyz
   const float* uv;        // uv
   const uint32_t* tri;    // 9 per triangle (pos a,b,c; uv a,b,c; nrm a,b,c)
-  const float* soa;       // opaque meshes: per-triangle vertex streams (kMeshStreams x n_tri), else null
+  const float* soa;       // opaque meshes: vertex streams and records (mesh_stream_floats), else null
   uint32_t n_tri;
   uint32_t tri_begin;     // first input-triangle index of this draw
   uint32_t material;      // slot in the material table

@@ -72,13 +72,52 @@ __device__ inline void tri_positions(const DevDraw& dr, uint32_t lt, V3& a, V3& 
   }
 }
 
-// Normals and texture coordinates of the vertices of triangle lt.
-__device__ inline void tri_attributes(const DevDraw& dr, uint32_t lt, V3 (&nrm)[3], float (&u)[3], float (&v)[3]) {
-  if (dr.soa) {
+// Whether a wave reads its triangles' vertex records rather than the streams:
+// when the input triangles of its active lanes span more than eight per lane,
+// a stream load fetches several cache lines for few values each, and one
+// 128-byte record per triangle moves fewer bytes.  (At denser spans the
+// streams' coalesced loads are faster, although they may move more bytes.)
+// The choice only selects which copy of the same values is read.  Every lane
+// of the wave calls it.
+__device__ inline bool wave_sparse(bool active, uint32_t t) {
+  const uint64_t m = __ballot(active);
+  if (!m) return false;
+  const int first = __builtin_ctzll(m), last = 63 - __builtin_clzll(m);
+  const uint32_t t0 = (uint32_t)__builtin_amdgcn_readlane((int)t, first);
+  const uint32_t t1 = (uint32_t)__builtin_amdgcn_readlane((int)t, last);
+  return t1 - t0 > 512u;
+}
+
+// Positions, normals and texture coordinates of the vertices of triangle lt:
+// from the mesh's records (sparse) or streams when it has them, else through
+// its index triplet.
+__device__ inline void tri_vertices(const DevDraw& dr, uint32_t lt, bool sparse, V3 (&P)[3], V3 (&nrm)[3],
+                                    float (&u)[3], float (&v)[3]) {
+  if (dr.soa && sparse) {
+    const uint4* r = reinterpret_cast<const uint4*>(dr.soa + mesh_records_offset(dr.n_tri)) +
+                     (uint64_t)(kMeshRecFloats / 4u) * lt;
+    float f[24];
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      const uint4 x = r[q];
+      f[4 * q] = __uint_as_float(x.x);
+      f[4 * q + 1] = __uint_as_float(x.y);
+      f[4 * q + 2] = __uint_as_float(x.z);
+      f[4 * q + 3] = __uint_as_float(x.w);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      P[k] = v3(f[3 * k], f[3 * k + 1], f[3 * k + 2]);
+      nrm[k] = v3(f[9 + 3 * k], f[10 + 3 * k], f[11 + 3 * k]);
+      u[k] = f[18 + 2 * k];
+      v[k] = f[19 + 2 * k];
+    }
+  } else if (dr.soa) {
     const uint64_t n = dr.n_tri;
     const float* s = dr.soa + lt;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
+      P[k] = v3(s[(3 * k) * n], s[(3 * k + 1) * n], s[(3 * k + 2) * n]);
       nrm[k] = v3(s[(9 + 3 * k) * n], s[(10 + 3 * k) * n], s[(11 + 3 * k) * n]);
       u[k] = s[(18 + 2 * k) * n];
       v[k] = s[(19 + 2 * k) * n];
@@ -87,6 +126,7 @@ __device__ inline void tri_attributes(const DevDraw& dr, uint32_t lt, V3 (&nrm)[
     const uint32_t* ti = dr.tri + 9ull * lt;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
+      P[k] = ld3(dr.pos, ti[k]);
       nrm[k] = ld3(dr.nrm, ti[6 + k]);
       const float* uv = dr.uv + 2ull * ti[3 + k];
       u[k] = uv[0];
@@ -95,13 +135,15 @@ __device__ inline void tri_attributes(const DevDraw& dr, uint32_t lt, V3 (&nrm)[
   }
 }
 
-// Builds a mesh's per-triangle vertex streams (once, when the mesh is created).
+// Builds a mesh's vertex streams and records (once, when the mesh is created).
 __global__ __launch_bounds__(256) void k_mesh_streams(const uint32_t* __restrict__ tri, uint32_t n_tri,
                                                       const float* __restrict__ pos, const float* __restrict__ nrm,
                                                       const float* __restrict__ uv, float* __restrict__ soa) {
   const uint64_t n = n_tri;
+  uint4* rec = reinterpret_cast<uint4*>(soa + mesh_records_offset(n_tri));
   for (uint32_t t = blockIdx.x * 256u + threadIdx.x; t < n_tri; t += gridDim.x * 256u) {
     const uint32_t* q = tri + 9ull * t;
+    float f[kMeshRecFloats];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       const float* p = pos + 3ull * q[k];
@@ -109,12 +151,21 @@ __global__ __launch_bounds__(256) void k_mesh_streams(const uint32_t* __restrict
       const float* w = uv + 2ull * q[3 + k];
 #pragma unroll
       for (int j = 0; j < 3; j++) {
-        soa[(uint64_t)(3 * k + j) * n + t] = p[j];
-        soa[(uint64_t)(9 + 3 * k + j) * n + t] = m[j];
+        f[3 * k + j] = p[j];
+        f[9 + 3 * k + j] = m[j];
       }
-      soa[(uint64_t)(18 + 2 * k) * n + t] = w[0];
-      soa[(uint64_t)(19 + 2 * k) * n + t] = w[1];
+      f[18 + 2 * k] = w[0];
+      f[19 + 2 * k] = w[1];
     }
+#pragma unroll
+    for (int k = 24; k < (int)kMeshRecFloats; k++) f[k] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < (int)kMeshStreams; k++) soa[(uint64_t)k * n + t] = f[k];
+#pragma unroll
+    for (int k = 0; k < (int)kMeshRecFloats / 4; k++)
+      rec[(uint64_t)(kMeshRecFloats / 4u) * t + k] = make_uint4(
+          __float_as_uint(f[4 * k]), __float_as_uint(f[4 * k + 1]), __float_as_uint(f[4 * k + 2]),
+          __float_as_uint(f[4 * k + 3]));
   }
 }
 
@@ -736,6 +787,7 @@ __device__ inline void geom_emit_clipped(uint32_t j, const DevDraw* __restrict__
                                          uint32_t* __restrict__ dkey) {
   const uint2 sv = surv[status->n_simple + j];
   const uint32_t t = sv.x, base = sv.y;
+  const bool sparse = wave_sparse(true, t);  // the lanes of the wave still in the loop
   const DevDraw& dr = draws[find_draw(draws, n_draws, t)];
   const FrameParamsPod& fr = views[dr.view].fr;
   const uint32_t lt = t - dr.tri_begin;
@@ -744,8 +796,7 @@ __device__ inline void geom_emit_clipped(uint32_t j, const DevDraw* __restrict__
   FV v[3];
   V3 P[3], N[3];
   float U[3], W[3];
-  tri_positions(dr, lt, P[0], P[1], P[2]);
-  tri_attributes(dr, lt, N, U, W);
+  tri_vertices(dr, lt, sparse, P, N, U, W);
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     const V3 p = P[k];
@@ -797,8 +848,9 @@ __device__ inline void geom_emit_simple(uint32_t j, bool active, uint4* stage, c
   uint4 ro[6], so[9];
   uint32_t idx = 0;
   bool ok = false, skip = false;
+  const uint2 sv = active ? surv[j] : make_uint2(0u, 0u);
+  const bool sparse = wave_sparse(active, sv.x);
   if (active) {
-    const uint2 sv = surv[j];
     const uint32_t t = sv.x;
     idx = sv.y;
     const DevDraw& dr = draws[find_draw(draws, n_draws, t)];
@@ -809,8 +861,7 @@ __device__ inline void geom_emit_simple(uint32_t j, bool active, uint4* stage, c
     FV v[3];
     V3 P[3], N[3];
     float U[3], W[3];
-    tri_positions(dr, lt, P[0], P[1], P[2]);
-    tri_attributes(dr, lt, N, U, W);
+    tri_vertices(dr, lt, sparse, P, N, U, W);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       const V3 p = P[k];

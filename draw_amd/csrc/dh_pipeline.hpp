@@ -88,7 +88,7 @@ void enqueue_visibility(const PipeArgs& a, uint32_t shard, uint64_t* out, hipStr
 // (host_dev_ptr: its device address) with a kernel, so that the copy can sit
 // inside a captured frame graph.
 void enqueue_status_out(const Status* dev, Status* host_dev_ptr, uint32_t n, hipStream_t s);
-// A mesh's per-triangle vertex streams (kMeshStreams x n_tri floats at soa).
+// A mesh's per-triangle vertex streams and records (mesh_stream_floats(n_tri) floats at soa).
 void enqueue_mesh_streams(const uint32_t* tri, uint32_t n_tri, const float* pos, const float* nrm, const float* uv,
                           float* soa, hipStream_t s);
 int enqueue_radix_sort(uint32_t* const keys[2], uint32_t* const vals[2], const uint32_t* n_ptr, uint32_t cap,
