@@ -241,7 +241,7 @@ uint64_t scratch_bytes_after(dh_ctx c, uint64_t n_in, uint64_t cap_tri, uint64_t
   auto mx = [](uint64_t have, uint64_t want) { return std::max<uint64_t>(have, std::max<uint64_t>(want, 1)); };
   uint64_t b = mx(c->cnt.n, n_in) + 4 * mx(c->blk.n, 3 * ((n_in + 255) / 256 + 1)) + 4 * mx(c->surv.n, 2 * n_in);
   b += (sizeof(dh::SetupRec) * mx(c->setup.n, cap_tri)) + sizeof(dh::ShadeRec) * mx(c->shade.n, cap_tri);
-  b += 4 * (mx(c->bincnt.n, cap_tri) + mx(c->binoff.n, cap_tri) + mx(c->big.n, cap_tri));
+  b += 4 * (mx(c->bincnt.n, cap_tri) + mx(c->binoff.n, cap_tri) + mx(c->big.n, 3 * cap_tri + 4));
   b += 4 * (mx(c->keys0.n, cap_pairs) + mx(c->keys1.n, cap_pairs) + mx(c->vals0.n, cap_pairs) + mx(c->vals1.n, cap_pairs));
   b += 4 * (mx(c->hist.n, dh::kRadixHistEntries) + mx(c->partials.n, 1024)) + 8 * mx(c->ranges.n, n_tiles);
   b += 4 * (mx(c->tile_cnt.n, 2 * n_tiles) + mx(c->tile_start.n, n_tiles) + mx(c->vis.n, n_pixels));
@@ -259,7 +259,7 @@ int ensure_scratch(dh_ctx c, uint32_t n_in, uint32_t cap_tri, uint32_t cap_pairs
   HIPCHK(c, c->shade.ensure(cap_tri));
   HIPCHK(c, c->bincnt.ensure(cap_tri));
   HIPCHK(c, c->binoff.ensure(cap_tri));
-  HIPCHK(c, c->big.ensure(cap_tri));
+  HIPCHK(c, c->big.ensure(3ull * cap_tri + 4));  // flag bytes, then 8-byte bin boxes (16-byte aligned)
   HIPCHK(c, c->keys0.ensure(cap_pairs));
   HIPCHK(c, c->keys1.ensure(cap_pairs));
   HIPCHK(c, c->vals0.ensure(cap_pairs));

@@ -667,33 +667,54 @@ __device__ inline uint32_t wave_tiles(const Spans& g, int TS, int lane, int* s_l
   return o;
 }
 
-// Ordered bins, one clipped triangle: its flag byte (0 binned here, 1 wide,
-// 2 dropped: no tiles, and its records were not stored) and, when it is not
-// wide, its tile count and its tiles (bit 4 * (ty - cy0) + (tx - cx0) of
-// tmask[i], within its at most 4 x 4 tile box; the wide ones are counted a
-// wave at a time later).
-__device__ inline void bin_count_one(const SetupRec& r, int TS, uint32_t i, uint32_t* __restrict__ bincnt,
-                                     uint8_t* __restrict__ wflag, uint16_t* __restrict__ tmask) {
-  uint32_t c = 0, mask = 0;
+// Ordered bins keep, per clipped triangle, a bin box: x the tile key of the
+// corner (cx0, cy0) of its tile box, y its tiles as bits 4 * (ty - cy0) +
+// (tx - cx0) (low 16 bits, within the at most 4 x 4 box) and the view's tiles
+// per row (high 16 bits), so that the pair emit finds every tile key from
+// these 8 bytes alone; x = kWideBox for a wide triangle, whose tiles a wave
+// counts and emits later.  They live in the flag buffer after the flag bytes
+// (12 bytes per clipped triangle in all).
+constexpr uint32_t kWideBox = 0xFFFFFFFFu;
+__device__ __host__ inline uint2* bin_boxes(void* big, uint32_t cap_tri) {
+  return reinterpret_cast<uint2*>(reinterpret_cast<uint8_t*>(big) + ((cap_tri + 15u) & ~15u));
+}
+
+// Ordered bins, one clipped triangle of view vw: its flag byte (0 binned here,
+// 1 wide, 2 dropped: no tiles, and its records were not stored), its bin box
+// and, when it is not wide, its tile count.
+__device__ inline void bin_count_one(const SetupRec& r, int TS, uint32_t i, const DevView& vw,
+                                     uint32_t* __restrict__ bincnt, uint8_t* __restrict__ wflag,
+                                     uint2* __restrict__ boxes) {
+  uint32_t c = 0, mask = 0, key0 = 0;
   bool is_wide = false;
   if (!(r.flags & kDrop)) {
     Spans g;
     spans_init(r, TS, g);
     is_wide = bin_is_wide(g);
+    key0 = vw.tile_base + (uint32_t)g.cy0 * vw.ntx + (uint32_t)g.cx0;
     if (!is_wide)
       c = thread_tiles(g, TS, [&](uint32_t, int ty, int tx) { mask |= 1u << (4 * (ty - g.cy0) + (tx - g.cx0)); });
   }
   wflag[i] = is_wide ? 1u : ((r.flags & kDrop) ? 2u : 0u);
-  if (!is_wide) {
-    bincnt[i] = c;
-    tmask[i] = (uint16_t)mask;
-  }
+  boxes[i] = make_uint2(is_wide ? kWideBox : key0, mask | (vw.ntx << 16));
+  if (!is_wide) bincnt[i] = c;
 }
 
-// The tile masks of ordered bins live in the wide-flag buffer after the flags
-// (the buffer holds 4 bytes per clipped triangle: a flag byte, a 2-byte mask).
-__device__ __host__ inline uint16_t* tile_masks(void* big, uint32_t cap_tri) {
-  return reinterpret_cast<uint16_t*>(reinterpret_cast<uint8_t*>(big) + ((cap_tri + 15u) & ~15u));
+// The pairs of a triangle that is not wide from its bin box, its e-th tile at
+// position o + e (tiles in the order of the count pass).
+__device__ inline void emit_box_pairs(uint2 box, uint32_t i, uint32_t o, uint32_t cap_pairs,
+                                      uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const uint32_t ntx = box.y >> 16;
+  uint32_t mask = box.y & 0xFFFFu;
+  while (mask) {
+    const uint32_t b = (uint32_t)__builtin_ctz(mask);
+    mask &= mask - 1u;
+    if (o < cap_pairs) {
+      keys[o] = box.x + (b >> 2) * ntx + (b & 3u);
+      vals[o] = i;
+    }
+    o++;
+  }
 }
 
 // ================================================================== geometry
@@ -829,7 +850,7 @@ __device__ inline void geom_emit_clipped(uint32_t j, const DevDraw* __restrict__
 #pragma unroll
       for (int q = 0; q < 9; q++) reinterpret_cast<uint4*>(shade + idx)[q] = so[q];
     }
-    if (BIN) bin_count_one(rec, TS, idx, bincnt, wflag, tile_masks(wflag, cap_tri));
+    if (BIN) bin_count_one(rec, TS, idx, views[view], bincnt, wflag, bin_boxes(wflag, cap_tri));
     if (dkey) dkey[idx] = depth_order_key(rec, views[view].depth_base);
   };
   cull_and_clip(fr, v[0], v[1], v[2], !(dr.flags & kDrawNoCull), out);
@@ -886,7 +907,7 @@ __device__ inline void geom_emit_simple(uint32_t j, bool active, uint4* stage, c
       ok = true;
       // with ordered bins nothing reads a dropped triangle's records
       skip = BIN && (rec.flags & kDrop);
-      if (BIN) bin_count_one(rec, TS, idx, bincnt, wflag, tile_masks(wflag, cap_tri));
+      if (BIN) bin_count_one(rec, TS, idx, views[dr.view], bincnt, wflag, bin_boxes(wflag, cap_tri));
       if (dkey) dkey[idx] = depth_order_key(rec, views[dr.view].depth_base);
     }
   }
@@ -1179,23 +1200,19 @@ __global__ __launch_bounds__(256) void k_bin_count_wide(const SetupRec* __restri
 }
 
 // Pass 2.  Ordered: (tile key, triangle) pairs at the triangle's scanned
-// offset, for the stable radix sort.  Unordered: the triangle id goes straight
-// into its tile's list at tile_start + atomicAdd(cursor).
+// offset, from its bin box, for the stable radix sort.  Unordered: the
+// triangle id goes straight into its tile's list at tile_start +
+// atomicAdd(cursor).
 //
-// The pairs of one triangle that is not wide, its e-th tile at position
-// o + e (ORDERED), or at tile_start + atomicAdd(cursor).
-template <bool ORDERED>
-__device__ inline void bin_emit_tri(const SetupRec* __restrict__ setup, const DevView* __restrict__ views, int TS,
-                                    const uint8_t* __restrict__ wflag, uint32_t cap_tri, uint32_t i, uint32_t o,
-                                    const uint32_t* __restrict__ tile_start, uint32_t* __restrict__ tile_cur,
-                                    uint32_t cap_pairs, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-  auto put = [&](const DevView& vw, uint32_t e, int ty, int tx) {
+// Unordered pairs of one triangle that is not wide.
+__device__ inline void bin_emit_tri_unordered(const SetupRec* __restrict__ setup, const DevView* __restrict__ views,
+                                              int TS, uint32_t i, const uint32_t* __restrict__ tile_start,
+                                              uint32_t* __restrict__ tile_cur, uint32_t cap_pairs,
+                                              uint32_t* __restrict__ vals) {
+  auto put = [&](const DevView& vw, int ty, int tx) {
     const uint32_t key = vw.tile_base + (uint32_t)ty * vw.ntx + (uint32_t)tx;
-    const uint32_t pos = ORDERED ? o + e : tile_start[key] + atomicAdd(&tile_cur[key], 1u);
-    if (pos < cap_pairs) {
-      if (ORDERED) keys[pos] = key;
-      vals[pos] = i;
-    }
+    const uint32_t pos = tile_start[key] + atomicAdd(&tile_cur[key], 1u);
+    if (pos < cap_pairs) vals[pos] = i;
   };
   // the record's last 16 bytes first: a triangle within one tile needs no edges
   const uint4 h = reinterpret_cast<const uint4*>(setup + i)[5];
@@ -1203,26 +1220,14 @@ __device__ inline void bin_emit_tri(const SetupRec* __restrict__ setup, const De
   const uint32_t cx0 = (h.x & 0xFFFFu) / (uint32_t)TS, cx1 = (h.x >> 16) / (uint32_t)TS;
   const uint32_t cy0 = (h.y & 0xFFFFu) / (uint32_t)TS, cy1 = (h.y >> 16) / (uint32_t)TS;
   if (cx0 == cx1 && cy0 == cy1) {
-    put(views[h.w], 0u, (int)cy0, (int)cx0);
-    return;
-  }
-  if (ORDERED) {
-    // the tiles the count pass found (bit 4 * row + column of the tile box),
-    // in its order
-    const DevView& vw = views[h.w];
-    uint32_t mask = tile_masks(const_cast<uint8_t*>(wflag), cap_tri)[i], e = 0;
-    while (mask) {
-      const uint32_t b = (uint32_t)__builtin_ctz(mask);
-      mask &= mask - 1u;
-      put(vw, e++, (int)(cy0 + (b >> 2)), (int)(cx0 + (b & 3u)));
-    }
+    put(views[h.w], (int)cy0, (int)cx0);
     return;
   }
   const SetupRec r = setup[i];
   Spans g;
   spans_init(r, TS, g);
   const DevView& vw = views[r.view];
-  thread_tiles(g, TS, [&](uint32_t e, int ty, int tx) { put(vw, e, ty, tx); });
+  thread_tiles(g, TS, [&](uint32_t, int ty, int tx) { put(vw, ty, tx); });
 }
 
 template <bool ORDERED>
@@ -1234,29 +1239,37 @@ __global__ __launch_bounds__(256) void k_bin_emit(const SetupRec* __restrict__ s
                                                   uint32_t* __restrict__ tile_cur, uint32_t cap_pairs,
                                                   uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   const uint32_t n = min(status->n_out, cap_tri);
-  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u)
-    if (wflag[i] == 0u)
-      bin_emit_tri<ORDERED>(setup, views, TS, wflag, cap_tri, i, ORDERED ? binoff[i] : 0u, tile_start, tile_cur,
-                            cap_pairs, keys, vals);
+  const uint2* boxes = bin_boxes(const_cast<uint8_t*>(wflag), cap_tri);
+  for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+    if constexpr (ORDERED) {
+      const uint2 box = boxes[i];
+      if (box.x != kWideBox) emit_box_pairs(box, i, binoff[i], cap_pairs, keys, vals);
+    } else if (wflag[i] == 0u) {
+      bin_emit_tri_unordered(setup, views, TS, i, tile_start, tile_cur, cap_pairs, vals);
+    }
+  }
 }
 
 // Near-first pairs with the scan of their counts fused in: block b takes
 // chunk b of the depth order (the chunks of k_scan_reduce, whose sums are in
 // partials), scans the counts bincnt[order[j]] from the sums before it, and
-// emits each triangle's pairs at its offset (a wide triangle's offset goes to
-// binoff for k_bin_emit_wide).  Block 0 publishes the pair total and the
-// overflow flag.
-__global__ __launch_bounds__(256) void k_bin_emit_near(const SetupRec* __restrict__ setup,
-                                                       const DevView* __restrict__ views, Status* status,
-                                                       uint32_t cap_tri, int TS, const uint8_t* __restrict__ wflag,
+// emits each triangle's pairs at its offset from its bin box (a wide
+// triangle's offset goes to binoff for k_bin_emit_wide).  The ids, counts and
+// boxes of up to four rounds of 256 are loaded before the first round's scan,
+// so that a block waits on two dependent loads per four rounds.  Block 0
+// publishes the pair total and the overflow flag.
+__global__ __launch_bounds__(256) void k_bin_emit_near(Status* status, uint32_t cap_tri,
+                                                       const uint8_t* __restrict__ wflag,
                                                        uint32_t* __restrict__ binoff, const uint32_t* __restrict__ order,
                                                        const uint32_t* __restrict__ bincnt,
                                                        const uint32_t* __restrict__ partials, uint32_t cap_pairs,
                                                        uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
   __shared__ uint32_t s_tmp[4];
+  constexpr int kRounds = 4;
   const uint32_t n = min(status->n_out, cap_tri);
   const uint32_t C = scan_chunk(n);
   const uint32_t b0 = blockIdx.x * C, b1 = min(b0 + C, n);
+  const uint2* boxes = bin_boxes(const_cast<uint8_t*>(wflag), cap_tri);
   const uint32_t upto = blockIdx.x == 0 ? (uint32_t)kScanG : blockIdx.x;
   uint32_t p = 0;
   for (uint32_t k = threadIdx.x; k < upto; k += 256) p += partials[k];
@@ -1271,23 +1284,33 @@ __global__ __launch_bounds__(256) void k_bin_emit_near(const SetupRec* __restric
   } else {
     running = sum;
   }
-  for (uint32_t tb = b0; tb < b1; tb += 256u) {
-    const uint32_t j = tb + threadIdx.x;
-    const bool act = j < b1;
-    const uint32_t i = act ? order[j] : 0u;
-    const uint32_t c = act ? bincnt[i] : 0u;
-    uint32_t tot;
-    const uint32_t o = block_exclusive_scan256(c, s_tmp, &tot) + running;
-    running += tot;
-    if (!act) continue;
-    const uint8_t f = wflag[i];
-    if (f == 1u)
-      binoff[i] = o;
-    else if (f == 0u)
-      bin_emit_tri<true>(setup, views, TS, wflag, cap_tri, i, o, nullptr, nullptr, cap_pairs, keys, vals);
+  for (uint32_t tg = b0; tg < b1; tg += 256u * kRounds) {
+    uint32_t id[kRounds], c[kRounds];
+    uint2 box[kRounds];
+#pragma unroll
+    for (int k = 0; k < kRounds; k++) {
+      const uint32_t j = tg + 256u * k + threadIdx.x;
+      id[k] = j < b1 ? order[j] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < kRounds; k++) {
+      const bool act = tg + 256u * k + threadIdx.x < b1;
+      c[k] = act ? bincnt[id[k]] : 0u;
+      box[k] = act ? boxes[id[k]] : make_uint2(0u, 0u);
+    }
+#pragma unroll
+    for (int k = 0; k < kRounds; k++) {
+      uint32_t tot;
+      const uint32_t o = block_exclusive_scan256(c[k], s_tmp, &tot) + running;
+      running += tot;
+      if (tg + 256u * k + threadIdx.x >= b1) continue;
+      if (box[k].x == kWideBox)
+        binoff[id[k]] = o;
+      else
+        emit_box_pairs(box[k], id[k], o, cap_pairs, keys, vals);
+    }
   }
 }
-
 
 template <bool ORDERED>
 __global__ __launch_bounds__(256) void k_bin_emit_wide(const SetupRec* __restrict__ setup,
@@ -1675,8 +1698,8 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
       // inside the pair emit
       hipLaunchKernelGGL(k_scan_reduce, dim3(kScanG), dim3(256), 0, s, a.bincnt, &a.status->n_out, 0u, a.cap_tri,
                          a.partials, order);
-      hipLaunchKernelGGL(k_bin_emit_near, dim3(kScanG), dim3(256), 0, s, a.setup, a.views, a.status, a.cap_tri,
-                         a.TS, wflag, a.binoff, order, a.bincnt, a.partials, a.cap_pairs, a.keys[0], a.vals[0]);
+      hipLaunchKernelGGL(k_bin_emit_near, dim3(kScanG), dim3(256), 0, s, a.status, a.cap_tri, wflag, a.binoff,
+                         order, a.bincnt, a.partials, a.cap_pairs, a.keys[0], a.vals[0]);
     } else {
       scan_u32(a.bincnt, a.binoff, &a.status->n_out, 0, a.cap_tri, a.partials, &a.status->n_pairs, a.cap_pairs,
                &a.status->overflow, 2u, s);

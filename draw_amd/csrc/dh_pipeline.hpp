@@ -32,8 +32,8 @@ struct PipeArgs {
   uint32_t cap_tri;
   uint32_t* bincnt;    // [cap_tri]
   uint32_t* binoff;    // [cap_tri]
-  uint32_t* big;       // [cap_tri] words: a byte per triangle (1 = wide, binned by a wave), then a u16
-                       // tile mask per triangle (ordered bins)
+  uint32_t* big;       // [3 cap_tri + 4] words: a byte per triangle (1 = wide, binned by a wave; 2 =
+                       // dropped), then 16-byte aligned an 8-byte bin box per triangle (ordered bins)
   bool ordered_bins;   // stable radix sort (per-tile lists in submission order)
   bool shade_in_raster;  // kind 0 rasters shade their own tiles (no visibility buffer, no k_shade)
   bool depth_order;    // ordered bins: per-tile lists near first (triangles sorted by depth key before binning)
