@@ -1459,6 +1459,77 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
   }
 }
 
+// One-pass sort by keys below 2048 whose order within a key is free: the
+// near-first tile lists, whose entries only need to stay roughly in depth
+// order (a tile's visibility does not depend on the order of its list).  Each
+// block owns one contiguous chunk of the input (chunks contiguous per XCD, as
+// above), so the histogram has 2048 x 256 entries instead of 2048 per wave,
+// and its items take their output slots with LDS atomic increments of their
+// key's cursor: a tile's entries come chunk by chunk in input order, in any
+// order within a chunk.
+constexpr int kRadixBG = 256;  // blocks (chunks) of the unordered one-pass sort
+static_assert(2048u * kRadixBG <= kRadixHistEntries, "histogram size");
+
+__device__ inline uint32_t radix_chunk_blk(uint32_t n) { return (((n + kRadixBG - 1u) / kRadixBG) + 255u) & ~255u; }
+
+__global__ __launch_bounds__(256) void k_radix_hist_blk(const uint32_t* __restrict__ keys, const uint32_t* n_ptr,
+                                                        uint32_t cap, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[2048];
+  const int tid = threadIdx.x;
+  const uint32_t n = min(*n_ptr, cap);
+  const uint32_t C = radix_chunk_blk(n);
+  const uint32_t cb = xcd_remap(blockIdx.x, gridDim.x);
+  const uint32_t b0 = min(cb * C, n), b1 = min(b0 + C, n);
+  for (int d = tid; d < 2048; d += 256) h[d] = 0u;
+  __syncthreads();
+  for (uint32_t tb = b0; tb < b1; tb += 256u * kRadixU) {
+    uint32_t k[kRadixU];
+#pragma unroll
+    for (int u = 0; u < kRadixU; u++) {
+      const uint32_t i = tb + 256u * (uint32_t)u + (uint32_t)tid;
+      k[u] = i < b1 ? keys[i] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < kRadixU; u++)
+      if (tb + 256u * (uint32_t)u + (uint32_t)tid < b1) atomicAdd(&h[k[u] & 2047u], 1u);
+  }
+  __syncthreads();
+  for (int d = tid; d < 2048; d += 256) hist[(uint32_t)d * kRadixBG + cb] = h[d];
+}
+
+// vout[slot] = vin[i] for the chunk's items, slots from the scanned histogram
+// (hist_off); ranges (may be null): [start, end) of each of the first n_ranges
+// keys in the output.
+__global__ __launch_bounds__(256) void k_radix_scatter_blk(const uint32_t* __restrict__ kin,
+                                                           const uint32_t* __restrict__ vin,
+                                                           uint32_t* __restrict__ vout, const uint32_t* n_ptr,
+                                                           uint32_t cap, const uint32_t* __restrict__ hist_off,
+                                                           uint2* __restrict__ ranges, uint32_t n_ranges) {
+  __shared__ uint32_t run[2048];
+  const int tid = threadIdx.x;
+  const uint32_t n = min(*n_ptr, cap);
+  const uint32_t C = radix_chunk_blk(n);
+  const uint32_t cb = xcd_remap(blockIdx.x, gridDim.x);
+  const uint32_t b0 = min(cb * C, n), b1 = min(b0 + C, n);
+  if (ranges)
+    for (uint32_t d = blockIdx.x * 256u + (uint32_t)tid; d < n_ranges; d += gridDim.x * 256u)
+      ranges[d] = make_uint2(hist_off[d * kRadixBG], d + 1u < 2048u ? hist_off[(d + 1u) * kRadixBG] : n);
+  for (int d = tid; d < 2048; d += 256) run[d] = hist_off[(uint32_t)d * kRadixBG + cb];
+  __syncthreads();
+  for (uint32_t tb = b0; tb < b1; tb += 256u * kRadixU) {
+    uint32_t k[kRadixU], v[kRadixU];
+#pragma unroll
+    for (int u = 0; u < kRadixU; u++) {
+      const uint32_t i = tb + 256u * (uint32_t)u + (uint32_t)tid;
+      k[u] = i < b1 ? kin[i] : 0u;
+      v[u] = i < b1 ? vin[i] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < kRadixU; u++)
+      if (tb + 256u * (uint32_t)u + (uint32_t)tid < b1) vout[atomicAdd(&run[k[u] & 2047u], 1u)] = v[u];
+  }
+}
+
 // A pass's first kernel: its Status words cleared and (opaque pass) the
 // draw/view tables copied in from pinned host memory.
 __global__ __launch_bounds__(256) void k_frame_begin(uint32_t* __restrict__ status, uint32_t status_words,
@@ -1554,7 +1625,15 @@ void enqueue_mesh_streams(const uint32_t* tri, uint32_t n_tri, const float* pos,
 // n_ranges in the sorted order; not written otherwise.
 int enqueue_radix_sort(uint32_t* const keys[2], uint32_t* const vals[2], const uint32_t* n_ptr, uint32_t cap,
                        int key_bits, uint32_t* hist, uint32_t* partials, hipStream_t s, bool keys_out,
-                       uint2* ranges, uint32_t n_ranges) {
+                       uint2* ranges, uint32_t n_ranges, bool stable) {
+  if (!stable && !keys_out && key_bits <= 11 && vals[0]) {
+    const uint32_t nh = 2048u * kRadixBG;
+    hipLaunchKernelGGL(k_radix_hist_blk, dim3(kRadixBG), dim3(256), 0, s, keys[0], n_ptr, cap, hist);
+    scan_u32(hist, hist, nullptr, nh, nh, partials, nullptr, 0, nullptr, 0, s);
+    hipLaunchKernelGGL(k_radix_scatter_blk, dim3(kRadixBG), dim3(256), 0, s, keys[0], vals[0], vals[1], n_ptr, cap,
+                       hist, ranges, ranges ? n_ranges : 0u);
+    return 1;
+  }
   int cur = 0;
   // one 8-bit pass for keys of up to 8 bits, one 11-bit pass up to 11 bits
   const bool one_pass = key_bits > 8 && key_bits <= 11;
@@ -1708,12 +1787,15 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
     }
     hipLaunchKernelGGL(k_bin_emit_wide<true>, dim3(gw), dim3(256), 0, s, a.setup, a.views, a.status, wide_list,
                        a.TS, a.binoff, a.tile_start, a.tile_cur, a.cap_pairs, a.keys[0], a.vals[0]);
-    // stable radix sort by tile key (one pass when the key has at most 11
-    // bits, else 8-bit passes), then the tile ranges: from the one pass's
-    // scanned histogram, else from the sorted keys
+    // radix sort by tile key (one pass when the key has at most 11 bits, else
+    // stable 8-bit passes), then the tile ranges: from the one pass's scanned
+    // histogram, else from the sorted keys.  Near-first lists take the one
+    // pass with free order within a chunk of the depth order;
+    // submission-ordered lists stay stable.
     const bool one_pass = a.key_bits <= 11;
     const int cur = enqueue_radix_sort(a.keys, a.vals, &a.status->n_pairs, a.cap_pairs, a.key_bits, a.hist,
-                                       a.partials, s, !one_pass, one_pass ? a.ranges : nullptr, a.n_tiles);
+                                       a.partials, s, !one_pass, one_pass ? a.ranges : nullptr, a.n_tiles,
+                                       !depth_sorted);
     b.vals[0] = a.vals[cur];
     if (!one_pass)
       hipLaunchKernelGGL(k_tile_ranges, dim3(std::min<uint32_t>(cdiv(a.cap_pairs, 256), 4096u)), dim3(256), 0, s,

@@ -93,7 +93,7 @@ void enqueue_mesh_streams(const uint32_t* tri, uint32_t n_tri, const float* pos,
                           float* soa, hipStream_t s);
 int enqueue_radix_sort(uint32_t* const keys[2], uint32_t* const vals[2], const uint32_t* n_ptr, uint32_t cap,
                        int key_bits, uint32_t* hist, uint32_t* partials, hipStream_t s, bool keys_out,
-                       uint2* ranges, uint32_t n_ranges);
+                       uint2* ranges, uint32_t n_ranges, bool stable = true);
 void enqueue_transparent_prep(const PipeArgs& opaque, const TransArgs& t, hipStream_t s);
 void enqueue_blend(const PipeArgs& b, const TransArgs& t, hipStream_t s);
 bool raster_config_supported(int TS, int waves, int kind);
