@@ -50,6 +50,21 @@ __device__ inline uint32_t find_draw(const DevDraw* draws, uint32_t n, uint32_t 
   return lo;
 }
 
+// The draw of input triangle t for every active lane of a wave (every lane
+// calls it).  The lanes' triangles increase with the lane (ordered lists), so
+// the draw of the first active lane's triangle, found with scalar loads, is
+// every lane's whenever the last active lane's triangle lies before the next
+// draw; otherwise each lane searches.
+__device__ inline uint32_t wave_find_draw(const DevDraw* draws, uint32_t n, uint32_t t, bool active) {
+  const uint64_t m = __ballot(active);
+  if (!m) return 0u;
+  const uint32_t t0 = (uint32_t)__builtin_amdgcn_readlane((int)t, __builtin_ctzll(m));
+  const uint32_t t1 = (uint32_t)__builtin_amdgcn_readlane((int)t, 63 - __builtin_clzll(m));
+  const uint32_t d0 = find_draw(draws, n, t0);
+  if (d0 + 1u >= n || t1 < draws[d0 + 1u].tri_begin) return d0;
+  return find_draw(draws, n, t);
+}
+
 __device__ inline V3 ld3(const float* a, uint32_t i) {
   const float* p = a + 3ull * i;
   return v3(p[0], p[1], p[2]);
@@ -752,8 +767,9 @@ __global__ __launch_bounds__(256) void k_geom_count(const DevDraw* __restrict__ 
   const uint32_t t = blockIdx.x * 256u + threadIdx.x;
   uint32_t c = 0;
   bool clip = false;
+  const uint32_t d = wave_find_draw(draws, n_draws, t, t < n_in);
   if (t < n_in) {
-    const DevDraw& dr = draws[find_draw(draws, n_draws, t)];
+    const DevDraw& dr = draws[d];
     const FrameParamsPod& fr = views[dr.view].fr;
     PV a, b, cc;
     tri_positions(dr, t - dr.tri_begin, a.p, b.p, cc.p);
@@ -808,8 +824,9 @@ __device__ inline void geom_emit_clipped(uint32_t j, const DevDraw* __restrict__
                                          uint32_t* __restrict__ dkey) {
   const uint2 sv = surv[status->n_simple + j];
   const uint32_t t = sv.x, base = sv.y;
-  const bool sparse = wave_sparse(true, t);  // the lanes of the wave still in the loop
-  const DevDraw& dr = draws[find_draw(draws, n_draws, t)];
+  // the lanes of the wave still in the loop
+  const bool sparse = wave_sparse(true, t);
+  const DevDraw& dr = draws[wave_find_draw(draws, n_draws, t, true)];
   const FrameParamsPod& fr = views[dr.view].fr;
   const uint32_t lt = t - dr.tri_begin;
   const V3 cam = v3(fr.cam[0], fr.cam[1], fr.cam[2]);
@@ -871,10 +888,11 @@ __device__ inline void geom_emit_simple(uint32_t j, bool active, uint4* stage, c
   bool ok = false, skip = false;
   const uint2 sv = active ? surv[j] : make_uint2(0u, 0u);
   const bool sparse = wave_sparse(active, sv.x);
+  const uint32_t d = wave_find_draw(draws, n_draws, sv.x, active);
   if (active) {
     const uint32_t t = sv.x;
     idx = sv.y;
-    const DevDraw& dr = draws[find_draw(draws, n_draws, t)];
+    const DevDraw& dr = draws[d];
     const FrameParamsPod& fr = views[dr.view].fr;
     const uint32_t lt = t - dr.tri_begin;
     const V3 cam = v3(fr.cam[0], fr.cam[1], fr.cam[2]);
