@@ -144,6 +144,10 @@ SIGNATURES = {
     "dh_write_jpg": [C.c_char_p, _P, C.c_int32, C.c_int32, C.c_int32],
     "dh_get_stats": [_P, C.POINTER(DhStats)],
     "dh_draw_gui": [_P, _H, _P, C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, _H],
+    "dh_image_info": [C.c_char_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_char_p,
+                      C.c_uint32],
+    "dh_image_load": [C.c_char_p, _P, C.c_uint64, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                      C.c_char_p, C.c_uint32],
     "dh_obj_load": [C.c_char_p, C.POINTER(_P), C.c_char_p, C.c_uint32],
     "dh_obj_info": [_P, C.POINTER(DhObjSummary)],
     "dh_obj_mesh": [_P, C.c_uint32, C.POINTER(DhObjMeshDesc)],

@@ -19,7 +19,7 @@ BUILD_DIR = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("DRAWHIP_ARCH", "gfx950")
 
-SOURCES = ["dh_kernels.hip", "dh_raster.hip", "dh_api.cpp", "dh_camera.cpp", "dh_server.cpp", "dh_export.hip", "dh_obj.cpp", "dh_gui.hip", "dh_jpeg.cpp"]
+SOURCES = ["dh_kernels.hip", "dh_raster.hip", "dh_api.cpp", "dh_camera.cpp", "dh_server.cpp", "dh_export.hip", "dh_obj.cpp", "dh_gui.hip", "dh_jpeg.cpp", "dh_image.cpp", "dh_jpgdec.cpp"]
 
 FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",

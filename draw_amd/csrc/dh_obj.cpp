@@ -11,9 +11,7 @@
 //        (strtof), as Rust's str::parse::<f32> does.
 //   MTL  newmtl, Ka, Kd, Ks (required: the reference unwraps them), d
 //        (default 1), map_Ka, map_Kd.
-//   PNG  8- or 16-bit RGB/RGBA and palette images, not interlaced, decoded to
-//        8-bit RGB or RGBA (16-bit samples keep their high byte, as stb does);
-//        other formats (JPEG, grey) are reported as unsupported.
+//   maps PNG or JPEG files, decoded as dh_image.cpp describes.
 // and then the reference's own post-processing, in its f32 operation order:
 // normals normalised, positions rescaled so the largest norm is 100, faces of
 // 3 or 4 vertices (quads split a,b,c + c,d,a), a dummy (0,0) texture
@@ -35,6 +33,7 @@
 #include <vector>
 
 #include "dh_common.hpp"
+#include "dh_image.hpp"
 
 using dh::V3;
 using dh::add;
@@ -44,10 +43,6 @@ using dh::sub;
 using dh::v3;
 
 namespace {
-
-// Largest decoded texture accepted from a file: 1 GiB (a 16384 x 16384 RGBA8
-// image).
-constexpr uint64_t kMaxImageBytes = 1ull << 30;
 
 struct ObjTexture {
   std::string name;
@@ -90,121 +85,6 @@ bool read_file(const std::string& path, std::vector<uint8_t>* out) {
   const size_t got = n > 0 ? fread(out->data(), 1, (size_t)n, f) : 0;
   fclose(f);
   return got == out->size();
-}
-
-uint32_t be32(const uint8_t* p) { return ((uint32_t)p[0] << 24) | ((uint32_t)p[1] << 16) | ((uint32_t)p[2] << 8) | p[3]; }
-
-// PNG -> 8-bit RGB (comp 3) or RGBA (comp 4), rows top-down.
-int png_decode(const std::vector<uint8_t>& f, std::vector<uint8_t>* img, int32_t* W, int32_t* H, int32_t* comp,
-               std::string* err) {
-  static const uint8_t sig[8] = {0x89, 'P', 'N', 'G', '\r', '\n', 0x1A, '\n'};
-  if (f.size() < 8 || memcmp(f.data(), sig, 8) != 0) {
-    *err = "not a PNG file (only PNG textures are decoded)";
-    return DH_E_UNSUPPORTED;
-  }
-  uint32_t w = 0, h = 0;
-  int depth = 0, ctype = 0, interlace = 0;
-  std::vector<uint8_t> idat, plte, trns;
-  for (size_t p = 8; p + 12 <= f.size();) {
-    const uint32_t n = be32(&f[p]);
-    if (p + 12 + (size_t)n > f.size()) break;
-    const char* t = reinterpret_cast<const char*>(&f[p + 4]);
-    const uint8_t* d = &f[p + 8];
-    if (!memcmp(t, "IHDR", 4) && n >= 13) {
-      w = be32(d);
-      h = be32(d + 4);
-      depth = d[8];
-      ctype = d[9];
-      interlace = d[12];
-    } else if (!memcmp(t, "PLTE", 4)) {
-      plte.assign(d, d + n);
-    } else if (!memcmp(t, "tRNS", 4)) {
-      trns.assign(d, d + n);
-    } else if (!memcmp(t, "IDAT", 4)) {
-      idat.insert(idat.end(), d, d + n);
-    } else if (!memcmp(t, "IEND", 4)) {
-      break;
-    }
-    p += 12 + (size_t)n;
-  }
-  if (w == 0 || h == 0 || w > 65535 || h > 65535) {
-    *err = "bad PNG size";
-    return DH_E_ARG;
-  }
-  int chans;
-  if (ctype == 2 && (depth == 8 || depth == 16)) chans = 3;
-  else if (ctype == 6 && (depth == 8 || depth == 16)) chans = 4;
-  else if (ctype == 3 && (depth == 1 || depth == 2 || depth == 4 || depth == 8) && !plte.empty()) chans = 1;
-  else {
-    *err = "PNG colour type/bit depth not handled (grey images have no 3/4-channel form)";
-    return DH_E_UNSUPPORTED;
-  }
-  if (interlace) {
-    *err = "interlaced PNG not handled";
-    return DH_E_UNSUPPORTED;
-  }
-  const size_t bpp = (size_t)std::max(1, chans * depth / 8);           // filter unit
-  const size_t stride = ((size_t)w * chans * depth + 7) / 8;           // bytes per row
-  const int out_c = (ctype == 3) ? (trns.empty() ? 3 : 4) : chans;
-  // The header's size is untrusted: refuse one that the compressed stream
-  // cannot hold (deflate expands at most 1032:1) or whose decoded image
-  // exceeds kMaxImageBytes, before allocating for it.
-  const uint64_t raw_bytes = (uint64_t)(stride + 1) * h;
-  if (raw_bytes > (uint64_t)idat.size() * 1032u + 1024u ||
-      (uint64_t)w * h * (uint64_t)out_c > kMaxImageBytes) {
-    *err = "PNG size does not fit its data or the image limit";
-    return DH_E_ARG;
-  }
-  std::vector<uint8_t> raw((stride + 1) * h);
-  uLongf rn = (uLongf)raw.size();
-  if (uncompress(raw.data(), &rn, idat.data(), (uLong)idat.size()) != Z_OK || rn != raw.size()) {
-    *err = "corrupt PNG data";
-    return DH_E_ARG;
-  }
-  std::vector<uint8_t> cur(stride), prev(stride, 0);
-  img->assign((size_t)w * h * out_c, 0);
-  for (uint32_t y = 0; y < h; y++) {
-    const uint8_t ft = raw[(stride + 1) * y];
-    const uint8_t* src = &raw[(stride + 1) * y + 1];
-    for (size_t i = 0; i < stride; i++) {
-      const int a = i >= bpp ? cur[i - bpp] : 0, b = prev[i], c = i >= bpp ? prev[i - bpp] : 0;
-      int v = src[i];
-      switch (ft) {
-        case 0: break;
-        case 1: v += a; break;
-        case 2: v += b; break;
-        case 3: v += (a + b) >> 1; break;
-        case 4: {
-          const int pp = a + b - c, pa = abs(pp - a), pb = abs(pp - b), pc = abs(pp - c);
-          v += (pa <= pb && pa <= pc) ? a : (pb <= pc ? b : c);
-          break;
-        }
-        default: *err = "bad PNG filter"; return DH_E_ARG;
-      }
-      cur[i] = (uint8_t)v;
-    }
-    uint8_t* dst = &(*img)[(size_t)y * w * out_c];
-    for (uint32_t x = 0; x < w; x++) {
-      if (ctype == 3) {
-        uint32_t idx;
-        if (depth == 8) idx = cur[x];
-        else {
-          const uint32_t per = 8 / depth, sh = 8 - depth * (1 + x % per);
-          idx = (cur[x / per] >> sh) & ((1u << depth) - 1u);
-        }
-        for (int k = 0; k < 3; k++) dst[x * out_c + k] = (3 * idx + k < plte.size()) ? plte[3 * idx + k] : 0;
-        if (out_c == 4) dst[x * 4 + 3] = idx < trns.size() ? trns[idx] : 255;
-      } else {
-        for (int k = 0; k < chans; k++)
-          dst[x * out_c + k] = depth == 8 ? cur[(size_t)x * chans + k] : cur[((size_t)x * chans + k) * 2];
-      }
-    }
-    std::swap(cur, prev);
-  }
-  *W = (int32_t)w;
-  *H = (int32_t)h;
-  *comp = out_c;
-  return DH_OK;
 }
 
 std::vector<std::string> split_ws(const std::string& s) {
@@ -258,7 +138,7 @@ int load_mtl(dh_obj o, const std::string& path, const std::string& dir) {
       std::vector<uint8_t> img;
       int32_t w, h, c;
       std::string e;
-      const int rc = png_decode(file, &img, &w, &h, &c, &e);
+      const int rc = dh::image_decode(file.data(), file.size(), &img, &w, &h, &c, &e);
       if (rc) {
         o->err = "texture " + fn + ": " + e;
         return rc;

@@ -58,6 +58,26 @@ class TextureMap:
         """TextureMap::default: a 1x1 white RGB map (scene/mod.rs:128-135)."""
         return TextureMap(np.full((1, 1, 3), 255, np.uint8))
 
+    @staticmethod
+    def load_from_file(path) -> "TextureMap":
+        """TextureMap::load_from_file (scene/mod.rs:174-202): a PNG or JPEG
+        file decoded by libdrawhip (dh_image_info, then dh_image_load)."""
+        import ctypes as C
+
+        from . import _lib
+        L = _lib.lib()
+        w, h, c = C.c_int32(), C.c_int32(), C.c_int32()
+        err = C.create_string_buffer(512)
+        rc = L.dh_image_info(str(path).encode(), C.byref(w), C.byref(h), C.byref(c), err, 512)
+        if rc != 0:
+            raise _lib.DrawHipError(rc, "dh_image_info", err.value.decode(errors="replace"))
+        img = np.empty((h.value, w.value, c.value), np.uint8)
+        rc = L.dh_image_load(str(path).encode(), img.ctypes.data, img.nbytes, C.byref(w), C.byref(h), C.byref(c),
+                             err, 512)
+        if rc != 0:
+            raise _lib.DrawHipError(rc, "dh_image_load", err.value.decode(errors="replace"))
+        return TextureMap(img)
+
 
 @dataclass
 class Texture:

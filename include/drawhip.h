@@ -299,10 +299,25 @@ typedef struct { float clip_rect[4]; uint32_t idx_offset, count, vtx_offset, pad
 int dh_draw_gui(dh_ctx ctx, dh_handle canvas, const dh_gui_vertex* vtx, uint32_t n_vtx, const uint32_t* idx,
                 uint32_t n_idx, const dh_gui_cmd* cmds, uint32_t n_cmds, dh_handle font);
 
+/* ---- texture files ---------------------------------------------------- */
+/* TextureMap::load_from_file (scene/mod.rs:174-202), whose stb_image calls
+ * are stbi_info_from_reader then stbi_load_from_reader: a PNG or JPEG file's
+ * size and component count (3 or 4; every colour JPEG reports 3), then its
+ * 8-bit pixels, rows top-down, into out (cap bytes, at least W*H*comp, else
+ * DH_E_CAPACITY).  PNG: 8/16-bit RGB, RGBA and palette images, not
+ * interlaced.  JPEG: baseline and progressive Huffman, 8-bit, YCbCr, RGB,
+ * CMYK or YCCK, any sampling factors that divide the largest.  Grey images
+ * (the reference reaches unreachable!()) and other forms return
+ * DH_E_UNSUPPORTED; corrupt files DH_E_ARG.  On failure err (if given)
+ * receives a message.  The pixels go to dh_texture_create. */
+int dh_image_info(const char* path, int32_t* W, int32_t* H, int32_t* comp, char* err, uint32_t err_len);
+int dh_image_load(const char* path, uint8_t* out, uint64_t cap, int32_t* W, int32_t* H, int32_t* comp,
+                  char* err, uint32_t err_len);
+
 /* ---- OBJ / MTL ingest ------------------------------------------------- */
 /* Object::load_from_file (object.rs:106-419) on the host: OBJ + MTL parsing
- * (the reference uses the `obj` crate), PNG textures (the reference uses
- * stb_image; JPEG is not decoded here: DH_E_UNSUPPORTED), normals normalised,
+ * (the reference uses the `obj` crate), PNG and JPEG textures decoded as
+ * dh_image_load decodes them, normals normalised,
  * positions rescaled to a largest norm of 100, quads split (a,b,c)+(c,d,a),
  * a dummy texture coordinate per group lacking them, generated normals per
  * group lacking them.  Textures: index 0 is Texture::default, then every

@@ -70,12 +70,20 @@ def _norm(v):
     return np.sqrt((v[..., 0] * v[..., 0] + v[..., 1] * v[..., 1]) + v[..., 2] * v[..., 2])
 
 
-def _write_model(tmp_path):
+def _write_model(tmp_path, fmt="png"):
+    """The test model; its diffuse map as a PNG, or as a progressive JPEG
+    (fmt "jpg", written by Pillow)."""
     rng = np.random.default_rng(3)
     tex = rng.integers(0, 256, (5, 7, 3), dtype=np.uint8)
-    (tmp_path / "wood.png").write_bytes(_png(tex))
+    if fmt == "png":
+        (tmp_path / "wood.png").write_bytes(_png(tex))
+    else:
+        from PIL import Image
+        tex = np.kron(tex, np.ones((8, 8, 1), np.uint8))
+        Image.fromarray(tex).save(tmp_path / "wood.jpg", quality=90, progressive=True)
     (tmp_path / "m.mtl").write_text(
-        "# materials\nnewmtl Wood\nNs 10\nKa 0.1 0.2 0.3\nKd 0.7 0.6 0.5\nKs 0.25 0.25 0.25\nd 1.0\nmap_Kd wood.png\n"
+        "# materials\nnewmtl Wood\nNs 10\nKa 0.1 0.2 0.3\nKd 0.7 0.6 0.5\nKs 0.25 0.25 0.25\nd 1.0\n"
+        f"map_Kd wood.{fmt}\n"
         "\nnewmtl Glass\nKa 0 0 0\nKd 0.1 0.2 0.9\nKs 1 1 1\nd 0.35\n")
     (tmp_path / "m.obj").write_text(
         "mtllib m.mtl\no Thing\n"
@@ -175,9 +183,13 @@ def test_loader_errors(tmp_path):
     (tmp_path / "c.obj").write_text("mtllib c.mtl\nv 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n")
     with pytest.raises(DrawHipError, match="lacks"):
         load_obj(str(tmp_path / "c.obj"))
+    # a damaged JPEG header, then an image format that is not decoded
     (tmp_path / "d.jpg").write_bytes(b"\xff\xd8\xff\xe0" + bytes(16))
     (tmp_path / "d.mtl").write_text("newmtl X\nKa 1 1 1\nKd 1 1 1\nKs 1 1 1\nmap_Kd d.jpg\n")
     (tmp_path / "d.obj").write_text("mtllib d.mtl\nv 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n")
+    with pytest.raises(DrawHipError, match="DH_E_ARG"):
+        load_obj(str(tmp_path / "d.obj"))
+    (tmp_path / "d.jpg").write_bytes(b"GIF89a" + bytes(16))
     with pytest.raises(DrawHipError, match="DH_E_UNSUPPORTED"):
         load_obj(str(tmp_path / "d.obj"))
 
@@ -207,11 +219,12 @@ def test_reference_donut_model():
 
 
 @pytest.mark.gpu
-def test_loaded_model_renders_bit_exact(renderer, tmp_path):
+@pytest.mark.parametrize("fmt", ["png", "jpg"])
+def test_loaded_model_renders_bit_exact(renderer, tmp_path, fmt):
     from draw_amd.obj import upload_obj
     from draw_amd.scene import Camera, Scene
     from parity import assert_same, oracle
-    _write_model(tmp_path)
+    _write_model(tmp_path, fmt)
     obj = load_obj(str(tmp_path / "m.obj"))
     s = Scene(640, 480, Camera(position=(30.0, 40.0, 150.0)), (0.0, 300.0, 300.0), [obj])
     c, w = renderer.render(s)
