@@ -239,6 +239,13 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
   uint32_t* const s_id = reinterpret_cast<uint32_t*>(s_hdr + CH + 3 * CH);
   __shared__ unsigned long long s_vis[TS * TS];
   __shared__ uint32_t s_q[NW][QCAP];
+  // How an exact hit's edge constants reach the wave.  With several quadrants
+  // per wave (kLdsHit): s*A and s*B from the staged s_pre, the wave's bases E
+  // from s_eh, where the pre-filtering lane wrote them -- broadcast LDS reads,
+  // no VALU work.  With one quadrant per wave the hit loop has too little work
+  // per hit to cover the LDS latency: readlanes from the pre-filtering lane.
+  constexpr bool kLdsHit = QPW >= 2;
+  __shared__ float4 s_eh[kLdsHit ? NW : 1][64];
   __shared__ uint32_t s_nfrag;  // the tile's fragment count (one global atomic per tile)
   __shared__ float s_unit[SHADE ? 256 : 1];
 
@@ -449,6 +456,10 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
           }
         }
       }
+      if constexpr (kLdsHit) {
+        if (hit && wexact) s_eh[wave][lane] = make_float4(pE[0], pE[1], pE[2], 0.0f);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // s_eh writes by other lanes of this wave
+      }
       // Exact two-FMA hits in a tight loop, then the rest (the visibility
       // minimum does not depend on the order of the visits).
       uint64_t hf = __ballot(hit && wexact);
@@ -458,18 +469,33 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
         const int k = g0 + kl;
         hf &= hf - 1;
         if (qn > kQFlush) flush();  // room for this triangle's visits
-        // the record's values, from the registers of the lane that pre-filtered it
+        // the record's quadrant mask and depth bound from the registers of the
+        // lane that pre-filtered it, its edge constants as kLdsHit says
         const uint32_t qmk = (uint32_t)__builtin_amdgcn_readlane((int)qm, kl);
         const uint32_t zkey = (uint32_t)__builtin_amdgcn_readlane((int)h.w, kl);
         float A[3], B[3], E[3];
+        if constexpr (kLdsHit) {
+          const float4 eh = s_eh[wave][kl];
+          E[0] = eh.x;
+          E[1] = eh.y;
+          E[2] = eh.z;
 #pragma unroll
-        for (int e = 0; e < 3; e++) {
-          A[e] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pA[e]), kl));
-          B[e] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pB[e]), kl));
-          const float Es = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pE[e]), kl));
-          // once per triangle into a vector register: a VALU instruction reads
-          // at most one scalar register, and each fma below already takes A or B
-          asm("v_mov_b32 %0, %1" : "=v"(E[e]) : "s"(Es));
+          for (int e = 0; e < 3; e++) {
+            const float4 q = s_pre[k][e];
+            A[e] = q.x;
+            B[e] = q.y;
+          }
+        } else {
+#pragma unroll
+          for (int e = 0; e < 3; e++) {
+            A[e] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pA[e]), kl));
+            B[e] = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pB[e]), kl));
+            const float Es = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(pE[e]), kl));
+            // once per triangle into a vector register: a VALU instruction
+            // reads at most one scalar register, and each fma below already
+            // takes A or B
+            asm("v_mov_b32 %0, %1" : "=v"(E[e]) : "s"(Es));
+          }
         }
 #pragma unroll
         for (int qq = 0; qq < QPW; qq++) {
