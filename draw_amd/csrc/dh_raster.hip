@@ -454,6 +454,8 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
               if (cmax < 0.0f) qm &= ~(1u << qq);
             }
           }
+          // an exact record that covers none of the wave's quadrants is no hit
+          if (wexact && qm == 0u) hit = false;
         }
       }
       if constexpr (kLdsHit) {
