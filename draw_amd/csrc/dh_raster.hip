@@ -11,6 +11,10 @@
 
 namespace dh {
 
+// Workgroups of `bytes` of LDS that fit the 160 KB of a CU, allocated in
+// 1 KB units.
+constexpr int lds_workgroups(int bytes) { return (160 * 1024) / (((bytes + 1023) / 1024) * 1024); }
+
 // Coverage of one pixel for a FAST triangle, decided without division.  Edge
 // values are integers here (snapped vertices are floor() results and pixel
 // centres are integers), so for a finite nonzero normalizer f the quotient
@@ -239,15 +243,19 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
   uint32_t* const s_id = reinterpret_cast<uint32_t*>(s_hdr + CH + 3 * CH);
   __shared__ unsigned long long s_vis[TS * TS];
   __shared__ uint32_t s_q[NW][QCAP];
-  // How an exact hit's edge constants reach the wave.  With several quadrants
-  // per wave (kLdsHit): s*A and s*B from the staged s_pre, the wave's bases E
-  // from s_eh, where the pre-filtering lane wrote them -- broadcast LDS reads,
-  // no VALU work.  With one quadrant per wave the hit loop has too little work
-  // per hit to cover the LDS latency: readlanes from the pre-filtering lane.
-  constexpr bool kLdsHit = QPW >= 2;
-  __shared__ float4 s_eh[kLdsHit ? NW : 1][64];
   __shared__ uint32_t s_nfrag;  // the tile's fragment count (one global atomic per tile)
   __shared__ float s_unit[SHADE ? 256 : 1];
+  // How an exact hit's edge constants reach the wave.  kLdsHit: s*A and s*B
+  // from the staged s_pre, the wave's bases E from s_eh, where the
+  // pre-filtering lane wrote them -- broadcast LDS reads, no VALU work.
+  // Otherwise readlanes from the pre-filtering lane.  The LDS path is taken
+  // with several quadrants per wave (with one, a hit has too little work to
+  // cover the LDS latency) and only when s_eh costs no resident workgroup
+  // (160 KB of LDS per CU, counted in 1 KB allocation units).
+  constexpr int kLdsBase = (int)(sizeof(s_chunk) + sizeof(s_vis) + sizeof(s_q) + sizeof(s_nfrag) + sizeof(s_unit));
+  constexpr int kLdsEh = NW * 64 * 16;
+  constexpr bool kLdsHit = QPW >= 2 && lds_workgroups(kLdsBase + kLdsEh) == lds_workgroups(kLdsBase);
+  __shared__ float4 s_eh[kLdsHit ? NW : 1][64];
 
   const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
   const DevView& vw = views[wave_find_view_tile(views, n_views, tile)];
