@@ -1359,6 +1359,112 @@ __global__ __launch_bounds__(256) void k_bin_emit_wide(const SetupRec* __restric
   });
 }
 
+// Binning of a small single-view frame in one workgroup of 1024 threads,
+// in place of the general path's dozen launches (which a frame of a few
+// thousand triangles spends mostly starting and draining): per-tile counts
+// with LDS atomics from each triangle's bin box (a wave per wide triangle,
+// as k_bin_count_wide), their scan into the tile ranges, then each
+// triangle's id written into its tiles' lists at LDS-atomic cursors.  The
+// lists are unordered, which changes no result: a tile's visibility is the
+// minimum of its fragments' keys whatever the order (the transparent pass,
+// which blends in list order, never comes here).  Pairs past cap_pairs are
+// counted, flagged (overflow bit 2) and not written; ranges end at cap_pairs.
+constexpr uint32_t kSmallBinTiles = 4096;  // LDS counters and cursors: 32 KB
+constexpr uint32_t kSmallBinTris = 16384;  // input triangles of a frame binned this way
+
+__global__ __launch_bounds__(1024) void k_bin_small(const SetupRec* __restrict__ setup,
+                                                    const DevView* __restrict__ views, Status* __restrict__ status,
+                                                    uint32_t cap_tri, int TS, const uint8_t* __restrict__ wflag,
+                                                    uint32_t* __restrict__ wide_list, uint32_t n_tiles,
+                                                    uint32_t cap_pairs, uint32_t* __restrict__ vals,
+                                                    uint2* __restrict__ ranges) {
+  __shared__ uint32_t s_cnt[kSmallBinTiles];
+  __shared__ uint32_t s_cur[kSmallBinTiles];
+  __shared__ int s_lo[16][64];
+  __shared__ uint32_t s_off[16][64];
+  __shared__ uint32_t s_w[16];
+  __shared__ uint32_t s_nwide;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (uint32_t k = (uint32_t)tid; k < n_tiles; k += 1024u) s_cnt[k] = 0u;
+  if (tid == 0) s_nwide = 0u;
+  __syncthreads();
+  const uint32_t n = min(status->n_out, cap_tri);
+  const uint2* boxes = bin_boxes(const_cast<uint8_t*>(wflag), cap_tri);
+  const DevView& vw = views[0];
+  // one pass over the triangles' tiles: f(tile key) for each
+  auto each_tile = [&](auto f) {
+    for (uint32_t i = (uint32_t)tid; i < n; i += 1024u) {
+      if (wflag[i] != 0u) continue;  // wide (below) or dropped
+      const uint2 box = boxes[i];
+      const uint32_t ntx = box.y >> 16;
+      uint32_t mask = box.y & 0xFFFFu;
+      while (mask) {
+        const uint32_t b = (uint32_t)__builtin_ctz(mask);
+        mask &= mask - 1u;
+        f(i, box.x + (b >> 2) * ntx + (b & 3u));
+      }
+    }
+  };
+  auto each_wide_tile = [&](uint32_t nw, auto f) {
+    for (uint32_t w = (uint32_t)wave; w < nw; w += 16u) {
+      const uint32_t i = (uint32_t)__builtin_amdgcn_readfirstlane((int)wide_list[w]);
+      const SetupRec r = setup[i];
+      Spans g;
+      spans_init(r, TS, g);
+      spans_init_rows(r, g);
+      wave_tiles(g, TS, lane, s_lo[wave], s_off[wave], [&](uint32_t, int ty, int tx) {
+        f(i, vw.tile_base + (uint32_t)ty * vw.ntx + (uint32_t)tx);
+      });
+    }
+  };
+  // counts, and the list of wide triangles
+  for (uint32_t i = (uint32_t)tid; i < n; i += 1024u)
+    if (wflag[i] == 1u) wide_list[atomicAdd(&s_nwide, 1u)] = i;
+  each_tile([&](uint32_t, uint32_t key) { atomicAdd(&s_cnt[key], 1u); });
+  __syncthreads();
+  const uint32_t nw = s_nwide;
+  each_wide_tile(nw, [&](uint32_t, uint32_t key) { atomicAdd(&s_cnt[key], 1u); });
+  __syncthreads();
+  // exclusive scan of the counts, 1024 tiles at a time with a carry
+  uint32_t carry = 0;
+  for (uint32_t k0 = 0; k0 < n_tiles; k0 += 1024u) {
+    const uint32_t k = k0 + (uint32_t)tid;
+    const uint32_t c = k < n_tiles ? s_cnt[k] : 0u;
+    uint32_t x = c;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) s_w[wave] = x;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (int w = 0; w < 16; w++) {
+      if (w < wave) before += s_w[w];
+      all += s_w[w];
+    }
+    __syncthreads();
+    const uint32_t start = carry + before + x - c;
+    if (k < n_tiles) {
+      s_cur[k] = start;
+      ranges[k] = make_uint2(start, min(start + c, cap_pairs));
+    }
+    carry += all;
+  }
+  if (tid == 0) {
+    status->n_pairs = carry;
+    if (carry > cap_pairs) atomicOr(&status->overflow, 2u);
+  }
+  __syncthreads();
+  // the lists
+  auto put = [&](uint32_t i, uint32_t key) {
+    const uint32_t pos = atomicAdd(&s_cur[key], 1u);
+    if (pos < cap_pairs) vals[pos] = i;
+  };
+  each_tile(put);
+  each_wide_tile(nw, put);
+}
+
 __global__ __launch_bounds__(256) void k_ranges_from_counts(const uint32_t* __restrict__ tile_start,
                                                             const uint32_t* __restrict__ tile_cnt, uint32_t n_tiles,
                                                             uint32_t cap_pairs, uint2* __restrict__ ranges) {
@@ -1709,6 +1815,9 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
   // near-first lists keep the depth sort in keys[1] / vals[1] (behind the
   // wide-triangle list), which needs cap_pairs >= 3 cap_tri
   const bool depth_sorted = a.ordered_bins && a.depth_order && (uint64_t)a.cap_pairs >= 3ull * a.cap_tri;
+  // a small single-view frame is binned by one workgroup (k_bin_small)
+  const bool small_bins = depth_sorted && a.n_views == 1 && a.n_tiles <= kSmallBinTiles &&
+                          a.n_tri_in <= kSmallBinTris;
   if (a.ev[0]) (void)hipEventRecord(a.ev[0], s);
   hipLaunchKernelGGL(k_frame_begin, dim3(a.tables_src ? std::min<uint32_t>(cdiv(a.tables_words, 1024), 64u) : 1u),
                      dim3(256), 0, s, reinterpret_cast<uint32_t*>(a.status), (uint32_t)(sizeof(Status) / 4),
@@ -1737,7 +1846,7 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
     // ordered bins: the per-triangle tile counts and wide flags come with the
     // records, and for near-first lists each record's depth key
     uint8_t* wf = reinterpret_cast<uint8_t*>(a.big);
-    uint32_t* dk = depth_sorted ? a.keys[1] + a.cap_tri : nullptr;
+    uint32_t* dk = depth_sorted && !small_bins ? a.keys[1] + a.cap_tri : nullptr;
     // the unclipped and clipped emits write disjoint outputs: the clipped
     // one runs beside the other on the side stream.  Capped grids (the lists'
     // lengths are on the device): 2048 and 1024 blocks, a grid-stride loop
@@ -1772,7 +1881,10 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
   const uint32_t gl = std::min<uint32_t>(cdiv(a.cap_tri, 4096), 1024u);
   const uint32_t gw = std::min<uint32_t>(cdiv(a.cap_tri, 64), 4096u);
   PipeArgs b = a;
-  if (a.ordered_bins) {
+  if (small_bins) {
+    hipLaunchKernelGGL(k_bin_small, dim3(1), dim3(1024), 0, s, a.setup, a.views, a.status, a.cap_tri, a.TS, wflag,
+                       wide_list, a.n_tiles, a.cap_pairs, a.vals[0], a.ranges);
+  } else if (a.ordered_bins) {
     // the wide-triangle list and counts (keys[1][0, cap_tri), bincnt) beside
     // the depth sort (keys[1] + cap_tri .., vals[1], hist, partials)
     const hipStream_t s2 = depth_sorted ? fork(s, a, 2) : s;

@@ -109,6 +109,29 @@ def test_bin_order_modes_identical(renderer):
     assert_same(*frames[0], oracle(s), label="soup200k full frame")
 
 
+@pytest.mark.parametrize("kind,tile,waves", [(0, 32, 8), (0, 16, 4), (0, 16, 2), (1, 32, 4), (0, 8, 1)])
+def test_small_frame_binning_matches_general_path(renderer, kind, tile, waves):
+    """A frame of at most 16384 input triangles and 4096 tiles is binned by one
+    workgroup (k_bin_small, unordered lists) in the default bin order; the
+    general path (submission-ordered lists, mode 1) gives the same frame bytes
+    and the same number of bin entries, and both give the oracle's frame.  The
+    soup's near-plane straddlers clip into triangles many tiles wide; at 8x8
+    tiles the frame has 14400 tiles and takes the general path in both modes."""
+    s = workloads.soup_scene(7, n_tri=16_000, width=1280, height=720, tex_size=256)
+    ref = oracle(s)
+    got = {}
+    try:
+        for mode in (2, 1):
+            renderer.set_bin_order(mode)
+            got[mode] = (_gpu(renderer, s, tile, waves, kind), renderer.stats()["bin_entries"])
+    finally:
+        renderer.set_bin_order(2)
+    assert got[2][1] == got[1][1] > 0
+    for x, y in zip(got[2][0], got[1][0]):
+        assert np.array_equal(x.view(np.uint8), y.view(np.uint8))
+    assert_same(*got[2][0], ref, label=f"16k soup, kind {kind} tile {tile}x{waves}")
+
+
 def test_near_first_back_to_front_submission(renderer):
     """A soup submitted back to front (the near-first order's worst case for
     submission-ordered lists) gives the oracle's frame in every bin mode."""
