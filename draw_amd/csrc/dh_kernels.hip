@@ -745,6 +745,12 @@ __device__ inline uint32_t depth_order_key(const SetupRec& r, uint32_t depth_bas
   return min((b - depth_base) >> 19, (1u << kDepthKeyBits) - 1u);
 }
 
+// Small single-view frames: at most kSmallBinTris input triangles and
+// kSmallBinTiles tiles take single-workgroup kernels for the geometry lists
+// (k_geom_small) and the binning (k_bin_small), and one launch for both emits.
+constexpr uint32_t kSmallBinTiles = 4096;  // LDS counters and cursors: 32 KB
+constexpr uint32_t kSmallBinTris = 16384;  // LDS count bytes: 16 KB
+
 // cnt[t]: clipped-triangle count (bits 0-2, at most 4) | kCntClip when the
 // triangle crosses the near or far plane.
 constexpr uint32_t kCntMask = 7u, kCntClip = 8u;
@@ -788,6 +794,97 @@ __global__ __launch_bounds__(256) void k_geom_count(const DevDraw* __restrict__ 
   }
 }
 
+// Exclusive scan of three values per thread over a block of 1024 threads,
+// packed 22 | 21 | 21 bits into 64 (a block has at most 4096 of the first,
+// 1024 of the others); *total gets the three totals, packed alike.
+__device__ inline uint64_t block_exclusive_scan1024x3(uint64_t v, uint64_t* s_w /*[16]*/, uint64_t* total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint64_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_w[wave] = x;
+  __syncthreads();
+  uint64_t before = 0, all = 0;
+  for (int w = 0; w < 16; w++) {
+    if (w < wave) before += s_w[w];
+    all += s_w[w];
+  }
+  __syncthreads();
+  *total = all;
+  return before + x - v;
+}
+
+// k_geom_count, the scan of its block counts and k_geom_compact for a small
+// single-view frame (at most kSmallBinTris input triangles), in one
+// workgroup of 1024 threads: the same cnt, the same per-256-triangle output
+// offsets in blk, the same status counts and the same two ordered lists,
+// without two launches and their gaps.
+__global__ __launch_bounds__(1024) void k_geom_small(const DevDraw* __restrict__ draws, uint32_t n_draws,
+                                                     const DevView* __restrict__ views, uint32_t n_in,
+                                                     uint8_t* __restrict__ cnt, uint32_t* __restrict__ blk,
+                                                     Status* __restrict__ status, uint32_t cap_tri,
+                                                     uint2* __restrict__ surv) {
+  __shared__ uint64_t s_w[16];
+  __shared__ uint8_t s_cnt[kSmallBinTris];
+  const uint32_t tid = threadIdx.x;
+  auto field = [](uint64_t x, int k) { return (uint32_t)((x >> (k == 0 ? 42 : (k == 1 ? 21 : 0))) & 0x1FFFFFu); };
+  auto pack = [](uint32_t c, bool clip) {
+    return ((uint64_t)c << 42) | ((uint64_t)(c && !clip) << 21) | (uint64_t)(c && clip);
+  };
+  // counts, output offsets, the unclipped emitters' entries
+  uint32_t run_o = 0, run_s = 0, run_c = 0;
+  for (uint32_t c0 = 0; c0 < n_in; c0 += 1024u) {
+    const uint32_t t = c0 + tid;
+    uint32_t c = 0;
+    bool clip = false;
+    const uint32_t d = wave_find_draw(draws, n_draws, t, t < n_in);
+    if (t < n_in) {
+      const DevDraw& dr = draws[d];
+      const FrameParamsPod& fr = views[dr.view].fr;
+      PV a, b, cc;
+      tri_positions(dr, t - dr.tri_begin, a.p, b.p, cc.p);
+      c = (uint32_t)cull_and_clip(fr, a, b, cc, !(dr.flags & kDrawNoCull),
+                                  [](int, const PV&, const PV&, const PV&) {});
+      if (c) clip = !(inside_plane(fr.planes[0], a.p, b.p, cc.p) && inside_plane(fr.planes[1], a.p, b.p, cc.p));
+      const uint8_t q = (uint8_t)(c | (clip ? kCntClip : 0u));
+      cnt[t] = q;
+      s_cnt[t] = q;
+    }
+    uint64_t tot;
+    const uint64_t ex = block_exclusive_scan1024x3(pack(c, clip), s_w, &tot);
+    const uint32_t first = run_o + field(ex, 0);
+    if (t < n_in && (t & 255u) == 0u) blk[t >> 8] = first;
+    if (c && !clip) surv[run_s + field(ex, 1)] = make_uint2(t, first);
+    run_o += field(tot, 0);
+    run_s += field(tot, 1);
+    run_c += field(tot, 2);
+  }
+  // the clipped emitters' entries, after the unclipped ones
+  if (run_c) {
+    uint32_t r_o = 0, r_c = 0;
+    for (uint32_t c0 = 0; c0 < n_in; c0 += 1024u) {
+      const uint32_t t = c0 + tid;
+      const uint32_t q = t < n_in ? (uint32_t)s_cnt[t] : 0u;
+      const uint32_t c = q & kCntMask;
+      const bool clip = (q & kCntClip) != 0u;
+      uint64_t tot;
+      const uint64_t ex = block_exclusive_scan1024x3(pack(c, clip), s_w, &tot);
+      if (c && clip) surv[run_s + r_c + field(ex, 2)] = make_uint2(t, r_o + field(ex, 0));
+      r_o += field(tot, 0);
+      r_c += field(tot, 2);
+    }
+  }
+  if (tid == 0) {
+    status->n_out = run_o;
+    status->n_simple = run_s;
+    status->n_clip = run_c;
+    if (run_o > cap_tri) atomicOr(&status->overflow, 1u);
+  }
+}
+
 // Ordered lists of the input triangles that emit, (triangle, first output
 // id): the unclipped ones in surv[0, n_simple), the clipped ones after them.
 __global__ __launch_bounds__(256) void k_geom_compact(uint32_t n_in, const uint8_t* __restrict__ cnt,
@@ -816,12 +913,12 @@ __global__ __launch_bounds__(256) void k_geom_compact(uint32_t n_in, const uint8
 // triangles take no lane).  The unclipped list (CLIP false) skips the
 // clipper, whose attribute registers would otherwise size the whole kernel.
 template <bool BIN>
-__device__ inline void geom_emit_clipped(uint32_t j, const DevDraw* __restrict__ draws, uint32_t n_draws,
-                                         const DevView* __restrict__ views, const uint2* __restrict__ surv,
-                                         const DevMaterial* __restrict__ mats, SetupRec* __restrict__ setup,
-                                         ShadeRec* __restrict__ shade, uint32_t cap_tri, Status* __restrict__ status,
-                                         int TS, uint32_t* __restrict__ bincnt, uint8_t* __restrict__ wflag,
-                                         uint32_t* __restrict__ dkey) {
+__device__ __forceinline__ void geom_emit_clipped(uint32_t j, const DevDraw* __restrict__ draws, uint32_t n_draws,
+                                                  const DevView* __restrict__ views, const uint2* __restrict__ surv,
+                                                  const DevMaterial* __restrict__ mats, SetupRec* __restrict__ setup,
+                                                  ShadeRec* __restrict__ shade, uint32_t cap_tri, Status* __restrict__ status,
+                                                  int TS, uint32_t* __restrict__ bincnt, uint8_t* __restrict__ wflag,
+                                                  uint32_t* __restrict__ dkey) {
   const uint2 sv = surv[status->n_simple + j];
   const uint32_t t = sv.x, base = sv.y;
   // the lanes of the wave still in the loop
@@ -877,12 +974,12 @@ __device__ inline void geom_emit_clipped(uint32_t j, const DevDraw* __restrict__
 // past the list end with active false): one record each at the entry's output
 // id, stored through the wave's LDS area stage[64 * 9].
 template <bool BIN>
-__device__ inline void geom_emit_simple(uint32_t j, bool active, uint4* stage, const DevDraw* __restrict__ draws,
-                                        uint32_t n_draws, const DevView* __restrict__ views,
-                                        const uint2* __restrict__ surv, const DevMaterial* __restrict__ mats,
-                                        SetupRec* __restrict__ setup, ShadeRec* __restrict__ shade, uint32_t cap_tri,
-                                        Status* __restrict__ status, int TS, uint32_t* __restrict__ bincnt,
-                                        uint8_t* __restrict__ wflag, uint32_t* __restrict__ dkey) {
+__device__ __forceinline__ void geom_emit_simple(uint32_t j, bool active, uint4* stage, const DevDraw* __restrict__ draws,
+                                                 uint32_t n_draws, const DevView* __restrict__ views,
+                                                 const uint2* __restrict__ surv, const DevMaterial* __restrict__ mats,
+                                                 SetupRec* __restrict__ setup, ShadeRec* __restrict__ shade, uint32_t cap_tri,
+                                                 Status* __restrict__ status, int TS, uint32_t* __restrict__ bincnt,
+                                                 uint8_t* __restrict__ wflag, uint32_t* __restrict__ dkey) {
   uint4 ro[6], so[9];
   uint32_t idx = 0;
   bool ok = false, skip = false;
@@ -954,6 +1051,41 @@ __global__ __launch_bounds__(256) void k_geom_emit(const DevDraw* __restrict__ d
     const uint32_t n = status->n_simple;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     for (uint32_t j0 = blockIdx.x * 256u + wave * 64u; j0 < n; j0 += gridDim.x * 256u)
+      geom_emit_simple<BIN>(j0 + lane, j0 + lane < n, s_stage[wave], draws, n_draws, views, surv, mats, setup, shade,
+                            cap_tri, status, TS, bincnt, wflag, dkey);
+  }
+}
+
+// Both emitter lists of a small frame in one launch on the stream (for a few
+// thousand triangles the fork to a second stream and the join back cost more
+// than the emits): of its g_clip + g_simple blocks, g_clip run the clipped
+// list and the others the unclipped one, interleaved in proportion (block b
+// is a clipped block when floor((b + 1) g_clip / G) > floor(b g_clip / G)).
+// No occupancy floor: the grid (at most 128 blocks) fills a fraction of the
+// chip, so registers go to the clipped path (167-172 VGPRs, no spills)
+// rather than to waves that would have nothing to run.
+template <bool BIN>
+__global__ __launch_bounds__(256) void k_geom_emit_both(const DevDraw* __restrict__ draws, uint32_t n_draws,
+                                                        const DevView* __restrict__ views,
+                                                        const uint2* __restrict__ surv,
+                                                        const DevMaterial* __restrict__ mats,
+                                                        SetupRec* __restrict__ setup, ShadeRec* __restrict__ shade,
+                                                        uint32_t cap_tri, Status* __restrict__ status, int TS,
+                                                        uint32_t* __restrict__ bincnt, uint8_t* __restrict__ wflag,
+                                                        uint32_t* __restrict__ dkey, uint32_t g_clip) {
+  __shared__ uint4 s_stage[4][64 * 9];
+  const uint64_t G = gridDim.x, b = blockIdx.x;
+  const uint32_t c0 = (uint32_t)(b * g_clip / G), c1 = (uint32_t)((b + 1) * g_clip / G);
+  if (c1 > c0) {
+    const uint32_t n = status->n_clip;
+    for (uint32_t j = c0 * 256u + threadIdx.x; j < n; j += g_clip * 256u)
+      geom_emit_clipped<BIN>(j, draws, n_draws, views, surv, mats, setup, shade, cap_tri, status, TS, bincnt, wflag,
+                             dkey);
+  } else {
+    const uint32_t ub = (uint32_t)b - c0, g_simple = (uint32_t)G - g_clip;
+    const uint32_t n = status->n_simple;
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    for (uint32_t j0 = ub * 256u + wave * 64u; j0 < n; j0 += g_simple * 256u)
       geom_emit_simple<BIN>(j0 + lane, j0 + lane < n, s_stage[wave], draws, n_draws, views, surv, mats, setup, shade,
                             cap_tri, status, TS, bincnt, wflag, dkey);
   }
@@ -1369,8 +1501,6 @@ __global__ __launch_bounds__(256) void k_bin_emit_wide(const SetupRec* __restric
 // minimum of its fragments' keys whatever the order (the transparent pass,
 // which blends in list order, never comes here).  Pairs past cap_pairs are
 // counted, flagged (overflow bit 2) and not written; ranges end at cap_pairs.
-constexpr uint32_t kSmallBinTiles = 4096;  // LDS counters and cursors: 32 KB
-constexpr uint32_t kSmallBinTris = 16384;  // input triangles of a frame binned this way
 
 __global__ __launch_bounds__(1024) void k_bin_small(const SetupRec* __restrict__ setup,
                                                     const DevView* __restrict__ views, Status* __restrict__ status,
@@ -1823,8 +1953,25 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
                      dim3(256), 0, s, reinterpret_cast<uint32_t*>(a.status), (uint32_t)(sizeof(Status) / 4),
                      a.tables_src, a.tables_dst, a.tables_src ? a.tables_words : 0u);
   const uint32_t n_blk = cdiv(a.n_tri_in, 256);
+  // a small single-view frame: one workgroup builds the emitter lists, and
+  // both emits share one launch on this stream
+  const bool small_geom = a.n_views == 1 && a.n_tri_in <= kSmallBinTris;
   // 1-3 geometry with ordered compaction
-  if (n_blk) {
+  if (n_blk && small_geom) {
+    hipLaunchKernelGGL(k_geom_small, dim3(1), dim3(1024), 0, s, a.draws, a.n_draws, a.views, a.n_tri_in, a.cnt, a.blk,
+                       a.status, a.cap_tri, a.surv);
+    uint8_t* wf = reinterpret_cast<uint8_t*>(a.big);
+    uint32_t* dk = depth_sorted && !small_bins ? a.keys[1] + a.cap_tri : nullptr;
+    const uint32_t g_simple = n_blk, g_clip = n_blk;
+    if (a.ordered_bins)
+      hipLaunchKernelGGL(k_geom_emit_both<true>, dim3(g_clip + g_simple), dim3(256), 0, s, a.draws, a.n_draws,
+                         a.views, a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, a.bincnt, wf, dk,
+                         g_clip);
+    else
+      hipLaunchKernelGGL(k_geom_emit_both<false>, dim3(g_clip + g_simple), dim3(256), 0, s, a.draws, a.n_draws,
+                         a.views, a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, nullptr, nullptr,
+                         nullptr, g_clip);
+  } else if (n_blk) {
     uint32_t* blk_simple = a.blk + n_blk + 1;
     uint32_t* blk_clip = a.blk + 2 * (n_blk + 1);
     hipLaunchKernelGGL(k_geom_count, dim3(n_blk), dim3(256), 0, s, a.draws, a.n_draws, a.views, a.n_tri_in, a.cnt,
