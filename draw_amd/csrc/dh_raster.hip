@@ -11,10 +11,6 @@
 
 namespace dh {
 
-// Workgroups of `bytes` of LDS that fit the 160 KB of a CU, allocated in
-// 1 KB units.
-constexpr int lds_workgroups(int bytes) { return (160 * 1024) / (((bytes + 1023) / 1024) * 1024); }
-
 // Coverage of one pixel for a FAST triangle, decided without division.  Edge
 // values are integers here (snapped vertices are floor() results and pixel
 // centres are integers), so for a finite nonzero normalizer f the quotient
@@ -210,9 +206,11 @@ __device__ inline void shade_wave(uint32_t id, bool active, uint32_t x, uint32_t
 // (depth_key(d) << 32 | id) to an LDS atomicMin, which leaves at each pixel the
 // first fragment of minimum depth -- the winner of the reference's sequential
 // strict `<` test.  Each wave owns QPW 8x8 quadrants of the tile:
-//  1. records are staged in LDS in chunks of CH;
-//  2. lanes pre-filter 64 records at once against the wave's rectangle and its
-//     largest recorded depth (hierarchical Z);
+//  1. records are staged in LDS in chunks of CH, each with the mask of the
+//     tile's quadrants it can cover (its scan rectangle and, for an exact
+//     record, exact per-quadrant edge tests), computed once for all waves;
+//  2. lanes pre-filter 64 records at once: the wave's bits of that mask and
+//     its largest recorded depth (hierarchical Z);
 //  3. the wave visits the hits in order; lanes test coverage of their own
 //     pixel without division (covered_fast) and against their pixel's recorded
 //     depth, and append covered fragments to a per-wave LDS queue;
@@ -232,30 +230,30 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
   // fragment queue per wave: drained once it holds more than kQFlush entries,
   // checked before each triangle, which appends at most 64 per quadrant
   constexpr int kQFlush = 256, QCAP = kQFlush + 64 * QPW;
-  // the staged chunk: records, headers, pre-filter constants, ids (one LDS
-  // area, which the shading epilogue reuses for its record runs)
-  constexpr int kChunkU4 = CH * (sizeof(SetupRec) + 16 + 48 + 4) / 16;
+  // the staged chunk: records, headers, edge constants, quadrant masks, ids
+  // (one LDS area, which the shading epilogue reuses for its record runs)
+  constexpr int kChunkU4 = CH * (sizeof(SetupRec) + 16 + 48 + 4 + 4) / 16;
   __shared__ uint4 s_chunk[kChunkU4];
   SetupRec* const s_rec = reinterpret_cast<SetupRec*>(s_chunk);
   uint4* const s_hdr = s_chunk + CH * sizeof(SetupRec) / 16;  // scan rectangle x, y; flags; depth-bound key
-  // per edge: s*A, s*B, s*(C-D), |C|+|D| with s = sign(f)
+  // per edge of an exact record: s*A, s*B and E, the edge's exact value
+  // s*e - t at the tile origin (s = sign(f); t = 0 on a tie edge, else 1)
   float4 (*const s_pre)[3] = reinterpret_cast<float4 (*)[3]>(s_hdr + CH);
   uint32_t* const s_id = reinterpret_cast<uint32_t*>(s_hdr + CH + 3 * CH);
+  // the tile's quadrants a record can cover (bit Q: quadrant row Q / QS,
+  // column Q % QS), | kQExact for an exact record
+  uint32_t* const s_qm = s_id + CH;
   __shared__ unsigned long long s_vis[TS * TS];
   __shared__ uint32_t s_q[NW][QCAP];
   __shared__ uint32_t s_nfrag;  // the tile's fragment count (one global atomic per tile)
+  __shared__ uint32_t s_wmax[NW];  // each wave's largest recorded depth key, as of its last flush
   __shared__ float s_unit[SHADE ? 256 : 1];
-  // How an exact hit's edge constants reach the wave.  kLdsHit: s*A and s*B
-  // from the staged s_pre, the wave's bases E from s_eh, where the
-  // pre-filtering lane wrote them -- broadcast LDS reads, no VALU work.
-  // Otherwise readlanes from the pre-filtering lane.  The LDS path is taken
-  // with several quadrants per wave (with one, a hit has too little work to
-  // cover the LDS latency) and only when s_eh costs no resident workgroup
-  // (160 KB of LDS per CU, counted in 1 KB allocation units).
-  constexpr int kLdsBase = (int)(sizeof(s_chunk) + sizeof(s_vis) + sizeof(s_q) + sizeof(s_nfrag) + sizeof(s_unit));
-  constexpr int kLdsEh = NW * 64 * 16;
-  constexpr bool kLdsHit = QPW >= 2 && lds_workgroups(kLdsBase + kLdsEh) == lds_workgroups(kLdsBase);
-  __shared__ float4 s_eh[kLdsHit ? NW : 1][64];
+  // How an exact hit's edge constants reach the wave.  kLdsHit: broadcast
+  // reads of the staged s_pre.  Otherwise the pre-filtering lane reads them
+  // and the wave takes them with readlanes: with one quadrant per wave a hit
+  // has too little work to cover the LDS latency.
+  constexpr bool kLdsHit = QPW >= 2;
+  constexpr uint32_t kQExact = 1u << 16;
 
   const uint32_t tile = xcd_remap(blockIdx.x, gridDim.x);
   const DevView& vw = views[wave_find_view_tile(views, n_views, tile)];
@@ -274,26 +272,25 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
   for (int p = tid; p < TS * TS; p += NT)
     s_vis[p] = (tx0 + p % TS < W && ty0 + p / TS < H) ? ~0ull : 0ull;
   if (tid == 0) s_nfrag = 0;
+  if (tid < NW) s_wmax[tid] = 0xFFFFFFFFu;
   if constexpr (SHADE) fill_unit_table(s_unit, tid, NT);  // read after the barrier before the shading
 
   const int lx = lane & 7, ly = lane >> 3;
   const uint64_t lanemask_lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  // The wave's quadrants form one rectangle: part of a quadrant row
-  // (QPW <= QS) or whole quadrant rows (QPW a multiple of QS).
+  // The wave's quadrants: QPW consecutive ones in row-major order (part of a
+  // quadrant row, or whole rows), bits wq0 .. wq0 + QPW - 1 of a record's
+  // quadrant mask.
   const int wq0 = wave * QPW;
-  const int wx0 = tx0 + (wq0 % QS) * 8, wy0 = ty0 + (wq0 / QS) * 8;
-  const int wx1 = wx0 + (QPW < QS ? QPW : QS) * 8 - 1;
-  const int wy1 = wy0 + (QPW < QS ? 1 : QPW / QS) * 8 - 1;
   int lp[QPW];        // local index of the lane's pixel in each of its quadrants
   uint32_t zk[QPW];   // recorded depth key at that pixel (0xFFFFFFFF: none yet; 0: off the screen)
-  float dxf[QPW], dyf[QPW];  // the pixel's offset from the wave rectangle's origin
+  float dxf[QPW], dyf[QPW];  // the pixel's offset from the tile's origin
 #pragma unroll
   for (int qq = 0; qq < QPW; qq++) {
     const int Q = wq0 + qq;
     lp[qq] = ((Q / QS) * 8 + ly) * TS + (Q % QS) * 8 + lx;
     const int px = tx0 + (Q % QS) * 8 + lx, py = ty0 + (Q / QS) * 8 + ly;
-    dxf[qq] = (float)(px - wx0);
-    dyf[qq] = (float)(py - wy0);
+    dxf[qq] = (float)(px - tx0);
+    dyf[qq] = (float)(py - ty0);
     zk[qq] = (px < W && py < H) ? 0xFFFFFFFFu : 0u;
   }
   uint32_t wmax = 0xFFFFFFFFu;  // largest recorded depth key over the wave's pixels
@@ -326,57 +323,174 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
       m = max(m, zk[qq]);
     }
     wmax = wave_max_u32(m);
+    s_wmax[wave] = wmax;  // read at the next staging, after a barrier
   };
 
-  // A record's 96 bytes as six uint4 (named values, not an array, so that
-  // a record held across the raster loop stays in registers).
-  auto load_rec = [&](uint32_t id, uint4& w0, uint4& w1, uint4& w2, uint4& w3, uint4& w4, uint4& w5) {
+  // Staging: P consecutive lanes of one wave per record of a chunk (P = NT /
+  // CH, at least 1).  Lane part sp loads the record's 16-byte words sp,
+  // sp + P, ... and stores them into LDS; the P lanes read back the words
+  // they need (stores of their own wave: no barrier) and each tests the
+  // quadrant rows sp, sp + P, ... of the tile; an OR over the P lanes joins
+  // the rows.  With NT >= CH the loads run ahead of the raster: while a chunk
+  // is rasterized, the next chunk's words (their ids loaded a chunk earlier)
+  // and the ids of the one after are in flight in registers.
+  constexpr int P = NT >= CH ? NT / CH : 1;
+  constexpr int NPC = (6 + P - 1) / P;  // words per lane
+  constexpr bool kPrefetch = NT >= CH;
+  const int sr = NT >= CH ? tid / P : tid, sp = NT >= CH ? tid % P : 0;
+  // This lane's words of a record: q_j = word sp + j P for j < NPC (named
+  // values, not an array, which would live in scratch memory).  A lane with
+  // fewer than NPC words loads word 5 again.
+  auto load_words = [&](uint32_t id, uint4& q0, uint4& q1, uint4& q2, uint4& q3, uint4& q4, uint4& q5) {
     const uint4* src = reinterpret_cast<const uint4*>(setup + id);
-    w0 = src[0]; w1 = src[1]; w2 = src[2]; w3 = src[3]; w4 = src[4]; w5 = src[5];
+    q0 = src[min(sp, 5)];
+    if constexpr (NPC > 1) q1 = src[min(sp + P, 5)];
+    if constexpr (NPC > 2) q2 = src[min(sp + 2 * P, 5)];
+    if constexpr (NPC > 3) q3 = src[min(sp + 3 * P, 5)];
+    if constexpr (NPC > 4) q4 = src[min(sp + 4 * P, 5)];
+    if constexpr (NPC > 5) q5 = src[min(sp + 5 * P, 5)];
   };
-  // Record i of a chunk into LDS.
-  auto stage = [&](int i, uint32_t id, uint4 w0, uint4 w1, uint4 w2, uint4 w3, uint4 w4, uint4 w5) {
-    s_id[i] = id;
+  // OR of v over the P lanes of a record (all of them active)
+  auto or_parts = [](uint32_t v) {
+    if constexpr (P >= 2) v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);  // lane ^ 1
+    if constexpr (P >= 4) v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);  // lane ^ 2
+    if constexpr (P >= 8) v |= (uint32_t)__shfl_xor((int)v, 4, 64);
+    return v;
+  };
+  uint32_t tile_wmax = 0xFFFFFFFFu;  // the tile's largest recorded depth key (s_wmax), for stage
+  // Record i of a chunk into LDS, from this lane's words.
+  auto stage = [&](int i, uint32_t id, uint4 q0, uint4 q1, uint4 q2, uint4 q3, uint4 q4, uint4 q5) {
     uint4* dst = reinterpret_cast<uint4*>(s_rec + i);
-    dst[0] = w0; dst[1] = w1; dst[2] = w2; dst[3] = w3; dst[4] = w4; dst[5] = w5;
+    if (sp < 6) dst[sp] = q0;
+    if constexpr (NPC > 1) if (sp + P < 6) dst[sp + P] = q1;
+    if constexpr (NPC > 2) if (sp + 2 * P < 6) dst[sp + 2 * P] = q2;
+    if constexpr (NPC > 3) if (sp + 3 * P < 6) dst[sp + 3 * P] = q3;
+    if constexpr (NPC > 4) if (sp + 4 * P < 6) dst[sp + 4 * P] = q4;
+    if constexpr (NPC > 5) if (sp + 5 * P < 6) dst[sp + 5 * P] = q5;
+    uint4 w0, w1, w2, w3, w5;
+    uint32_t zb;  // the depth lower bound's bits
+    if constexpr (P == 1) {
+      w5 = q5;
+      zb = q4.w;
+    } else {
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // the other parts' stores
+      w5 = dst[5];
+      zb = __float_as_uint(s_rec[i].d[3]);
+    }
     // depth bound key, at least 1: only off-screen pixels hold key 0 (a
     // fragment key of 0 is a NaN depth, which never passes the depth test)
-    s_hdr[i] = make_uint4(w5.x, w5.y, w5.z, max(depth_key(__uint_as_float(w4.w)), 1u));
-    auto pre = [&](int e, uint4 edge, uint32_t fbits) {
-      const float sg = __uint_as_float(fbits) > 0.0f ? 1.0f : -1.0f;
-      const float A = __uint_as_float(edge.x), B = __uint_as_float(edge.y);
-      const float C = __uint_as_float(edge.z), D = __uint_as_float(edge.w);
-      s_pre[i][e] = make_float4(sg * A, sg * B, sg * (C - D), fabsf(C) + fabsf(D));
-    };
-    pre(0, w0, w3.x);
-    pre(1, w1, w3.y);
-    pre(2, w2, w3.z);
-  };
-  // With one thread per staged record (CH <= NT <= 256) the loads run ahead
-  // of the raster: while a chunk is rasterized, the next chunk's records
-  // (their ids loaded a chunk earlier) and the ids of the one after are in
-  // flight in registers.
-  constexpr bool kPrefetch = NT >= CH && NT <= 256;
-  uint32_t pf_id = 0, pf_id2 = 0;  // ids of the next chunk and of the one after it
-  uint4 p0{}, p1{}, p2{}, p3{}, p4{}, p5{};  // the next chunk's record
-  if constexpr (kPrefetch) {
-    if (tid < CH && rg.x + (uint32_t)tid < rg.y) {
-      pf_id = vals[rg.x + tid];
-      load_rec(pf_id, p0, p1, p2, p3, p4, p5);
+    const uint32_t zkey = max(depth_key(__uint_as_float(zb)), 1u);
+    if (sp == 0) {
+      s_id[i] = id;
+      s_hdr[i] = make_uint4(w5.x, w5.y, w5.z, zkey);
     }
-    if (tid < CH && rg.x + CH + (uint32_t)tid < rg.y) pf_id2 = vals[rg.x + CH + tid];
+    // A record whose depth bound is above every wave's largest recorded key
+    // is hidden on the whole tile (each wave's pre-filter would drop it): no
+    // quadrants, and no edge work.
+    if (zkey > tile_wmax) {
+      if (sp == 0) s_qm[i] = 0u;
+      return;
+    }
+    if constexpr (P == 1) {
+      w0 = q0; w1 = q1; w2 = q2; w3 = q3;
+    } else {
+      w0 = dst[0]; w1 = dst[1]; w2 = dst[2]; w3 = dst[3];
+    }
+    // The tile's quadrants the scan rectangle meets: quadrant column c meets
+    // [x0, x1] iff floor((x0 - tx0) / 8) <= c <= floor((x1 - tx0) / 8), rows
+    // likewise.
+    const uint32_t flags = w5.z;
+    const int c0 = max(((int)(w5.x & 0xFFFFu) - tx0) >> 3, 0), c1 = min(((int)(w5.x >> 16) - tx0) >> 3, QS - 1);
+    const int r0 = max(((int)(w5.y & 0xFFFFu) - ty0) >> 3, 0), r1 = min(((int)(w5.y >> 16) - ty0) >> 3, QS - 1);
+    uint32_t qmask = 0;
+    if (c0 <= c1 && r0 <= r1) {
+      const uint32_t cols = (2u << c1) - (1u << c0);
+#pragma unroll
+      for (int rr = 0; rr < QS; rr++)
+        if (rr >= r0 && rr <= r1) qmask |= cols << (rr * QS);
+    }
+    // Exact records.  With S = |A| x1 + |B| y1 + |C| + |D| over the tile's
+    // largest pixel coordinates (x1, y1) below 2^24 - 216 on every edge (the
+    // float S is within 3 units of the exact one), every product and partial
+    // sum of the edge function at the tile's pixels is an integer of
+    // magnitude <= S, so s*e(p) - t = E + sA*(px - tx0) + sB*(py - ty0) is
+    // exact in two fma steps, E = s*e(tx0, ty0) - t, and covered_fast on the
+    // edge <=> s*e >= t <=> that value >= 0.  Pixels outside the triangle's
+    // bounding box need no separate test: a covered pixel is inside the
+    // snapped triangle, hence inside its bounding box unless it is off the
+    // screen.
+    bool exact = (flags & kFast) != 0u;
+    float sA[3], sB[3], E[3];
+    const uint4 edge[3] = {w0, w1, w2};
+    const uint32_t fb[3] = {w3.x, w3.y, w3.z};
+    const float tx0f = (float)tx0, ty0f = (float)ty0, tx1f = (float)(tx0 + TS - 1), ty1f = (float)(ty0 + TS - 1);
+#pragma unroll
+    for (int e = 0; e < 3; e++) {
+      const float sg = __uint_as_float(fb[e]) > 0.0f ? 1.0f : -1.0f;
+      const float A = __uint_as_float(edge[e].x), B = __uint_as_float(edge[e].y);
+      const float C = __uint_as_float(edge[e].z), D = __uint_as_float(edge[e].w);
+      sA[e] = sg * A;
+      sB[e] = sg * B;
+      const float S = (fabsf(sA[e]) * tx1f + fabsf(sB[e]) * ty1f) + (fabsf(C) + fabsf(D));
+      if (!(S < 16777000.0f)) exact = false;
+      E[e] = ((sA[e] * tx0f + sB[e] * ty0f) + sg * (C - D)) - ((flags & (kTieA << e)) ? 0.0f : 1.0f);
+      if (e % P == sp) s_pre[i][e] = make_float4(sA[e], sB[e], E[e], 0.0f);
+    }
+    if (exact) {  // the same on the record's P lanes
+      // ...less the quadrants an edge rules out: an edge's largest value over
+      // quadrant (qx, qy) is at its pixel (8 qx + (sA > 0 ? 7 : 0), 8 qy +
+      // (sB > 0 ? 7 : 0)) of the tile, computed exactly (a pixel of the
+      // tile) as cv[qx] + sB * 8 qy; below zero there, no pixel of the
+      // quadrant is covered.
+      float cv[3][QS];
+#pragma unroll
+      for (int e = 0; e < 3; e++) {
+        cv[e][0] = fmaf(sA[e], sA[e] > 0.0f ? 7.0f : 0.0f, fmaf(sB[e], sB[e] > 0.0f ? 7.0f : 0.0f, E[e]));
+#pragma unroll
+        for (int k = 1; k < QS; k++) cv[e][k] = fmaf(sA[e], (float)(8 * k), cv[e][0]);
+      }
+      uint32_t keep = 0;
+#pragma unroll
+      for (int j = 0; j < (QS >= P ? QS / P : 1); j++) {
+        const int qy = sp + j * P;
+        if (QS >= P || qy < QS) {
+          float rv[3];
+#pragma unroll
+          for (int e = 0; e < 3; e++) rv[e] = sB[e] * (float)(8 * qy);
+#pragma unroll
+          for (int qx = 0; qx < QS; qx++) {
+            const float v = fminf(fminf(cv[0][qx] + rv[0], cv[1][qx] + rv[1]), cv[2][qx] + rv[2]);
+            keep |= (v >= 0.0f ? 1u : 0u) << (qy * QS + qx);
+          }
+        }
+      }
+      qmask = (qmask & or_parts(keep)) | kQExact;
+    }
+    if (sp == 0) s_qm[i] = qmask;
+  };
+  uint32_t pf_id = 0, pf_id2 = 0;  // ids of the next chunk and of the one after it
+  uint4 p0{}, p1{}, p2{}, p3{}, p4{}, p5{};  // this lane's words of the next chunk's record
+  if constexpr (kPrefetch) {
+    if (rg.x + (uint32_t)sr < rg.y) {
+      pf_id = vals[rg.x + sr];
+      load_words(pf_id, p0, p1, p2, p3, p4, p5);
+    }
+    if (rg.x + CH + (uint32_t)sr < rg.y) pf_id2 = vals[rg.x + CH + sr];
   }
 
   for (uint32_t base = rg.x; base < rg.y; base += CH) {
     const int n = (int)min((uint32_t)CH, rg.y - base);
     __syncthreads();
+    tile_wmax = 0u;
+#pragma unroll
+    for (int w = 0; w < NW; w++) tile_wmax = max(tile_wmax, s_wmax[w]);
     if constexpr (kPrefetch) {
-      if (tid < n) stage(tid, pf_id, p0, p1, p2, p3, p4, p5);
+      if (sr < n) stage(sr, pf_id, p0, p1, p2, p3, p4, p5);
     } else {
       for (int i = tid; i < n; i += NT) {
         const uint32_t id = vals[base + i];
         uint4 w0, w1, w2, w3, w4, w5;
-        load_rec(id, w0, w1, w2, w3, w4, w5);
+        load_words(id, w0, w1, w2, w3, w4, w5);
         stage(i, id, w0, w1, w2, w3, w4, w5);
       }
     }
@@ -385,90 +499,32 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
       // the next chunk's records, the ids of the chunk after it
       const uint32_t b1 = base + CH, b2 = base + 2u * CH;
       pf_id = pf_id2;
-      if (tid < CH && b1 + (uint32_t)tid < rg.y) load_rec(pf_id, p0, p1, p2, p3, p4, p5);
+      if (b1 + (uint32_t)sr < rg.y) load_words(pf_id, p0, p1, p2, p3, p4, p5);
       pf_id2 = 0;
-      if (tid < CH && b2 + (uint32_t)tid < rg.y) pf_id2 = vals[b2 + tid];
+      if (b2 + (uint32_t)sr < rg.y) pf_id2 = vals[b2 + sr];
     }
     for (int g0 = 0; g0 < n; g0 += 64) {
       bool hit = false, wexact = false;
       float pA[3] = {0.0f, 0.0f, 0.0f}, pB[3] = {0.0f, 0.0f, 0.0f}, pE[3] = {0.0f, 0.0f, 0.0f};
-      uint32_t qm = 0;  // exact hits: the wave's quadrants the record's rectangle meets
+      uint32_t qm = 0;  // the wave's quadrants the record can cover
       uint4 h = make_uint4(0u, 0u, 0u, 0u);
       if (g0 + lane < n) {
         h = s_hdr[g0 + lane];
-        hit = !((int)(h.x >> 16) < wx0 || (int)(h.x & 0xFFFFu) > wx1 || (int)(h.y >> 16) < wy0 ||
-                (int)(h.y & 0xFFFFu) > wy1) &&
-              !(h.w > wmax);
-        if (hit && (h.z & kFast)) {
-          // Conservative edge test over the wave's rectangle: a pixel can be
-          // covered only if s*e >= 0 on every edge (s = sign of its
-          // normalizer).  With val the float value of s*(A x* + B y* + C - D)
-          // at the corner maximising it and S = |A| xmax + |B| ymax + |C| + |D|,
-          // every pixel p of the rectangle has s*e(p) <= val + 8.01 u S
-          // (u = 2^-24), so val + 2^-20 S + 1 < 0 rules the edge out.
-          //
-          // When S < 2^24 - 216 on every edge (the float S is within 3 units
-          // of the exact one), every product and partial sum of the edge
-          // function over the rectangle is an integer of magnitude <= S, so
-          // s*e(p) - t = E + sA*(px - wx0) + sB*(py - wy0) is exact in two
-          // fma steps, E = s*e(wx0, wy0) - t with t = 0 for a tie edge and
-          // 1 otherwise.  covered_fast on the edge <=> s*e >= t <=> that
-          // value >= 0.  Pixels outside the triangle's bounding box need no
-          // separate test: a covered pixel is inside the snapped triangle,
-          // hence inside its bounding box unless it is off the screen.
-          wexact = true;
+        const uint32_t m = s_qm[g0 + lane];
+        qm = (m >> wq0) & ((1u << QPW) - 1u);
+        hit = qm != 0u && !(h.w > wmax);
+        wexact = (m & kQExact) != 0u;
+        if constexpr (!kLdsHit) {
+          if (hit && wexact) {
 #pragma unroll
-          for (int e = 0; e < 3; e++) {
-            const float4 q = s_pre[g0 + lane][e];
-            const float xs = q.x > 0.0f ? (float)wx1 : (float)wx0;
-            const float ys = q.y > 0.0f ? (float)wy1 : (float)wy0;
-            const float val = (q.x * xs + q.y * ys) + q.z;
-            const float S = (fabsf(q.x) * (float)wx1 + fabsf(q.y) * (float)wy1) + q.w;
-            if (val + (9.5367431640625e-07f * S + 1.0f) < 0.0f) hit = false;
-            if (!(S < 16777000.0f)) wexact = false;
-            pA[e] = q.x;
-            pB[e] = q.y;
-            pE[e] = ((q.x * (float)wx0 + q.y * (float)wy0) + q.z) - ((h.z & (kTieA << e)) ? 0.0f : 1.0f);
-          }
-          // the wave's quadrants the scan rectangle meets, one bit each:
-          // quadrant column c meets [x0, x1] iff floor((x0 - tx0) / 8) <= c <=
-          // floor((x1 - tx0) / 8), rows likewise (the rectangle meets the
-          // wave's, so the clamped ranges are not empty)
-          const int c0 = max(((int)(h.x & 0xFFFFu) - tx0) >> 3, 0), c1 = min(((int)(h.x >> 16) - tx0) >> 3, QS - 1);
-          const int r0 = max(((int)(h.y & 0xFFFFu) - ty0) >> 3, 0), r1 = min(((int)(h.y >> 16) - ty0) >> 3, QS - 1);
-          const uint32_t cols = (2u << c1) - (1u << c0), rows = (2u << r1) - (1u << r0);
-          if constexpr (QPW <= QS) {
-            // part of one quadrant row: columns wq0 % QS .. + QPW - 1, row wq0 / QS
-            qm = ((cols >> (wq0 % QS)) & ((1u << QPW) - 1u)) & (0u - ((rows >> (wq0 / QS)) & 1u));
-          } else {
-            // whole quadrant rows wq0 / QS .. + QPW / QS - 1
-#pragma unroll
-            for (int rr = 0; rr < QPW / QS; rr++)
-              if ((rows >> (wq0 / QS + rr)) & 1u) qm |= (cols & ((1u << QS) - 1u)) << (rr * QS);
-          }
-          // ...less the quadrants an edge rules out.  A visit computes
-          // E + A dx + B dy at the quadrant's pixels, exactly (integers below
-          // S); its largest value is at the corner dx = ox + (A > 0 ? 7 : 0),
-          // dy = oy + (B > 0 ? 7 : 0), computed here by the same exact fmas.
-          // Below zero there, no pixel of the quadrant is covered.
-#pragma unroll
-          for (int e = 0; e < 3; e++) {
-            const float c00 = fmaf(pA[e], pA[e] > 0.0f ? 7.0f : 0.0f, fmaf(pB[e], pB[e] > 0.0f ? 7.0f : 0.0f, pE[e]));
-#pragma unroll
-            for (int qq = 0; qq < QPW; qq++) {
-              constexpr int kRowQ = QPW < QS ? QPW : QS;  // quadrants per row of the wave's rectangle
-              const float ox = (float)((qq % kRowQ) * 8), oy = (float)((qq / kRowQ) * 8);
-              const float cmax = fmaf(pA[e], ox, fmaf(pB[e], oy, c00));
-              if (cmax < 0.0f) qm &= ~(1u << qq);
+            for (int e = 0; e < 3; e++) {
+              const float4 q = s_pre[g0 + lane][e];
+              pA[e] = q.x;
+              pB[e] = q.y;
+              pE[e] = q.z;
             }
           }
-          // an exact record that covers none of the wave's quadrants is no hit
-          if (wexact && qm == 0u) hit = false;
         }
-      }
-      if constexpr (kLdsHit) {
-        if (hit && wexact) s_eh[wave][lane] = make_float4(pE[0], pE[1], pE[2], 0.0f);
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");  // s_eh writes by other lanes of this wave
       }
       // Exact two-FMA hits in a tight loop, then the rest (the visibility
       // minimum does not depend on the order of the visits).
@@ -485,15 +541,12 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
         const uint32_t zkey = (uint32_t)__builtin_amdgcn_readlane((int)h.w, kl);
         float A[3], B[3], E[3];
         if constexpr (kLdsHit) {
-          const float4 eh = s_eh[wave][kl];
-          E[0] = eh.x;
-          E[1] = eh.y;
-          E[2] = eh.z;
 #pragma unroll
           for (int e = 0; e < 3; e++) {
             const float4 q = s_pre[k][e];
             A[e] = q.x;
             B[e] = q.y;
+            E[e] = q.z;
           }
         } else {
 #pragma unroll
