@@ -229,17 +229,20 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
   constexpr int CH = 128;    // records staged per round
   // fragment queue per wave: drained once it holds more than kQFlush entries,
   // checked before each triangle, which appends at most 64 per quadrant
-  constexpr int kQFlush = 256, QCAP = kQFlush + 64 * QPW;
-  // the staged chunk: records, headers, edge constants, quadrant masks, ids
-  // (one LDS area, which the shading epilogue reuses for its record runs)
-  constexpr int kChunkU4 = CH * (sizeof(SetupRec) + 16 + 48 + 4 + 4) / 16;
+  constexpr int kQFlush = 224, QCAP = kQFlush + 64 * QPW;
+  // the staged chunk: records, edge constants, depth-bound keys, ids,
+  // quadrant masks (one LDS area, which the shading epilogue reuses for its
+  // record runs).  A workgroup's LDS must fit in one of the CU's two 80 KB
+  // halves, so <32, 2> (8 waves, 80 VGPRs) keeps below 40 KB to fit two per
+  // half and run three per CU.
+  constexpr int kChunkU4 = CH * (sizeof(SetupRec) + 48 + 4 + 4 + 4) / 16;
   __shared__ uint4 s_chunk[kChunkU4];
   SetupRec* const s_rec = reinterpret_cast<SetupRec*>(s_chunk);
-  uint4* const s_hdr = s_chunk + CH * sizeof(SetupRec) / 16;  // scan rectangle x, y; flags; depth-bound key
   // per edge of an exact record: s*A, s*B and E, the edge's exact value
   // s*e - t at the tile origin (s = sign(f); t = 0 on a tie edge, else 1)
-  float4 (*const s_pre)[3] = reinterpret_cast<float4 (*)[3]>(s_hdr + CH);
-  uint32_t* const s_id = reinterpret_cast<uint32_t*>(s_hdr + CH + 3 * CH);
+  float4 (*const s_pre)[3] = reinterpret_cast<float4 (*)[3]>(s_chunk + CH * sizeof(SetupRec) / 16);
+  uint32_t* const s_zk = reinterpret_cast<uint32_t*>(s_chunk + CH * (sizeof(SetupRec) + 48) / 16);  // depth-bound key
+  uint32_t* const s_id = s_zk + CH;
   // the tile's quadrants a record can cover (bit Q: quadrant row Q / QS,
   // column Q % QS), | kQExact for an exact record
   uint32_t* const s_qm = s_id + CH;
@@ -310,7 +313,7 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
     for (uint32_t j = (uint32_t)lane; j < qn; j += 64) {
       const uint32_t q = s_q[wave][j];
       const uint32_t k = q >> 16, l = q & 0xFFFFu;
-      if (s_hdr[k].w > (uint32_t)(s_vis[l] >> 32)) continue;  // early-Z recheck
+      if (s_zk[k] > (uint32_t)(s_vis[l] >> 32)) continue;  // early-Z recheck
       submit(s_rec[k], s_id[k], tx0 + (int)(l % TS), ty0 + (int)(l / TS));
     }
     nfrag += qn;
@@ -382,7 +385,7 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
     const uint32_t zkey = max(depth_key(__uint_as_float(zb)), 1u);
     if (sp == 0) {
       s_id[i] = id;
-      s_hdr[i] = make_uint4(w5.x, w5.y, w5.z, zkey);
+      s_zk[i] = zkey;
     }
     // A record whose depth bound is above every wave's largest recorded key
     // is hidden on the whole tile (each wave's pre-filter would drop it): no
@@ -507,12 +510,12 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
       bool hit = false, wexact = false;
       float pA[3] = {0.0f, 0.0f, 0.0f}, pB[3] = {0.0f, 0.0f, 0.0f}, pE[3] = {0.0f, 0.0f, 0.0f};
       uint32_t qm = 0;  // the wave's quadrants the record can cover
-      uint4 h = make_uint4(0u, 0u, 0u, 0u);
+      uint32_t zb = 0;  // the record's depth-bound key
       if (g0 + lane < n) {
-        h = s_hdr[g0 + lane];
+        zb = s_zk[g0 + lane];
         const uint32_t m = s_qm[g0 + lane];
         qm = (m >> wq0) & ((1u << QPW) - 1u);
-        hit = qm != 0u && !(h.w > wmax);
+        hit = qm != 0u && !(zb > wmax);
         wexact = (m & kQExact) != 0u;
         if constexpr (!kLdsHit) {
           if (hit && wexact) {
@@ -538,7 +541,7 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
         // the record's quadrant mask and depth bound from the registers of the
         // lane that pre-filtered it, its edge constants as kLdsHit says
         const uint32_t qmk = (uint32_t)__builtin_amdgcn_readlane((int)qm, kl);
-        const uint32_t zkey = (uint32_t)__builtin_amdgcn_readlane((int)h.w, kl);
+        const uint32_t zkey = (uint32_t)__builtin_amdgcn_readlane((int)zb, kl);
         float A[3], B[3], E[3];
         if constexpr (kLdsHit) {
 #pragma unroll
@@ -583,12 +586,11 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
         hs &= hs - 1;
         if (qn > kQFlush) flush();
         const SetupRec& r = s_rec[k];
-        const uint32_t hx = (uint32_t)__builtin_amdgcn_readlane((int)h.x, kl);
-        const uint32_t hy = (uint32_t)__builtin_amdgcn_readlane((int)h.y, kl);
+        const uint32_t hx = r.rx, hy = r.ry;
         const int rx0 = (int)(hx & 0xFFFFu), rx1 = (int)(hx >> 16);
         const int ry0 = (int)(hy & 0xFFFFu), ry1 = (int)(hy >> 16);
-        const uint32_t flags = (uint32_t)__builtin_amdgcn_readlane((int)h.z, kl);
-        const uint32_t zkey = (uint32_t)__builtin_amdgcn_readlane((int)h.w, kl);
+        const uint32_t flags = r.flags;
+        const uint32_t zkey = (uint32_t)__builtin_amdgcn_readlane((int)zb, kl);
 #pragma unroll
         for (int qq = 0; qq < QPW; qq++) {
           const int Q = wq0 + qq;
