@@ -321,6 +321,30 @@ def app_scene(seed: int = 6, screen=(1920, 1080), canvas=(800, 600), n_tri: int 
                  canvas_offset=(ox, oy))
 
 
+def sliver_scene(seed: int = 23, n_tri: int = 16_000, width: int = 160, height: int = 120) -> Scene:
+    """Slivers that cross both clip planes: one vertex between the camera and
+    the near plane, two beyond the far plane, so each front-facing one clips
+    into about three triangles (half are back-facing).  16000 of them emit
+    about 24000 clipped triangles, more than a fresh context's first capacity
+    (n + n/4 + 1024)."""
+    rng = _rng(seed)
+    tex = _phong(noise_texture(64, rng))
+    n = n_tri
+    a = np.stack([rng.uniform(-0.01, 0.01, n), rng.uniform(-0.01, 0.01, n), np.full(n, 3.0 - 0.05)], -1)
+    b = np.stack([rng.uniform(-60.0, 60.0, n), rng.uniform(-45.0, 45.0, n), np.full(n, 3.0 - 150.0)], -1)
+    c = b + np.stack([rng.uniform(-8.0, 8.0, n), rng.uniform(-8.0, 8.0, n), rng.uniform(-20.0, 20.0, n)], -1)
+    pc = np.stack([a, b, c], 1)
+    flip = rng.random(n) < 0.5
+    pc[flip] = pc[flip][:, ::-1, :]
+    nrm = np.tile(np.array([0.0, 0.0, 1.0], F32), (3 * n, 1))
+    uv = rng.random((3 * n, 2))
+    idx = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+    obj = Object(pc.reshape(-1, 3).astype(F32), nrm, uv.astype(F32),
+                 [IndexedMesh(np.concatenate([idx, idx, idx], 1), 0, "slivers")], [tex], name="slivers")
+    cam = Camera(position=(0.0, 0.0, 3.0), fov_x=60.0, near=-0.1, far=-100.0)
+    return Scene(width, height, cam, (0.0, 6.0, 6.0), [obj])
+
+
 def small_random_scene(seed: int, n_tri: int = 300, width: int = 96, height: int = 64,
                        cross_frac: float = 0.2, tex_size: int = 16, power: int = 2) -> Scene:
     """Tiny soup for exhaustive parity checks."""

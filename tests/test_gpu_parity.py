@@ -260,6 +260,43 @@ def test_bin_capacity_regrowth_large_triangles():
         r.close()
 
 
+@pytest.mark.parametrize("n_tri", [16384, 16385])
+def test_small_frame_limit(renderer, n_tri):
+    """16384 input triangles is the largest frame the single-workgroup
+    geometry and binning kernels take; one more takes the general path.  Both
+    match the oracle."""
+    s = workloads.soup_scene(11, n_tri=n_tri, width=320, height=240, tex_size=256)
+    ref = oracle(s)
+    f, d, w = _gpu(renderer, s, 32, 8)
+    assert_same(f, d, w, ref, label=f"soup of {n_tri}")
+    assert renderer.stats()["tris_out"] == ref.stats["tris_emitted"]
+
+
+@pytest.mark.parametrize("kind,tile,waves", [(0, 16, 4), (1, 32, 4)])
+def test_small_frame_capacity_regrowth(kind, tile, waves):
+    """A fresh context's first small frame (16000 input triangles: the
+    single-workgroup geometry and binning kernels) emits more clipped
+    triangles than its initial capacity, and their bin entries overflow the
+    pair buffers: the frame reports the overflow, regrows, is retried and
+    matches the oracle, in both bin orders."""
+    from draw_amd.render import Renderer
+    s = workloads.sliver_scene(23)
+    ref = oracle(s)
+    assert ref.stats["tris_emitted"] > 16000 + 16000 // 4 + 1024
+    for mode in (2, 1):
+        r = Renderer(0)
+        try:
+            r.set_bin_order(mode)
+            r.set_raster(kind, tile, waves)
+            canvas, winner = r.render(s)
+            assert_same(canvas.frame, canvas.depth, winner, ref, label=f"slivers, bin order {mode}")
+            st = r.stats()
+            assert st["retries"] >= 1, st
+            assert st["tris_out"] == ref.stats["tris_emitted"], st
+        finally:
+            r.close()
+
+
 @pytest.mark.parametrize("screen,kind,tile,waves", [((1920, 1080), 0, 32, 8), ((2560, 1440), 0, 16, 4),
                                                     ((1920, 1080), 1, 32, 4)])
 def test_reference_app_configuration_offset_canvas(renderer, screen, kind, tile, waves):
