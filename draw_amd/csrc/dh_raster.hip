@@ -297,6 +297,13 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
     zk[qq] = (px < W && py < H) ? 0xFFFFFFFFu : 0u;
   }
   uint32_t wmax = 0xFFFFFFFFu;  // largest recorded depth key over the wave's pixels
+  // ...and over each of its quadrants (hierarchical Z per quadrant, with 4
+  // quadrants per wave: a record's quadrant bits drop where it is behind;
+  // with 2, the wave maximum per quadrant cost more than it saved)
+  constexpr bool kQuadZ = QPW == 4;
+  uint32_t qmax[QPW];
+#pragma unroll
+  for (int qq = 0; qq < QPW; qq++) qmax[qq] = 0xFFFFFFFFu;
   uint32_t qn = 0;
   uint32_t nfrag = 0;
 
@@ -323,9 +330,14 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
 #pragma unroll
     for (int qq = 0; qq < QPW; qq++) {
       zk[qq] = (uint32_t)(s_vis[lp[qq]] >> 32);
-      m = max(m, zk[qq]);
+      if constexpr (kQuadZ) {
+        qmax[qq] = wave_max_u32(zk[qq]);
+        m = max(m, qmax[qq]);
+      } else {
+        m = max(m, zk[qq]);
+      }
     }
-    wmax = wave_max_u32(m);
+    wmax = kQuadZ ? m : wave_max_u32(m);
     s_wmax[wave] = wmax;  // read at the next staging, after a barrier
   };
 
@@ -515,6 +527,11 @@ __global__ __launch_bounds__(64 * (TS / 8) * (TS / 8) / QPW) void k_raster(
         zb = s_zk[g0 + lane];
         const uint32_t m = s_qm[g0 + lane];
         qm = (m >> wq0) & ((1u << QPW) - 1u);
+        if constexpr (kQuadZ) {
+#pragma unroll
+          for (int qq = 0; qq < QPW; qq++)
+            if (zb > qmax[qq]) qm &= ~(1u << qq);
+        }
         hit = qm != 0u && !(zb > wmax);
         wexact = (m & kQExact) != 0u;
         if constexpr (!kLdsHit) {
