@@ -550,8 +550,8 @@ def main():
         print(json.dumps(copy_calibration(args.copy_calibration)), flush=True)
         return
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (asked for {args.gpus}); the path shards as described "
-              f"in DESIGN.md but no multi-GPU run is made", file=sys.stderr)
+        print(f"bench.py: measures one GPU only (asked for {args.gpus}); the sort-last shard leg times one GPU's "
+              f"share of an 8-way split, but no multi-GPU run is made", file=sys.stderr)
         sys.exit(2)
 
     from draw_amd import workloads
