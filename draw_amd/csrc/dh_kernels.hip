@@ -1721,12 +1721,13 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
 // and its items take their output slots with LDS atomic increments of their
 // key's cursor: a tile's entries come chunk by chunk in input order, in any
 // order within a chunk.
-constexpr int kRadixBG = 256;  // blocks (chunks) of the unordered one-pass sort
+constexpr int kRadixBG = 256;   // blocks (chunks) of the unordered one-pass sort
+constexpr int kRadixBT = 1024;  // threads per chunk: 16 waves, one block per CU
 static_assert(2048u * kRadixBG <= kRadixHistEntries, "histogram size");
 
 __device__ inline uint32_t radix_chunk_blk(uint32_t n) { return (((n + kRadixBG - 1u) / kRadixBG) + 255u) & ~255u; }
 
-__global__ __launch_bounds__(256) void k_radix_hist_blk(const uint32_t* __restrict__ keys, const uint32_t* n_ptr,
+__global__ __launch_bounds__(kRadixBT) void k_radix_hist_blk(const uint32_t* __restrict__ keys, const uint32_t* n_ptr,
                                                         uint32_t cap, uint32_t* __restrict__ hist) {
   __shared__ uint32_t h[2048];
   const int tid = threadIdx.x;
@@ -1734,27 +1735,27 @@ __global__ __launch_bounds__(256) void k_radix_hist_blk(const uint32_t* __restri
   const uint32_t C = radix_chunk_blk(n);
   const uint32_t cb = xcd_remap(blockIdx.x, gridDim.x);
   const uint32_t b0 = min(cb * C, n), b1 = min(b0 + C, n);
-  for (int d = tid; d < 2048; d += 256) h[d] = 0u;
+  for (int d = tid; d < 2048; d += kRadixBT) h[d] = 0u;
   __syncthreads();
-  for (uint32_t tb = b0; tb < b1; tb += 256u * kRadixU) {
+  for (uint32_t tb = b0; tb < b1; tb += (uint32_t)kRadixBT * kRadixU) {
     uint32_t k[kRadixU];
 #pragma unroll
     for (int u = 0; u < kRadixU; u++) {
-      const uint32_t i = tb + 256u * (uint32_t)u + (uint32_t)tid;
+      const uint32_t i = tb + (uint32_t)kRadixBT * (uint32_t)u + (uint32_t)tid;
       k[u] = i < b1 ? keys[i] : 0u;
     }
 #pragma unroll
     for (int u = 0; u < kRadixU; u++)
-      if (tb + 256u * (uint32_t)u + (uint32_t)tid < b1) atomicAdd(&h[k[u] & 2047u], 1u);
+      if (tb + (uint32_t)kRadixBT * (uint32_t)u + (uint32_t)tid < b1) atomicAdd(&h[k[u] & 2047u], 1u);
   }
   __syncthreads();
-  for (int d = tid; d < 2048; d += 256) hist[(uint32_t)d * kRadixBG + cb] = h[d];
+  for (int d = tid; d < 2048; d += kRadixBT) hist[(uint32_t)d * kRadixBG + cb] = h[d];
 }
 
 // vout[slot] = vin[i] for the chunk's items, slots from the scanned histogram
 // (hist_off); ranges (may be null): [start, end) of each of the first n_ranges
 // keys in the output.
-__global__ __launch_bounds__(256) void k_radix_scatter_blk(const uint32_t* __restrict__ kin,
+__global__ __launch_bounds__(kRadixBT) void k_radix_scatter_blk(const uint32_t* __restrict__ kin,
                                                            const uint32_t* __restrict__ vin,
                                                            uint32_t* __restrict__ vout, const uint32_t* n_ptr,
                                                            uint32_t cap, const uint32_t* __restrict__ hist_off,
@@ -1766,21 +1767,21 @@ __global__ __launch_bounds__(256) void k_radix_scatter_blk(const uint32_t* __res
   const uint32_t cb = xcd_remap(blockIdx.x, gridDim.x);
   const uint32_t b0 = min(cb * C, n), b1 = min(b0 + C, n);
   if (ranges)
-    for (uint32_t d = blockIdx.x * 256u + (uint32_t)tid; d < n_ranges; d += gridDim.x * 256u)
+    for (uint32_t d = blockIdx.x * (uint32_t)kRadixBT + (uint32_t)tid; d < n_ranges; d += gridDim.x * (uint32_t)kRadixBT)
       ranges[d] = make_uint2(hist_off[d * kRadixBG], d + 1u < 2048u ? hist_off[(d + 1u) * kRadixBG] : n);
-  for (int d = tid; d < 2048; d += 256) run[d] = hist_off[(uint32_t)d * kRadixBG + cb];
+  for (int d = tid; d < 2048; d += kRadixBT) run[d] = hist_off[(uint32_t)d * kRadixBG + cb];
   __syncthreads();
-  for (uint32_t tb = b0; tb < b1; tb += 256u * kRadixU) {
+  for (uint32_t tb = b0; tb < b1; tb += (uint32_t)kRadixBT * kRadixU) {
     uint32_t k[kRadixU], v[kRadixU];
 #pragma unroll
     for (int u = 0; u < kRadixU; u++) {
-      const uint32_t i = tb + 256u * (uint32_t)u + (uint32_t)tid;
+      const uint32_t i = tb + (uint32_t)kRadixBT * (uint32_t)u + (uint32_t)tid;
       k[u] = i < b1 ? kin[i] : 0u;
       v[u] = i < b1 ? vin[i] : 0u;
     }
 #pragma unroll
     for (int u = 0; u < kRadixU; u++)
-      if (tb + 256u * (uint32_t)u + (uint32_t)tid < b1) vout[atomicAdd(&run[k[u] & 2047u], 1u)] = v[u];
+      if (tb + (uint32_t)kRadixBT * (uint32_t)u + (uint32_t)tid < b1) vout[atomicAdd(&run[k[u] & 2047u], 1u)] = v[u];
   }
 }
 
@@ -1882,9 +1883,9 @@ int enqueue_radix_sort(uint32_t* const keys[2], uint32_t* const vals[2], const u
                        uint2* ranges, uint32_t n_ranges, bool stable) {
   if (!stable && !keys_out && key_bits <= 11 && vals[0]) {
     const uint32_t nh = 2048u * kRadixBG;
-    hipLaunchKernelGGL(k_radix_hist_blk, dim3(kRadixBG), dim3(256), 0, s, keys[0], n_ptr, cap, hist);
+    hipLaunchKernelGGL(k_radix_hist_blk, dim3(kRadixBG), dim3(kRadixBT), 0, s, keys[0], n_ptr, cap, hist);
     scan_u32(hist, hist, nullptr, nh, nh, partials, nullptr, 0, nullptr, 0, s);
-    hipLaunchKernelGGL(k_radix_scatter_blk, dim3(kRadixBG), dim3(256), 0, s, keys[0], vals[0], vals[1], n_ptr, cap,
+    hipLaunchKernelGGL(k_radix_scatter_blk, dim3(kRadixBG), dim3(kRadixBT), 0, s, keys[0], vals[0], vals[1], n_ptr, cap,
                        hist, ranges, ranges ? n_ranges : 0u);
     return 1;
   }
