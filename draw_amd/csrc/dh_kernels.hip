@@ -1713,29 +1713,30 @@ __global__ __launch_bounds__(256) void k_radix_scatter(const uint32_t* __restric
   }
 }
 
-// One-pass sort by keys below 2048 whose order within a key is free: the
-// near-first tile lists, whose entries only need to stay roughly in depth
-// order (a tile's visibility does not depend on the order of its list).  Each
-// block owns one contiguous chunk of the input (chunks contiguous per XCD, as
-// above), so the histogram has 2048 x 256 entries instead of 2048 per wave,
-// and its items take their output slots with LDS atomic increments of their
-// key's cursor: a tile's entries come chunk by chunk in input order, in any
-// order within a chunk.
+// One-pass sort by keys below DIG (256 or 2048) whose order within a key is
+// free: the near-first depth order and tile lists, whose entries only need to
+// stay roughly in depth order (a tile's visibility does not depend on the
+// order of its list).  Each block owns one contiguous chunk of the input
+// (chunks contiguous per XCD, as above), so the histogram has DIG x 256
+// entries instead of DIG per wave, and its items take their output slots
+// with LDS atomic increments of their key's cursor: a key's entries come
+// chunk by chunk in input order, in any order within a chunk.
 constexpr int kRadixBG = 256;   // blocks (chunks) of the unordered one-pass sort
 constexpr int kRadixBT = 1024;  // threads per chunk: 16 waves, one block per CU
 static_assert(2048u * kRadixBG <= kRadixHistEntries, "histogram size");
 
 __device__ inline uint32_t radix_chunk_blk(uint32_t n) { return (((n + kRadixBG - 1u) / kRadixBG) + 255u) & ~255u; }
 
+template <int DIG>
 __global__ __launch_bounds__(kRadixBT) void k_radix_hist_blk(const uint32_t* __restrict__ keys, const uint32_t* n_ptr,
-                                                        uint32_t cap, uint32_t* __restrict__ hist) {
-  __shared__ uint32_t h[2048];
+                                                             uint32_t cap, uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[DIG];
   const int tid = threadIdx.x;
   const uint32_t n = min(*n_ptr, cap);
   const uint32_t C = radix_chunk_blk(n);
   const uint32_t cb = xcd_remap(blockIdx.x, gridDim.x);
   const uint32_t b0 = min(cb * C, n), b1 = min(b0 + C, n);
-  for (int d = tid; d < 2048; d += kRadixBT) h[d] = 0u;
+  for (int d = tid; d < DIG; d += kRadixBT) h[d] = 0u;
   __syncthreads();
   for (uint32_t tb = b0; tb < b1; tb += (uint32_t)kRadixBT * kRadixU) {
     uint32_t k[kRadixU];
@@ -1746,21 +1747,22 @@ __global__ __launch_bounds__(kRadixBT) void k_radix_hist_blk(const uint32_t* __r
     }
 #pragma unroll
     for (int u = 0; u < kRadixU; u++)
-      if (tb + (uint32_t)kRadixBT * (uint32_t)u + (uint32_t)tid < b1) atomicAdd(&h[k[u] & 2047u], 1u);
+      if (tb + (uint32_t)kRadixBT * (uint32_t)u + (uint32_t)tid < b1) atomicAdd(&h[k[u] & (DIG - 1u)], 1u);
   }
   __syncthreads();
-  for (int d = tid; d < 2048; d += kRadixBT) hist[(uint32_t)d * kRadixBG + cb] = h[d];
+  for (int d = tid; d < DIG; d += kRadixBT) hist[(uint32_t)d * kRadixBG + cb] = h[d];
 }
 
-// vout[slot] = vin[i] for the chunk's items, slots from the scanned histogram
-// (hist_off); ranges (may be null): [start, end) of each of the first n_ranges
-// keys in the output.
+// vout[slot] = vin[i] (vin null: i itself) for the chunk's items, slots from
+// the scanned histogram (hist_off); ranges (may be null): [start, end) of each
+// of the first n_ranges keys in the output.
+template <int DIG>
 __global__ __launch_bounds__(kRadixBT) void k_radix_scatter_blk(const uint32_t* __restrict__ kin,
-                                                           const uint32_t* __restrict__ vin,
-                                                           uint32_t* __restrict__ vout, const uint32_t* n_ptr,
-                                                           uint32_t cap, const uint32_t* __restrict__ hist_off,
-                                                           uint2* __restrict__ ranges, uint32_t n_ranges) {
-  __shared__ uint32_t run[2048];
+                                                                const uint32_t* __restrict__ vin,
+                                                                uint32_t* __restrict__ vout, const uint32_t* n_ptr,
+                                                                uint32_t cap, const uint32_t* __restrict__ hist_off,
+                                                                uint2* __restrict__ ranges, uint32_t n_ranges) {
+  __shared__ uint32_t run[DIG];
   const int tid = threadIdx.x;
   const uint32_t n = min(*n_ptr, cap);
   const uint32_t C = radix_chunk_blk(n);
@@ -1768,8 +1770,8 @@ __global__ __launch_bounds__(kRadixBT) void k_radix_scatter_blk(const uint32_t* 
   const uint32_t b0 = min(cb * C, n), b1 = min(b0 + C, n);
   if (ranges)
     for (uint32_t d = blockIdx.x * (uint32_t)kRadixBT + (uint32_t)tid; d < n_ranges; d += gridDim.x * (uint32_t)kRadixBT)
-      ranges[d] = make_uint2(hist_off[d * kRadixBG], d + 1u < 2048u ? hist_off[(d + 1u) * kRadixBG] : n);
-  for (int d = tid; d < 2048; d += kRadixBT) run[d] = hist_off[(uint32_t)d * kRadixBG + cb];
+      ranges[d] = make_uint2(hist_off[d * kRadixBG], d + 1u < (uint32_t)DIG ? hist_off[(d + 1u) * kRadixBG] : n);
+  for (int d = tid; d < DIG; d += kRadixBT) run[d] = hist_off[(uint32_t)d * kRadixBG + cb];
   __syncthreads();
   for (uint32_t tb = b0; tb < b1; tb += (uint32_t)kRadixBT * kRadixU) {
     uint32_t k[kRadixU], v[kRadixU];
@@ -1777,11 +1779,11 @@ __global__ __launch_bounds__(kRadixBT) void k_radix_scatter_blk(const uint32_t* 
     for (int u = 0; u < kRadixU; u++) {
       const uint32_t i = tb + (uint32_t)kRadixBT * (uint32_t)u + (uint32_t)tid;
       k[u] = i < b1 ? kin[i] : 0u;
-      v[u] = i < b1 ? vin[i] : 0u;
+      v[u] = i < b1 ? (vin ? vin[i] : i) : 0u;
     }
 #pragma unroll
     for (int u = 0; u < kRadixU; u++)
-      if (tb + (uint32_t)kRadixBT * (uint32_t)u + (uint32_t)tid < b1) vout[atomicAdd(&run[k[u] & 2047u], 1u)] = v[u];
+      if (tb + (uint32_t)kRadixBT * (uint32_t)u + (uint32_t)tid < b1) vout[atomicAdd(&run[k[u] & (DIG - 1u)], 1u)] = v[u];
   }
 }
 
@@ -1881,12 +1883,20 @@ void enqueue_mesh_streams(const uint32_t* tri, uint32_t n_tri, const float* pos,
 int enqueue_radix_sort(uint32_t* const keys[2], uint32_t* const vals[2], const uint32_t* n_ptr, uint32_t cap,
                        int key_bits, uint32_t* hist, uint32_t* partials, hipStream_t s, bool keys_out,
                        uint2* ranges, uint32_t n_ranges, bool stable) {
-  if (!stable && !keys_out && key_bits <= 11 && vals[0]) {
-    const uint32_t nh = 2048u * kRadixBG;
-    hipLaunchKernelGGL(k_radix_hist_blk, dim3(kRadixBG), dim3(kRadixBT), 0, s, keys[0], n_ptr, cap, hist);
-    scan_u32(hist, hist, nullptr, nh, nh, partials, nullptr, 0, nullptr, 0, s);
-    hipLaunchKernelGGL(k_radix_scatter_blk, dim3(kRadixBG), dim3(kRadixBT), 0, s, keys[0], vals[0], vals[1], n_ptr, cap,
-                       hist, ranges, ranges ? n_ranges : 0u);
+  if (!stable && !keys_out && key_bits <= 11) {
+    if (key_bits <= 8) {
+      const uint32_t nh = 256u * kRadixBG;
+      hipLaunchKernelGGL(k_radix_hist_blk<256>, dim3(kRadixBG), dim3(kRadixBT), 0, s, keys[0], n_ptr, cap, hist);
+      scan_u32(hist, hist, nullptr, nh, nh, partials, nullptr, 0, nullptr, 0, s);
+      hipLaunchKernelGGL(k_radix_scatter_blk<256>, dim3(kRadixBG), dim3(kRadixBT), 0, s, keys[0], vals[0], vals[1],
+                         n_ptr, cap, hist, ranges, ranges ? n_ranges : 0u);
+    } else {
+      const uint32_t nh = 2048u * kRadixBG;
+      hipLaunchKernelGGL(k_radix_hist_blk<2048>, dim3(kRadixBG), dim3(kRadixBT), 0, s, keys[0], n_ptr, cap, hist);
+      scan_u32(hist, hist, nullptr, nh, nh, partials, nullptr, 0, nullptr, 0, s);
+      hipLaunchKernelGGL(k_radix_scatter_blk<2048>, dim3(kRadixBG), dim3(kRadixBT), 0, s, keys[0], vals[0], vals[1],
+                         n_ptr, cap, hist, ranges, ranges ? n_ranges : 0u);
+    }
     return 1;
   }
   int cur = 0;
@@ -2041,14 +2051,14 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_bin_count_wide<true>, dim3(gw), dim3(256), 0, s2, a.setup, a.views, a.status, a.TS,
                        wide_list, a.bincnt, a.tile_cnt);
     if (depth_sorted) {
-      // near first: the clipped triangles in depth-key order (one stable
-      // 8-bit pass over keys[1] + cap_tri, the values starting as the
-      // triangle ids themselves; results in vals[1] + cap_tri, the sorted
-      // keys not written), then the bin offsets in that order
+      // near first: the clipped triangles in depth-key order (one 8-bit
+      // pass over keys[1] + cap_tri, the values the triangle ids themselves,
+      // in chunk order within a key; results in vals[1] + cap_tri, the
+      // sorted keys not written), then the bin offsets in that order
       uint32_t* const dkeys[2] = {a.keys[1] + a.cap_tri, a.keys[1] + 2ull * a.cap_tri};
       uint32_t* const dvals[2] = {nullptr, a.vals[1] + a.cap_tri};
       const int dc = enqueue_radix_sort(dkeys, dvals, &a.status->n_out, a.cap_tri, kDepthKeyBits, a.hist,
-                                        a.partials, s, false, nullptr, 0u);
+                                        a.partials, s, false, nullptr, 0u, false);
       join(s, a, 3);
       uint32_t* order = dvals[dc];
       // the scan of the counts in that order: its chunk sums here, the rest
