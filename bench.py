@@ -48,10 +48,11 @@ def shade_bytes(P: int, p_cov: int, t_vis: int) -> dict:
     """Algorithmic HBM bytes of one k_shade launch: per pixel a 4 B winner id
     read and a 4 B BGRA8 write; per covered pixel the 4 B diffuse texel (the
     ambient map is 1x1); per visible triangle 3 vertices x 14 f32 (normal,
-    light, halfway, uv, depth).  ``record`` is what this implementation moves:
-    it also writes f32 depth and the winner id per pixel and reads a 96 B raster
-    record plus a 144 B shading record per visible triangle."""
-    return {"algorithmic": 8 * P + 4 * p_cov + 168 * t_vis, "record": 16 * P + 4 * p_cov + 240 * t_vis}
+    light, halfway, uv, depth).  ``record`` is what this implementation moves
+    into the bench's canvases (no winner buffer): it also writes the f32 depth
+    per pixel and reads a 96 B raster record plus a 144 B shading record per
+    visible triangle."""
+    return {"algorithmic": 8 * P + 4 * p_cov + 168 * t_vis, "record": 12 * P + 4 * p_cov + 240 * t_vis}
 
 
 def raster_bytes(P: int, entries: int) -> dict:
@@ -68,10 +69,11 @@ def raster_shade_bytes(P: int, entries: int, p_cov: int, t_vis: int) -> dict:
     tiles: per pixel a 4 B BGRA8 and an f32 depth written; per bin entry a 4 B
     id and a 48 B setup record read; per visible triangle 3 vertices x 14 f32
     and per covered pixel the 4 B diffuse texel.  ``record``: this
-    implementation also writes the u32 winner per pixel, reads a 96 B raster
-    record per entry and 224 B of records per visible triangle."""
+    implementation reads a 96 B raster record per entry and 224 B of records
+    per visible triangle (the bench's canvases have no winner buffer; one that
+    keeps winners adds 4 B per pixel)."""
     return {"algorithmic": 8 * P + 52 * entries + 168 * t_vis + 4 * p_cov,
-            "record": 12 * P + 100 * entries + 224 * t_vis + 4 * p_cov}
+            "record": 8 * P + 100 * entries + 224 * t_vis + 4 * p_cov}
 
 
 TRAFFIC_PROFILE = "profiles/traffic_soup1m_1080p.md"
@@ -232,6 +234,18 @@ def cpu_baseline(scene, p_cov_full: int) -> dict:
     }
 
 
+def winner_frame(r, render, W: int, H: int):
+    """The winner ids of one more frame, rendered into a canvas that keeps
+    them: the timed frames use canvases without a winner buffer
+    (DH_CANVAS_NO_WINNER), as the reference's Canvas has none."""
+    r.winners = True
+    try:
+        _, _, winner = r.read(render(), W, H)
+    finally:
+        r.winners = False
+    return winner
+
+
 def bench_config(name: str, scene, steps: int, warmup: int, cpu_runs: int, cpu_warmup: int) -> dict:
     """One more plain-path workload at its configured size: autotuned raster,
     ``steps`` timed frames (one dh_render each, graph-replayed), p50/p99 frame
@@ -240,6 +254,7 @@ def bench_config(name: str, scene, steps: int, warmup: int, cpu_runs: int, cpu_w
     from draw_amd.render import Renderer, frame_uniforms
     fr = frame_uniforms(scene)
     r = Renderer(0, profiling=True)
+    r.winners = False
     meshes = r.mesh_handles(scene)
     (kind, tile, waves), table = autotune(r, lambda: r.render_device(scene, fr, meshes))
     for _ in range(warmup):
@@ -251,7 +266,7 @@ def bench_config(name: str, scene, steps: int, warmup: int, cpu_runs: int, cpu_w
         cv = r.render_device(scene, fr, meshes)
         lat.append(time.perf_counter() - a)
     total = time.perf_counter() - t0
-    _, _, winner = r.read(cv, fr.W, fr.H)
+    winner = winner_frame(r, lambda: r.render_device(scene, fr, meshes), fr.W, fr.H)
     p_cov, t_vis = visible_triangles(winner)
     st = r.stats()
     r.close()
@@ -285,6 +300,7 @@ def bench_batched(reps: int) -> dict:
     views = workloads.batched_scenes(seed=4)
     items = [(s, frame_uniforms(s)) for s in views]
     r = Renderer(0, profiling=True)
+    r.winners = False  # as the reference's Canvas: frame and depth only
     prepared = r.batch_views(items)
     meshes = [r.mesh_handles(s) for s in views]
     log("batched: autotune ...")
@@ -313,8 +329,13 @@ def bench_batched(reps: int) -> dict:
     # one eager batch with per-stage HIP events (the graph replay records only the whole launch)
     r.render_batch(items, graph=False, prepared=prepared)
     st = r.stats()
+    # covered pixels and winning triangles from one more batch into canvases
+    # that keep the winner ids
+    r.winners = True
+    wcvs = r.render_batch(items, graph=False, prepared=r.batch_views(items, slot_base=len(items)))
+    r.winners = False
     covered = t_vis = 0
-    for cv, s in zip(cvs, views):
+    for cv, s in zip(wcvs, views):
         _, _, w = r.read(cv, s.width, s.height)
         pc, tv = visible_triangles(w)
         covered += pc
@@ -562,6 +583,7 @@ def main():
     fr = frame_uniforms(scene)
     W, H = fr.W, fr.H
     r = Renderer(0, profiling=True)
+    r.winners = False  # as the reference's Canvas: frame and depth only
     r.set_raster(args.raster_kind, args.tile, args.waves)
     meshes = r.mesh_handles(scene)
     tune_table = None
@@ -582,7 +604,7 @@ def main():
         lat.append(time.perf_counter() - a)
         graph_ms.append(r.stats()["ms_frame"])
     total = time.perf_counter() - t0
-    _, _, winner = r.read(cv, W, H)
+    winner = winner_frame(r, lambda: r.render_device(scene, fr, meshes), W, H)
     p_cov, t_vis = visible_triangles(winner)
     # the drop-in's whole Scene::render: the frame plus dh_canvas_read of the
     # BGRA8 frame and f32 depth into caller-owned (pageable) host buffers
@@ -620,7 +642,8 @@ def main():
         "config": {"workload": "plain GPU path, 1920x1080, seeded 1M-triangle soup (20% near-plane straddlers, "
                                "log-normal area median 40 px, random winding: half the triangles face away and "
                                "are culled), 2048x2048 RGBA8 texture, one frame per dh_render into a canvas "
-                               "resident in HBM (launch sequence replayed from a hipGraph)",
+                               "resident in HBM (BGRA8 and f32 depth, no winner buffer, as the reference's "
+                               "Canvas; launch sequence replayed from a hipGraph)",
                    "width": W, "height": H, "triangles": scene.triangle_count, "back_facing_frac": 0.5,
                    "raster_kind": args.raster_kind,
                    "tile": args.tile,

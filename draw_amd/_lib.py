@@ -41,6 +41,7 @@ class DhView(C.Structure):
 
 
 DH_BATCH_GRAPH = 1
+DH_CANVAS_NO_WINNER = 1
 
 
 class DhServerConfig(C.Structure):
@@ -130,6 +131,7 @@ SIGNATURES = {
     "dh_object_create": [_P, _P, C.c_uint32, _P, C.c_uint32, _P, C.c_uint32, C.POINTER(_H)],
     "dh_mesh_create": [_P, _H, _P, C.c_uint32, _H, C.POINTER(_H)],
     "dh_canvas_create": [_P, C.c_int32, C.c_int32, C.POINTER(_H)],
+    "dh_canvas_create_ex": [_P, C.c_int32, C.c_int32, C.c_uint32, C.POINTER(_H)],
     "dh_mesh_read": [_P, _H, _P],
     "dh_release": [_P, _H],
     "dh_render": [_P, C.POINTER(DhFrameParams), C.POINTER(_H), C.c_uint32, _H],

@@ -594,7 +594,12 @@ extern "C" int dh_mesh_read(dh_ctx c, dh_handle mesh, uint32_t* tri) {
 }
 
 extern "C" int dh_canvas_create(dh_ctx c, int32_t W, int32_t H, dh_handle* out) {
-  if (!c || !out || W <= 0 || H <= 0 || W > 65535 || H > 65535) return DH_E_ARG;
+  return dh_canvas_create_ex(c, W, H, 0u, out);
+}
+
+extern "C" int dh_canvas_create_ex(dh_ctx c, int32_t W, int32_t H, uint32_t flags, dh_handle* out) {
+  if (!c || !out || W <= 0 || H <= 0 || W > 65535 || H > 65535 || (flags & ~(uint32_t)DH_CANVAS_NO_WINNER))
+    return DH_E_ARG;
   std::lock_guard<std::mutex> lk(c->res->mu);
   HIPCHK(c, hipSetDevice(c->device));
   CanvasRes cv;
@@ -603,7 +608,7 @@ extern "C" int dh_canvas_create(dh_ctx c, int32_t W, int32_t H, dh_handle* out) 
   const size_t n = (size_t)W * (size_t)H;
   HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&cv.bgra), n * 4));
   HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&cv.depth), n * 4));
-  HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&cv.winner), n * 4));
+  if (!(flags & DH_CANVAS_NO_WINNER)) HIPCHK(c, hipMalloc(reinterpret_cast<void**>(&cv.winner), n * 4));
   const uint64_t id = c->res->next_id++;
   c->res->canvases[id] = cv;
   *out = make_handle(kCanvas, id);
@@ -1120,6 +1125,7 @@ extern "C" int dh_canvas_read(dh_ctx c, dh_handle canvas, uint8_t* bgra, float* 
   CanvasRes cv;
   const int rc = find_canvas(c, canvas, &cv);
   if (rc) return rc;
+  if (winner && !cv.winner) return fail(c, DH_E_ARG, "canvas created without a winner buffer");
   const size_t n = (size_t)cv.W * (size_t)cv.H;
   HIPCHK(c, hipSetDevice(c->device));
   if (bgra) HIPCHK(c, hipMemcpyAsync(bgra, cv.bgra, n * 4, hipMemcpyDeviceToHost, c->stream));

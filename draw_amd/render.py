@@ -53,7 +53,11 @@ class Renderer:
         self._tex: Dict[int, int] = {}
         self._mat: Dict[Tuple[int, int], int] = {}
         self._obj: Dict[int, Tuple[int, List[int]]] = {}
-        self._canvas: Dict[Tuple[int, int], int] = {}
+        self._canvas: Dict[tuple, int] = {}
+        # canvases created from here on keep a winner id per pixel (the
+        # reference's Canvas has none: False drops that buffer and its writes)
+        self.winners = True
+        self._nowin = set()  # canvases created without winners
         self._keep = []
         self.set_tiling(tile, waves)
         self.set_profiling(profiling)
@@ -99,6 +103,7 @@ class Renderer:
         child.ctx = out
         child._tex, child._mat, child._obj, child._canvas, child._keep = (
             self._tex, self._mat, self._obj, self._canvas, self._keep)
+        child.winners, child._nowin = self.winners, self._nowin
         return child
 
     # ------------------------------------------------- sort-last shards
@@ -182,12 +187,18 @@ class Renderer:
             self._keep.append(obj)
         return self._obj[key][1]
 
+    def _new_canvas(self, W: int, H: int) -> int:
+        h = C.c_uint64()
+        flags = 0 if self.winners else _lib.DH_CANVAS_NO_WINNER
+        self._check(self.L.dh_canvas_create_ex(self.ctx, int(W), int(H), flags, C.byref(h)), "dh_canvas_create_ex")
+        if not self.winners:
+            self._nowin.add(h.value)
+        return h.value
+
     def canvas(self, W: int, H: int) -> int:
-        key = (W, H)
+        key = (W, H, self.winners)
         if key not in self._canvas:
-            h = C.c_uint64()
-            self._check(self.L.dh_canvas_create(self.ctx, int(W), int(H), C.byref(h)), "dh_canvas_create")
-            self._canvas[key] = h.value
+            self._canvas[key] = self._new_canvas(W, H)
         return self._canvas[key]
 
     def mesh_triangles(self, handle: int, n_tri: int) -> np.ndarray:
@@ -225,11 +236,9 @@ class Renderer:
     def canvas_slot(self, slot, W: int, H: int) -> int:
         """A device canvas owned by one batch slot (views of a batch need
         distinct canvases even when their sizes agree)."""
-        key = ("slot", slot, W, H)
+        key = ("slot", slot, W, H, self.winners)
         if key not in self._canvas:
-            h = C.c_uint64()
-            self._check(self.L.dh_canvas_create(self.ctx, int(W), int(H), C.byref(h)), "dh_canvas_create")
-            self._canvas[key] = h.value
+            self._canvas[key] = self._new_canvas(W, H)
         return self._canvas[key]
 
     def batch_views(self, items, slot_base: int = 0):
@@ -257,11 +266,13 @@ class Renderer:
         return canvases
 
     def read(self, canvas_handle: int, W: int, H: int):
+        """(bgra, depth, winner) of a canvas; winner is None for a canvas
+        created without winners."""
         bgra = np.empty((H, W, 4), np.uint8)
         depth = np.empty((H, W), np.float32)
-        winner = np.empty((H, W), np.uint32)
+        winner = None if canvas_handle in self._nowin else np.empty((H, W), np.uint32)
         self._check(self.L.dh_canvas_read(self.ctx, canvas_handle, bgra.ctypes.data, depth.ctypes.data,
-                                          winner.ctypes.data), "dh_canvas_read")
+                                          None if winner is None else winner.ctypes.data), "dh_canvas_read")
         return bgra, depth, winner
 
     def read_into(self, canvas_handle: int, bgra: np.ndarray, depth: np.ndarray | None = None,

@@ -179,6 +179,13 @@ int dh_mesh_read(dh_ctx ctx, dh_handle mesh, uint32_t* tri);
 /* Device-resident Canvas (canvas.rs:353-433): BGRA8 frame + f32 depth +
  * u32 winner id per pixel. */
 int dh_canvas_create(dh_ctx ctx, int32_t W, int32_t H, dh_handle* out);
+/* The same with flags.  DH_CANVAS_NO_WINNER: no winner buffer, as the
+ * reference's Canvas (canvas.rs:353-433) has none; frames render and shade
+ * the same, without the 4 B per pixel of winner writes, and
+ * dh_canvas_read / dh_canvas_device_ptrs report no winners (DH_E_ARG if a
+ * winner array is asked of dh_canvas_read).  Other flags: DH_E_ARG. */
+enum { DH_CANVAS_NO_WINNER = 1u };
+int dh_canvas_create_ex(dh_ctx ctx, int32_t W, int32_t H, uint32_t flags, dh_handle* out);
 int dh_release(dh_ctx ctx, dh_handle h);
 
 /* ---- rendering -------------------------------------------------------- */

@@ -260,6 +260,33 @@ def test_bin_capacity_regrowth_large_triangles():
         r.close()
 
 
+def test_canvas_without_winners(renderer):
+    """A canvas created with DH_CANVAS_NO_WINNER (the reference's Canvas has
+    no winner buffer) gets the same frame and depth bytes, in a plain frame
+    and in a batch; asking it for winners is an argument error."""
+    import ctypes as C
+    s = workloads.soup_scene(7, n_tri=20_000, width=320, height=240, tex_size=256, cross_frac=0.05)
+    items = [(workloads.torus_scene(1, width=160, height=120), None), (s, None)]
+    fr = frame_uniforms(s)
+    f1, d1, w1 = renderer.read(renderer.render_device(s, fr), fr.W, fr.H)
+    b1 = [renderer.read(cv, it[0].width, it[0].height) for cv, it in zip(renderer.render_batch(items), items)]
+    renderer.winners = False
+    try:
+        cv = renderer.render_device(s, fr)
+        f2, d2, w2 = renderer.read(cv, fr.W, fr.H)
+        b2 = [renderer.read(c2, it[0].width, it[0].height)
+              for c2, it in zip(renderer.render_batch(items, prepared=renderer.batch_views(items, slot_base=100)), items)]
+        win = np.empty((fr.H, fr.W), np.uint32)
+        assert renderer.L.dh_canvas_read(renderer.ctx, cv, None, None, win.ctypes.data) == -1
+    finally:
+        renderer.winners = True
+    assert w2 is None and w1 is not None
+    assert np.array_equal(f1, f2) and np.array_equal(d1.view(np.uint32), d2.view(np.uint32))
+    for (fa, da, _), (fb, db, wb) in zip(b1, b2):
+        assert wb is None
+        assert np.array_equal(fa, fb) and np.array_equal(da.view(np.uint32), db.view(np.uint32))
+
+
 @pytest.mark.parametrize("n_tri", [16384, 16385])
 def test_small_frame_limit(renderer, n_tri):
     """16384 input triangles is the largest frame the single-workgroup
