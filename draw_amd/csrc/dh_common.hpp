@@ -156,6 +156,14 @@ __host__ __device__ inline float edge_eval(const float e[4], float x, float y) {
   return ((e[0] * x + e[1] * y) + e[2]) - e[3];
 }
 
+// Coverage of one pixel for a FAST triangle, decided without division.  Edge
+// values are integers here (snapped vertices are floor() results and pixel
+// centres are integers), so for a finite nonzero normalizer f the quotient
+// e/f is NaN iff e is NaN, zero iff e is zero, and otherwise nonzero with the
+// sign of e*f (|e/f| >= 1/FLT_MAX is representable as a denormal).  Hence
+// alpha >= 0 && (alpha > 0 || tie) <=> e*f > 0 || (e == 0 && tie).
+__device__ inline bool covered_fast(float e, float f, bool tie) { return (e * f > 0.0f) || (e == 0.0f && tie); }
+
 // Rust `f32 as usize` on a 64-bit target: NaN and negatives -> 0, saturating.
 __host__ __device__ inline uint64_t f2usize(float v) {
   if (!(v > 0.0f)) return 0;

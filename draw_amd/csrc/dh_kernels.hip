@@ -350,6 +350,9 @@ __device__ inline void rect_axis(uint64_t a0, uint64_t a1, uint64_t smax, uint32
 // snapping (canvas.rs:585-587, 896-904), edge constants, normalizers,
 // tie-break products (660-666) and the scan rectangle (640-658).  Writes the
 // SetupRec and ShadeRec of one clipped triangle with 16-byte stores.
+// TINY_DROP: drop a FAST triangle of at most 2x2 pixels that covers none of
+// them (the unclipped emits; the clipper's registers leave no room for it).
+template <bool TINY_DROP>
 __device__ inline void make_setup(const FrameParamsPod& fr, const FV& A, const FV& B, const FV& C, uint32_t material,
                                   const DevMaterial& mat, uint32_t view, uint32_t group, uint4 (&ro)[6],
                                   uint4 (&so)[9], SetupRec& rec) {
@@ -402,6 +405,22 @@ __device__ inline void make_setup(const FrameParamsPod& fr, const FV& A, const F
   rect_axis(f2usize(mny), f2usize(mxy), (uint64_t)fr.H - 1u, &y0, &y1);
   const bool fin = isfinite(fa) && isfinite(fb) && isfinite(fg);
   if (fin && fa != 0.0f && fb != 0.0f && fg != 0.0f) flags |= kFast;
+  // (0) A FAST triangle whose scan rectangle is at most 2x2 pixels and holds
+  // no covered pixel: the raster's own test (covered_fast at each pixel of
+  // the rectangle) settled here, where the triangle is still in registers.
+  if (TINY_DROP && (flags & kFast) && x1 - x0 <= 1u && y1 - y0 <= 1u) {
+    bool any = false;
+#pragma unroll
+    for (uint32_t k = 0; k < 4u; k++) {
+      const uint32_t px = x0 + (k & 1u), py = y0 + (k >> 1);
+      const float xf = (float)px, yf = (float)py;
+      const bool cov = covered_fast(edge_eval(ea, xf, yf), fa, flags & kTieA) &&
+                       covered_fast(edge_eval(eb, xf, yf), fb, flags & kTieB) &&
+                       covered_fast(edge_eval(eg, xf, yf), fg, flags & kTieG);
+      any |= cov && px <= x1 && py <= y1;
+    }
+    if (!any) flags |= kDrop;
+  }
   // Exact drops.  (1) A NaN normalizer makes that barycentric NaN at every
   // pixel, which fails `>= 0`.  (2) If all three normalizers are +0 and every
   // edge evaluation over the scan rectangle and the vertices is exact (all
@@ -956,7 +975,7 @@ __device__ __forceinline__ void geom_emit_clipped(uint32_t j, const DevDraw* __r
     }
     SetupRec rec;
     uint4 ro[6], so[9];
-    make_setup(fr, A, B, C, material, mats[material], view, group, ro, so, rec);
+    make_setup<false>(fr, A, B, C, material, mats[material], view, group, ro, so, rec);
     // with ordered bins nothing reads a dropped triangle's records
     if (!BIN || !(rec.flags & kDrop)) {
 #pragma unroll
@@ -1018,7 +1037,7 @@ __device__ __forceinline__ void geom_emit_simple(uint32_t j, bool active, uint4*
     } else {
       // passed the back-face and lateral tests in k_geom_count
       SetupRec rec;
-      make_setup(fr, v[0], v[1], v[2], dr.material, mats[dr.material], dr.view, dr.group, ro, so, rec);
+      make_setup<true>(fr, v[0], v[1], v[2], dr.material, mats[dr.material], dr.view, dr.group, ro, so, rec);
       ok = true;
       // with ordered bins nothing reads a dropped triangle's records
       skip = BIN && (rec.flags & kDrop);
