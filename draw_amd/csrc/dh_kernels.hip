@@ -350,7 +350,7 @@ __device__ inline void rect_axis(uint64_t a0, uint64_t a1, uint64_t smax, uint32
 // snapping (canvas.rs:585-587, 896-904), edge constants, normalizers,
 // tie-break products (660-666) and the scan rectangle (640-658).  Writes the
 // SetupRec and ShadeRec of one clipped triangle with 16-byte stores.
-// TINY_DROP: drop a FAST triangle of at most 2x2 pixels that covers none of
+// TINY_DROP: drop a FAST triangle of at most 3x3 pixels that covers none of
 // them (the unclipped emits; the clipper's registers leave no room for it).
 template <bool TINY_DROP>
 __device__ inline void make_setup(const FrameParamsPod& fr, const FV& A, const FV& B, const FV& C, uint32_t material,
@@ -405,14 +405,14 @@ __device__ inline void make_setup(const FrameParamsPod& fr, const FV& A, const F
   rect_axis(f2usize(mny), f2usize(mxy), (uint64_t)fr.H - 1u, &y0, &y1);
   const bool fin = isfinite(fa) && isfinite(fb) && isfinite(fg);
   if (fin && fa != 0.0f && fb != 0.0f && fg != 0.0f) flags |= kFast;
-  // (0) A FAST triangle whose scan rectangle is at most 2x2 pixels and holds
+  // (0) A FAST triangle whose scan rectangle is at most 3x3 pixels and holds
   // no covered pixel: the raster's own test (covered_fast at each pixel of
   // the rectangle) settled here, where the triangle is still in registers.
-  if (TINY_DROP && (flags & kFast) && x1 - x0 <= 1u && y1 - y0 <= 1u) {
+  if (TINY_DROP && (flags & kFast) && x1 - x0 <= 2u && y1 - y0 <= 2u) {
     bool any = false;
 #pragma unroll
-    for (uint32_t k = 0; k < 4u; k++) {
-      const uint32_t px = x0 + (k & 1u), py = y0 + (k >> 1);
+    for (uint32_t k = 0; k < 9u; k++) {
+      const uint32_t px = x0 + k % 3u, py = y0 + k / 3u;
       const float xf = (float)px, yf = (float)py;
       const bool cov = covered_fast(edge_eval(ea, xf, yf), fa, flags & kTieA) &&
                        covered_fast(edge_eval(eb, xf, yf), fb, flags & kTieB) &&
