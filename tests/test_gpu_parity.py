@@ -81,6 +81,32 @@ def test_shells_20m_4k_full_frame_bit_exact(renderer):
     assert emitted == ref.stats["tris_emitted"]
 
 
+@pytest.mark.parametrize("name", ["shells200k_320", "soup120k_160"])
+@pytest.mark.parametrize("kind,tile,waves", [(0, 16, 4), (0, 32, 8), (1, 16, 4)])
+@pytest.mark.parametrize("mode", [2, 1])
+def test_dense_frames_bit_exact(renderer, name, kind, tile, waves, mode):
+    """Frames with at least one input triangle per pixel (past the small-frame
+    limit), where many unclipped triangles of at most 3x3 pixels cover none and
+    the emits drop them, in both ordered bin modes (near first, submission
+    order): the dropped triangles keep their clipped-triangle ids, so winners,
+    depth and colour match the oracle on every pixel, as does the emitted
+    count."""
+    s = {"shells200k_320": lambda: workloads.shells_scene(31, n_tri=200_000, width=320, height=240, tex_size=256,
+                                                           n_shells=4),
+         "soup120k_160": lambda: workloads.soup_scene(32, n_tri=120_000, width=160, height=120, tex_size=256,
+                                                      cross_frac=0.1)}[name]()
+    assert sum(o.triangle_count for o in s.objects) >= s.width * s.height
+    try:
+        renderer.set_bin_order(mode)
+        f, d, w = _gpu(renderer, s, tile, waves, kind)
+        emitted = renderer.stats()["tris_out"]
+    finally:
+        renderer.set_bin_order(2)
+    ref = oracle(s)
+    assert_same(f, d, w, ref, label=f"{name} kind {kind} tile {tile}x{waves}")
+    assert emitted == ref.stats["tris_emitted"]
+
+
 def test_soup_1m_tiling_invariance(renderer):
     s = workloads.soup_scene(2)
     base = _gpu(renderer, s, 16, 4)
