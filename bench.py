@@ -246,17 +246,27 @@ def winner_frame(r, render, W: int, H: int):
     return winner
 
 
-def bench_config(name: str, scene, steps: int, warmup: int, cpu_runs: int, cpu_warmup: int) -> dict:
-    """One more plain-path workload at its configured size: autotuned raster,
-    ``steps`` timed frames (one dh_render each, graph-replayed), p50/p99 frame
-    latency, and the oracle on whole frames of the same scene on this host."""
+SHELLS_WORKLOAD = ("20M micro-triangles on 8 nested jittered shells at 3840x2160 (5% flipped, 5% outside the "
+                   "frustum), 8192x8192 texture, one dh_render per frame")
+
+
+def bench_config(name: str, scene, steps: int, warmup: int, cpu_runs: int, cpu_warmup: int,
+                 tiling=None) -> dict:
+    """One more plain-path workload at its configured size: autotuned raster
+    (or ``tiling`` = (kind, tile, waves)), ``steps`` timed frames (one
+    dh_render each, graph-replayed), p50/p99 frame latency, and the oracle on
+    whole frames of the same scene on this host."""
     from draw_amd.autotune import autotune
     from draw_amd.render import Renderer, frame_uniforms
     fr = frame_uniforms(scene)
     r = Renderer(0, profiling=True)
     r.winners = False
     meshes = r.mesh_handles(scene)
-    (kind, tile, waves), table = autotune(r, lambda: r.render_device(scene, fr, meshes))
+    if tiling is None:
+        (kind, tile, waves), table = autotune(r, lambda: r.render_device(scene, fr, meshes))
+    else:
+        kind, tile, waves = tiling
+        r.set_raster(kind, tile, waves)
     for _ in range(warmup):
         r.render_device(scene, fr, meshes)
     lat = []
@@ -555,7 +565,8 @@ def main():
     ap.add_argument("--n-tri", type=int, default=1_000_000)
     ap.add_argument("--cpu", type=int, default=1, help="0 disables the CPU oracle timings")
     ap.add_argument("--configs", type=int, default=1,
-                    help="0 skips the 640x480 torus and the 3840x2160 20M-triangle legs")
+                    help="0 skips the 640x480 torus and the 3840x2160 20M-triangle legs; 2 runs only the "
+                         "20M-triangle leg (for profiling it alone)")
     ap.add_argument("--batched-reps", type=int, default=5, help="timed batched launches; 0 skips the batched run")
     ap.add_argument("--shard-steps", type=int, default=5,
                     help="timed calls of the 1-of-8 sort-last shard leg; 0 skips it")
@@ -577,6 +588,14 @@ def main():
 
     from draw_amd import workloads
     from draw_amd.render import Renderer, frame_uniforms
+
+    if args.configs == 2:
+        log("shells 3840x2160, 20M triangles only ...")
+        print(json.dumps({"shells_20m_4k": bench_config(
+            SHELLS_WORKLOAD, workloads.shells_scene(3), steps=args.steps, warmup=args.warmup,
+            cpu_runs=1 if args.cpu else 0, cpu_warmup=0,
+            tiling=None if args.autotune else (args.raster_kind, args.tile, args.waves))}), flush=True)
+        return
 
     log("generating workload ...")
     scene = workloads.soup_scene(2, n_tri=args.n_tri)
@@ -679,9 +698,7 @@ def main():
             "2304-triangle torus (R 1.0, r 0.4, 48x24) at 640x480, 256x256 checker, one dh_render per frame",
             workloads.torus_scene(1), steps=200, warmup=20, cpu_runs=5 if args.cpu else 0, cpu_warmup=1)
         log("shells 3840x2160, 20M triangles ...")
-        out["shells_20m_4k"] = bench_config(
-            "20M micro-triangles on 8 nested jittered shells at 3840x2160 (5% flipped, 5% outside the frustum), "
-            "8192x8192 texture, one dh_render per frame", workloads.shells_scene(3), steps=20, warmup=3,
+        out["shells_20m_4k"] = bench_config(SHELLS_WORKLOAD, workloads.shells_scene(3), steps=20, warmup=3,
             cpu_runs=1 if args.cpu else 0, cpu_warmup=0)
     if args.cpu > 0:
         log("CPU baseline: one whole frame of the soup ...")

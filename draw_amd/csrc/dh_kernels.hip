@@ -88,19 +88,23 @@ __device__ inline void tri_positions(const DevDraw& dr, uint32_t lt, V3& a, V3& 
 }
 
 // Whether a wave reads its triangles' vertex records rather than the streams:
-// when the input triangles of its active lanes span more than eight per lane,
-// a stream load fetches several cache lines for few values each, and one
-// 128-byte record per triangle moves fewer bytes.  (At denser spans the
-// streams' coalesced loads are faster, although they may move more bytes.)
-// The choice only selects which copy of the same values is read.  Every lane
-// of the wave calls it.
-__device__ inline bool wave_sparse(bool active, uint32_t t) {
+// when the input triangles of its active lanes span more than `span`, a
+// stream load fetches several cache lines for few values each, and one
+// 128-byte record per triangle moves fewer bytes.  The 24 streams of a wave
+// spanning S triangles move about 1.5 S bytes per triangle, a record 128, but
+// a record costs each lane its own lines: kSparseSpan (eight triangles per
+// lane) by default, and kSparseSpanDense on frames of at least one input
+// triangle per pixel, whose micro-triangles' culling survivors interleave
+// finely (about 45% of them, a span near 145).  The choice only selects which
+// copy of the same values is read.  Every lane of the wave calls it.
+constexpr uint32_t kSparseSpan = 512u, kSparseSpanDense = 96u;
+__device__ inline bool wave_sparse(bool active, uint32_t t, uint32_t span) {
   const uint64_t m = __ballot(active);
   if (!m) return false;
   const int first = __builtin_ctzll(m), last = 63 - __builtin_clzll(m);
   const uint32_t t0 = (uint32_t)__builtin_amdgcn_readlane((int)t, first);
   const uint32_t t1 = (uint32_t)__builtin_amdgcn_readlane((int)t, last);
-  return t1 - t0 > 512u;
+  return t1 - t0 > span;
 }
 
 // Positions, normals and texture coordinates of the vertices of triangle lt:
@@ -937,11 +941,11 @@ __device__ __forceinline__ void geom_emit_clipped(uint32_t j, const DevDraw* __r
                                                   const DevMaterial* __restrict__ mats, SetupRec* __restrict__ setup,
                                                   ShadeRec* __restrict__ shade, uint32_t cap_tri, Status* __restrict__ status,
                                                   int TS, uint32_t* __restrict__ bincnt, uint8_t* __restrict__ wflag,
-                                                  uint32_t* __restrict__ dkey) {
+                                                  uint32_t* __restrict__ dkey, uint32_t span) {
   const uint2 sv = surv[status->n_simple + j];
   const uint32_t t = sv.x, base = sv.y;
   // the lanes of the wave still in the loop
-  const bool sparse = wave_sparse(true, t);
+  const bool sparse = wave_sparse(true, t, span);
   const DevDraw& dr = draws[wave_find_draw(draws, n_draws, t, true)];
   const FrameParamsPod& fr = views[dr.view].fr;
   const uint32_t lt = t - dr.tri_begin;
@@ -998,12 +1002,12 @@ __device__ __forceinline__ void geom_emit_simple(uint32_t j, bool active, uint4*
                                                  const uint2* __restrict__ surv, const DevMaterial* __restrict__ mats,
                                                  SetupRec* __restrict__ setup, ShadeRec* __restrict__ shade, uint32_t cap_tri,
                                                  Status* __restrict__ status, int TS, uint32_t* __restrict__ bincnt,
-                                                 uint8_t* __restrict__ wflag, uint32_t* __restrict__ dkey) {
+                                                 uint8_t* __restrict__ wflag, uint32_t* __restrict__ dkey, uint32_t span) {
   uint4 ro[6], so[9];
   uint32_t idx = 0;
   bool ok = false, skip = false;
   const uint2 sv = active ? surv[j] : make_uint2(0u, 0u);
-  const bool sparse = wave_sparse(active, sv.x);
+  const bool sparse = wave_sparse(active, sv.x, span);
   const uint32_t d = wave_find_draw(draws, n_draws, sv.x, active);
   if (active) {
     const uint32_t t = sv.x;
@@ -1059,19 +1063,19 @@ __global__ __launch_bounds__(256) void k_geom_emit(const DevDraw* __restrict__ d
                                                    const DevMaterial* __restrict__ mats, SetupRec* __restrict__ setup,
                                                    ShadeRec* __restrict__ shade, uint32_t cap_tri,
                                                    Status* __restrict__ status, int TS, uint32_t* __restrict__ bincnt,
-                                                   uint8_t* __restrict__ wflag, uint32_t* __restrict__ dkey) {
+                                                   uint8_t* __restrict__ wflag, uint32_t* __restrict__ dkey, uint32_t span) {
   if constexpr (CLIP) {
     const uint32_t n = status->n_clip;
     for (uint32_t j = blockIdx.x * 256u + threadIdx.x; j < n; j += gridDim.x * 256u)
       geom_emit_clipped<BIN>(j, draws, n_draws, views, surv, mats, setup, shade, cap_tri, status, TS, bincnt, wflag,
-                             dkey);
+                             dkey, span);
   } else {
     __shared__ uint4 s_stage[4][64 * 9];
     const uint32_t n = status->n_simple;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     for (uint32_t j0 = blockIdx.x * 256u + wave * 64u; j0 < n; j0 += gridDim.x * 256u)
       geom_emit_simple<BIN>(j0 + lane, j0 + lane < n, s_stage[wave], draws, n_draws, views, surv, mats, setup, shade,
-                            cap_tri, status, TS, bincnt, wflag, dkey);
+                            cap_tri, status, TS, bincnt, wflag, dkey, span);
   }
 }
 
@@ -1099,14 +1103,14 @@ __global__ __launch_bounds__(256) void k_geom_emit_both(const DevDraw* __restric
     const uint32_t n = status->n_clip;
     for (uint32_t j = c0 * 256u + threadIdx.x; j < n; j += g_clip * 256u)
       geom_emit_clipped<BIN>(j, draws, n_draws, views, surv, mats, setup, shade, cap_tri, status, TS, bincnt, wflag,
-                             dkey);
+                             dkey, kSparseSpan);
   } else {
     const uint32_t ub = (uint32_t)b - c0, g_simple = (uint32_t)G - g_clip;
     const uint32_t n = status->n_simple;
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     for (uint32_t j0 = ub * 256u + wave * 64u; j0 < n; j0 += g_simple * 256u)
       geom_emit_simple<BIN>(j0 + lane, j0 + lane < n, s_stage[wave], draws, n_draws, views, surv, mats, setup, shade,
-                            cap_tri, status, TS, bincnt, wflag, dkey);
+                            cap_tri, status, TS, bincnt, wflag, dkey, kSparseSpan);
   }
 }
 
@@ -2029,19 +2033,20 @@ PipeArgs enqueue_front(const PipeArgs& a, hipStream_t s) {
     // lengths are on the device): 2048 and 1024 blocks, a grid-stride loop
     // beyond that.
     const uint32_t g_simple = std::min<uint32_t>(n_blk, 2048u), g_clip = std::min<uint32_t>(n_blk, 1024u);
+    const uint32_t span = (uint64_t)a.n_tri_in >= (uint64_t)a.n_pixels ? kSparseSpanDense : kSparseSpan;
     const hipStream_t s2 = fork(s, a, 0);
     // (the clipped emit, the longer one per thread, starts first on the main
     // stream; the unclipped one waits for the fork on the side stream)
     if (a.ordered_bins) {
       hipLaunchKernelGGL((k_geom_emit<true, true>), dim3(g_clip), dim3(256), 0, s, a.draws, a.n_draws, a.views,
-                         a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, a.bincnt, wf, dk);
+                         a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, a.bincnt, wf, dk, span);
       hipLaunchKernelGGL((k_geom_emit<true, false>), dim3(g_simple), dim3(256), 0, s2, a.draws, a.n_draws, a.views,
-                         a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, a.bincnt, wf, dk);
+                         a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, a.bincnt, wf, dk, span);
     } else {
       hipLaunchKernelGGL((k_geom_emit<false, true>), dim3(g_clip), dim3(256), 0, s, a.draws, a.n_draws, a.views,
-                         a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, nullptr, nullptr, nullptr);
+                         a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, nullptr, nullptr, nullptr, span);
       hipLaunchKernelGGL((k_geom_emit<false, false>), dim3(g_simple), dim3(256), 0, s2, a.draws, a.n_draws, a.views,
-                         a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, nullptr, nullptr, nullptr);
+                         a.surv, a.mats, a.setup, a.shade, a.cap_tri, a.status, a.TS, nullptr, nullptr, nullptr, span);
     }
     join(s, a, 1);
   }
